@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Headline benchmark: grouped-strike LSM path-steps/s including delta/vega, on one MI355X.
+
+Workload (one step = one complete pricing request): one simulated state, 64 strikes
+linspace(80, 120), S0 100, r 0.03, q 0.02, sigma 0.25, T 1, 50 equal intervals (49 exercise
+nodes + expiry), 2^20 training + 2^20 evaluation paths, streams (11, 1) / (11, 2); FP32 bulk paths,
+payoff and basis with FP64 Gram/solve and price/SE reductions, pathwise delta + vega.
+Path-steps = intervals x (training + evaluation paths), the reference's transition unit.
+
+Also reported: the plain FP64 method's price time and its price + bump-and-reprice delta/vega
+time (1 + 4 pricings), the accuracy of every fast price and Greek in FP64 standard errors, the
+dominant kernel's roofline fraction (HIP events on the executor's stream) and the CPU checker's
+throughput on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "finoptions-engine_amd"))
+
+METRIC = "LSM path-steps/sec incl. delta/vega vs plain FP64, 1 GPU, |bias|<0.1 SE"
+HBM_PEAK_GBS = 8000.0
+INTERVALS = 50
+STRIKES = 64
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_request(fh, paths: int, *, spot: float = 100.0, vol: float = 0.25):
+    groups = fh.LsmStrikeGroups(
+        group_offsets=np.array([0, STRIKES], dtype=np.uint32),
+        initial_spots=np.array([spot]), strikes=np.linspace(80.0, 120.0, STRIKES),
+        maturities=np.array([1.0]), rates=np.array([0.03]), volatilities=np.array([vol]),
+        dividends=fh.ContinuousDividendYieldBatch(annualized_decimals=np.array([0.02])))
+    mesh = fh.UniformExerciseMesh(base_intervals=INTERVALS,
+                                  origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = fh.Lsm(training_paths=paths, evaluation_paths=paths,
+                    training_stream=fh.RandomStream(seed=11, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=11, domain=2),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.FAIL)
+    return groups, mesh, method
+
+
+def price(fh, executor, paths, greeks=(), **kw):
+    groups, mesh, method = build_request(fh, paths, **kw)
+    return fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=executor,
+                                           method=method, greeks=greeks)
+
+
+def algorithmic_bytes(kernel: str, paths: int, real: int) -> int | None:
+    """HBM bytes a launch must move at this workload (paths resident, no re-reads)."""
+    obs = INTERVALS
+    if kernel == "k_simulate":
+        return obs * paths * real                            # write every observation once
+    if kernel == "k_fit_prepare":
+        return 2 * paths * real + 2 * STRIKES * paths * real  # two spot rows, values read+write
+    if kernel in ("k_fit_gram", "k_fit_moments"):
+        return paths * real + STRIKES * paths * real         # one spot row, values read
+    if kernel == "k_fit_final":
+        return paths * real + STRIKES * paths * real
+    return None
+
+
+def cpu_baseline(sample_paths: int) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    req = pyoracle.GroupedRequest(
+        group_offsets=np.array([0, STRIKES], dtype=np.uint32), initial_spots=np.array([100.0]),
+        strikes=np.linspace(80.0, 120.0, STRIKES), maturities=np.array([1.0]),
+        rates=np.array([0.03]), volatilities=np.array([0.25]), dividend_yields=np.array([0.02]),
+        training_paths=sample_paths, evaluation_paths=sample_paths, training_seed=11,
+        evaluation_seed=11, training_domain_start=1, evaluation_domain_start=2, mesh_kind=0,
+        base_intervals=INTERVALS, origin_exercise=0)
+    start = time.perf_counter()
+    pyoracle.price_grouped(req, threads=1)
+    seconds = time.perf_counter() - start
+    steps = INTERVALS * 2 * sample_paths
+    return {"value": steps / seconds, "unit": "path-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"same workload at {sample_paths} + {sample_paths} paths (FP64 price only, "
+                       f"single thread, as the reference prices one group serially); "
+                       f"{seconds:.2f} s"),
+            "cpu": _cpu_model()}
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--paths", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 16)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU only: every box in this setup has a single MI355X and the "
+            "multi-GPU sharding of this workload is a design note, not a measured path. "
+            "Run with --gpus 1.")
+        return 2
+
+    import finoptions_hip as fh
+
+    paths = args.paths
+    fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
+    plain = fh.HipExecutor(device=0, precision=fh.HipPrecision.F64)
+    greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
+    log(f"device: {fast.device_name}; paths {paths}+{paths}, {STRIKES} strikes, "
+        f"{INTERVALS} intervals")
+
+    # ---- fast path: FP32 bulk + pathwise delta/vega
+    for i in range(args.warmup):
+        price(fh, fast, paths, greeks)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        fast_res = price(fh, fast, paths, greeks)   # synchronous: returns after the device drains
+        times.append(time.perf_counter() - t0)
+        log(f"fast step {i + 1}/{args.steps}: {times[-1] * 1e3:.2f} ms")
+    ms_fast = 1e3 * float(np.mean(times))
+    path_steps = INTERVALS * 2 * paths
+    value = path_steps / (ms_fast / 1e3)
+
+    # ---- dominant kernel, timed live with HIP events on the executor's stream
+    fast.profile(True)
+    price(fh, fast, paths, greeks)
+    stats = fast.kernel_stats()
+    fast.profile(False)
+    dominant = max(stats.items(), key=lambda kv: kv[1][1])
+    name, (launches, total_ms) = dominant
+    avg_ms = total_ms / launches
+    kernel_table = {k: {"launches": n, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / n, 2)}
+                    for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+    log("kernel times (one fast step): " + json.dumps(kernel_table))
+    hbm_kernels = [(k, v) for k, v in stats.items() if algorithmic_bytes(k, paths, 4)]
+    hbm_name, (hbm_n, hbm_ms) = max(hbm_kernels, key=lambda kv: kv[1][1])
+    hbm_avg_ms = hbm_ms / hbm_n
+    alg = algorithmic_bytes(hbm_name, paths, 4)
+    achieved = alg / (hbm_avg_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": hbm_name, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_us": round(hbm_avg_ms * 1e3, 2),
+                "algorithmic_bytes_per_launch": alg,
+                "dominant_kernel": name, "dominant_avg_launch_us": round(avg_ms * 1e3, 2),
+                "dominant_share": round(total_ms / sum(v[1] for v in stats.values()), 3)}
+
+    # ---- plain FP64: price and price + bump-and-reprice delta/vega (1 + 4 pricings)
+    plain_info = None
+    accuracy = None
+    if not args.skip_plain:
+        t0 = time.perf_counter()
+        base = price(fh, plain, paths)
+        t_price = time.perf_counter() - t0
+        h_s, h_v = 1e-4 * 100.0, 1e-4 * 0.25
+        t0 = time.perf_counter()
+        up_s = price(fh, plain, paths, spot=100.0 + h_s)
+        dn_s = price(fh, plain, paths, spot=100.0 - h_s)
+        up_v = price(fh, plain, paths, vol=0.25 + h_v)
+        dn_v = price(fh, plain, paths, vol=0.25 - h_v)
+        t_bumps = time.perf_counter() - t0
+        ms_plain = 1e3 * (t_price + t_bumps)
+        fd_delta = (up_s.prices - dn_s.prices) / (2 * h_s)
+        fd_vega = (up_v.prices - dn_v.prices) / (2 * h_v)
+        log(f"plain FP64: price {t_price * 1e3:.1f} ms, price+4 bumps {ms_plain:.1f} ms")
+        ref = price(fh, plain, paths, greeks)   # FP64 pathwise Greeks: accuracy reference
+        bias_price = np.abs(fast_res.prices - base.prices) / base.standard_errors
+        bias_delta = (np.abs(fast_res.greeks[fh.Greek.DELTA] - ref.greeks[fh.Greek.DELTA])
+                      / ref.greek_standard_errors[fh.Greek.DELTA])
+        bias_vega = (np.abs(fast_res.greeks[fh.Greek.VEGA] - ref.greeks[fh.Greek.VEGA])
+                     / ref.greek_standard_errors[fh.Greek.VEGA])
+        accuracy = {
+            "max_price_bias_over_se": float(bias_price.max()),
+            "max_delta_bias_over_se": float(bias_delta.max()),
+            "max_vega_bias_over_se": float(bias_vega.max()),
+            "all_within_0.1_se": bool(max(bias_price.max(), bias_delta.max(),
+                                          bias_vega.max()) < 0.1),
+            "reference": "plain FP64 GPU method (matches the CPU checker to 1e-10 relative)",
+            "fd_vs_pathwise_delta_max_abs": float(np.max(np.abs(fd_delta
+                                                                - ref.greeks[fh.Greek.DELTA]))),
+            "fd_vs_pathwise_vega_max_abs": float(np.max(np.abs(fd_vega
+                                                               - ref.greeks[fh.Greek.VEGA]))),
+        }
+        plain_info = {"ms_price": round(t_price * 1e3, 2),
+                      "ms_price_bump_greeks": round(ms_plain, 2),
+                      "fast_speedup_vs_plain_price_bump_greeks": round(ms_plain / ms_fast, 2),
+                      "plain_path_steps_per_s_price_bump_greeks": path_steps * 5 / (ms_plain / 1e3)}
+
+    cpu = None
+    if not args.skip_cpu:
+        log("timing the CPU checker on a bounded sample ...")
+        cpu = cpu_baseline(args.cpu_sample_paths)
+        cpu["fast_speedup_vs_cpu"] = round(value / cpu["value"], 1)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_fast,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"grouped-strike LSM call: 1 state x {STRIKES} strikes linspace(80,120), "
+                         f"S0 100, r 0.03, q 0.02, sigma 0.25, T 1, {INTERVALS} intervals, "
+                         f"{paths}+{paths} paths, Philox seed 11 domains 1/2"),
+            "method": "FP32 bulk paths/payoff/basis, FP64 Gram + LDL^T solve + price/SE",
+            "greeks": "pathwise delta+vega, frozen policy, paths regenerated from Philox",
+            "path_steps_per_step": path_steps,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64": plain_info,
+        "accuracy": accuracy,
+        "kernels": kernel_table,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

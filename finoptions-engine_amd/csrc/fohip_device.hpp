@@ -1,0 +1,296 @@
+// Device building blocks of the LSM engine for gfx950 (CDNA4, wave64).
+//
+// Random stream, normal transform and GBM transition restate the reference kernels bit for bit
+// where the arithmetic allows it (integer Philox, uniform bit maps, table lookups, FP64 products);
+// transcendental calls use the device libm (exp/log), whose results can differ from glibc by an
+// ulp -- parity tests count those differences instead of assuming them away.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fohip_layout.hpp"
+
+namespace fohip {
+
+// ------------------------------------------------------------------ Philox4x32-10 (philox.rs)
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+// Random123 Philox4x32 with ten rounds (philox.rs:18-41). Round output lanes are
+// [hi(M1*c2)^c1^k0, lo(M1*c2), hi(M0*c0)^c3^k1, lo(M0*c0)]; the key is bumped before rounds 2..10.
+__device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                               uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round) {
+            k0 += kPhiloxW0;
+            k1 += kPhiloxW1;
+        }
+        const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        c1 = static_cast<uint32_t>(p1);
+        c3 = static_cast<uint32_t>(p0);
+        c0 = n0;
+        c2 = n2;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// One path-local stream: counter [block_lo, block_hi, path, domain], key [seed_lo, seed_hi]
+// (philox.rs:64-113). A u64 draw is word i | word i+1 << 32 and never straddles a block, so each
+// block yields exactly two draws; the second is kept for the next call.
+struct PathStream {
+    uint32_t k0, k1, path, domain;
+    uint32_t block_lo, block_hi;
+    uint64_t spare;
+    bool has_spare;
+
+    __device__ __forceinline__ void init(uint64_t seed, uint32_t domain_id, uint32_t path_id) {
+        k0 = static_cast<uint32_t>(seed);
+        k1 = static_cast<uint32_t>(seed >> 32);
+        path = path_id;
+        domain = domain_id;
+        block_lo = 0;
+        block_hi = 0;
+        spare = 0;
+        has_spare = false;
+    }
+
+    __device__ __forceinline__ uint64_t next_u64() {
+        if (has_spare) {
+            has_spare = false;
+            return spare;
+        }
+        const uint4 w = philox4x32_10(block_lo, block_hi, path, domain, k0, k1);
+        block_lo += 1u;
+        block_hi += block_lo == 0u ? 1u : 0u;
+        spare = static_cast<uint64_t>(w.z) | (static_cast<uint64_t>(w.w) << 32);
+        has_spare = true;
+        return static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32);
+    }
+};
+
+// ------------------------------------------------------------ ZIGNOR normal (normal.rs:33-109)
+
+constexpr uint64_t kFractionMask = (uint64_t(1) << 52) - 1;
+constexpr uint64_t kExponent0 = uint64_t(1023) << 52;
+constexpr uint64_t kExponent1 = uint64_t(1024) << 52;
+constexpr int kMaxRejectionIterations = 1048576;
+constexpr double kEpsilon = 2.220446049250313e-16;
+
+__device__ __forceinline__ double symmetric_unit(uint64_t bits) {
+    return __longlong_as_double(static_cast<long long>(((bits >> 12) & kFractionMask) | kExponent1)) - 3.0;
+}
+__device__ __forceinline__ double open_unit(uint64_t bits) {
+    return __longlong_as_double(static_cast<long long>(((bits >> 12) & kFractionMask) | kExponent0)) -
+           (1.0 - kEpsilon / 2.0);
+}
+__device__ __forceinline__ double standard_unit(uint64_t bits) {
+    return static_cast<double>(bits >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Rejected draws: wedge test, redraw, or tail (normal.rs:50-91). Returns false at the bounded
+// rejection limit. `draws` counts consumed u64 draws (parity probe only).
+__device__ __noinline__ bool zignor_rejected(PathStream &s, int index, double unit, double value,
+                                             const double *__restrict__ X,
+                                             const double *__restrict__ F, double zr, double &out,
+                                             uint32_t &draws) {
+    for (int it = 0; it < kMaxRejectionIterations; ++it) {
+        if (index == 0) {
+            for (int tail = 0; tail < kMaxRejectionIterations; ++tail) {
+                const double x = log(open_unit(s.next_u64())) / zr;
+                const double y = log(open_unit(s.next_u64()));
+                draws += 2;
+                if (-2.0 * y >= x * x) {
+                    out = unit < 0.0 ? x - zr : zr - x;
+                    return true;
+                }
+            }
+            return false;
+        }
+        const double ordinate = F[index + 1] + (F[index] - F[index + 1]) * standard_unit(s.next_u64());
+        draws += 1;
+        if (ordinate < exp(-0.5 * value * value)) {
+            out = value;
+            return true;
+        }
+        const uint64_t bits = s.next_u64();
+        draws += 1;
+        index = static_cast<int>(bits & 0xff);
+        unit = symmetric_unit(bits);
+        value = unit * X[index];
+        if (fabs(value) < X[index + 1]) {
+            out = value;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool zignor_normal(PathStream &s, const double *__restrict__ X,
+                                              const double *__restrict__ F, double zr, double &out,
+                                              uint32_t &draws) {
+    const uint64_t bits = s.next_u64();
+    draws += 1;
+    const int index = static_cast<int>(bits & 0xff);
+    const double unit = symmetric_unit(bits);
+    const double value = unit * X[index];
+    if (fabs(value) < X[index + 1]) {
+        out = value;
+        return true;
+    }
+    return zignor_rejected(s, index, unit, value, X, F, zr, out, draws);
+}
+
+// ------------------------------------------------------- GBM transition and cash events
+
+// step.rs:89-95: zero is absorbing; S' = S * exp(fma(diffusion, Z, drift)).
+__device__ __forceinline__ double gbm_advance(double spot, double diffusion, double drift, double z,
+                                              double &growth) {
+    growth = exp(fma(diffusion, z, drift));
+    return spot == 0.0 ? 0.0 : spot * growth;
+}
+__device__ __forceinline__ float gbm_advance(float spot, float diffusion, float drift, float z,
+                                             float &growth) {
+    growth = expf(fmaf(diffusion, z, drift));
+    return spot == 0.0f ? 0.0f : spot * growth;
+}
+
+// dividend.rs:59-81: liquidator max(S - D, 0); survivor S > D ? S - D : S.
+template <typename Real>
+__device__ __forceinline__ Real apply_cash(int policy, Real amount, Real spot) {
+    if (policy == 0) {
+        const Real after = spot - amount;
+        return after > Real(0) ? after : Real(0);
+    }
+    return spot > amount ? spot - amount : spot;
+}
+
+// ------------------------------------------------------------------ regression prediction
+
+// regression.rs:197-224: z = (x - mean) / scale; p_{k+1} = fma(z - alpha_k, p_k, -beta_k p_{k-1});
+// prediction = fma(coef_{k+1}, p_{k+1}, prediction).
+__device__ __forceinline__ double predict(const Model &m, int degree, double x) {
+    const double z = degree == 0 ? 0.0 : (x - m.mean) / m.scale;
+    double prediction = m.coef[0], previous = 0.0, current = 1.0;
+    for (int k = 0; k < degree; ++k) {
+        const double next = fma(z - m.alpha[k], current, -m.beta[k] * previous);
+        prediction = fma(m.coef[k + 1], next, prediction);
+        previous = current;
+        current = next;
+    }
+    return prediction;
+}
+__device__ __forceinline__ float predict(const Model &m, int degree, float x) {
+    const float z = degree == 0 ? 0.0f : (x - static_cast<float>(m.mean)) / static_cast<float>(m.scale);
+    float prediction = static_cast<float>(m.coef[0]), previous = 0.0f, current = 1.0f;
+    for (int k = 0; k < degree; ++k) {
+        const float next =
+            fmaf(z - static_cast<float>(m.alpha[k]), current, -static_cast<float>(m.beta[k]) * previous);
+        prediction = fmaf(static_cast<float>(m.coef[k + 1]), next, prediction);
+        previous = current;
+        current = next;
+    }
+    return prediction;
+}
+
+__device__ __forceinline__ bool should_exercise(int tie, double intrinsic, double continuation) {
+    return tie == 0 ? intrinsic > continuation : intrinsic >= continuation;
+}
+
+// ------------------------------------------------------- deterministic block reductions
+
+constexpr int kBlock = 256;      // threads per block
+constexpr int kWaves = kBlock / 64;
+
+// Down-sweep wave tree; the total is valid in lane 0. The combination order depends only on lane
+// indices, so results are reproducible bit for bit.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
+    return v;
+}
+
+// Block sum of N values per thread; the result is returned in thread 0 (fixed order
+// ((w0 + w1) + (w2 + w3))). `scratch` holds kWaves * N doubles.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double *scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) scratch[wave * N + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            v[i] = (scratch[0 * N + i] + scratch[1 * N + i]) + (scratch[2 * N + i] + scratch[3 * N + i]);
+    }
+    __syncthreads();
+}
+
+// Chan et al. pairwise merge of (count, mean, M2) with a formula symmetric in its operands.
+struct Moments {
+    double n, mean, m2;
+};
+__device__ __forceinline__ Moments chan_merge(Moments a, Moments b) {
+    const double n = a.n + b.n;
+    if (n == 0.0) return {0.0, 0.0, 0.0};
+    if (a.n == 0.0) return b;
+    if (b.n == 0.0) return a;
+    const double delta = b.mean - a.mean;
+    Moments r;
+    r.n = n;
+    r.mean = (a.n * a.mean + b.n * b.mean) / n;
+    r.m2 = (a.m2 + b.m2) + delta * delta * (a.n * b.n / n);
+    return r;
+}
+
+__device__ __forceinline__ Moments wave_chan(Moments m) {
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        Moments o;
+        o.n = __shfl_down(m.n, offset, 64);
+        o.mean = __shfl_down(m.mean, offset, 64);
+        o.m2 = __shfl_down(m.m2, offset, 64);
+        m = chan_merge(m, o);
+    }
+    return m;
+}
+
+// Block Chan merge; result valid in thread 0. `scratch` holds 3 * kWaves doubles.
+__device__ __forceinline__ Moments block_chan(Moments m, double *scratch) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    m = wave_chan(m);
+    if (lane == 0) {
+        scratch[wave * 3 + 0] = m.n;
+        scratch[wave * 3 + 1] = m.mean;
+        scratch[wave * 3 + 2] = m.m2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Moments w[kWaves];
+        for (int i = 0; i < kWaves; ++i) w[i] = {scratch[i * 3], scratch[i * 3 + 1], scratch[i * 3 + 2]};
+        m = chan_merge(chan_merge(w[0], w[1]), chan_merge(w[2], w[3]));
+    }
+    __syncthreads();
+    return m;
+}
+
+// Per-thread Welford update of regression.rs:288-293.
+__device__ __forceinline__ void welford_push(Moments &m, double x) {
+    m.n += 1.0;
+    const double delta = x - m.mean;
+    m.mean += delta / m.n;
+    m.m2 = fma(delta, x - m.mean, m.m2);
+}
+
+}  // namespace fohip

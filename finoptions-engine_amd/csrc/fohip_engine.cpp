@@ -1,0 +1,1669 @@
+// Host side of libfohip: request validation, event-grid and curve construction, HBM layout,
+// launch orchestration and the C ABI declared in include/fohip.h.
+//
+// The pricing pipeline per batch of groups is:
+//   training simulation -> for each backward step: prepare, support/scale reduce,
+//   (d + 1) x [moments, Stieltjes reduce] (FP64) or [Gram, LDL^T reduce] (FP32 bulk)
+//   -> final discount + origin decision -> fused evaluation -> per-row summary.
+// Everything is enqueued on the executor's stream; the host synchronizes once per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fohip.h"
+#include "fohip_kernels.hpp"
+#include "fohip_layout.hpp"
+
+using namespace fohip;
+
+// ================================================================================ errors
+
+namespace {
+
+int fail(fohip_error *err, int code, int phase, const char *fmt, ...) {
+    if (err) {
+        err->code = code;
+        err->phase = phase;
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(err->message, sizeof err->message, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+void clear_error(fohip_error *err) {
+    if (!err) return;
+    std::memset(err, 0, sizeof *err);
+    err->row = err->node = err->path = err->index = -1;
+}
+
+void set_fields(fohip_error *err, int64_t row, int64_t node, int64_t path, int64_t index,
+                double observed, double limit) {
+    if (!err) return;
+    err->row = row;
+    err->node = node;
+    err->path = path;
+    err->index = index;
+    err->observed = observed;
+    err->limit = limit;
+}
+
+bool canonical_zero_or_positive(double v) {
+    return std::isfinite(v) && v >= 0.0 && !(v == 0.0 && std::signbit(v));
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(err, _e == hipErrorOutOfMemory ? FOHIP_E_OUT_OF_MEMORY : FOHIP_E_HIP,  \
+                        FOHIP_PHASE_NONE, "HIP call %s failed: %s", #expr,                     \
+                        hipGetErrorString(_e));                                                \
+    } while (0)
+
+}  // namespace
+
+// ============================================================================ validation
+
+extern "C" int fohip_validate_time_grid(const double *times, size_t nodes, fohip_error *err) {
+    clear_error(err);
+    // time.rs:31-64
+    if (nodes == 0 || !times) return fail(err, FOHIP_E_INVALID_TIME_GRID, 0, "time grid must contain at least the origin");
+    if (!std::isfinite(times[0]))
+        return fail(err, FOHIP_E_INVALID_TIME_GRID, 0, "time at node 0 must be finite; observed %.17g", times[0]);
+    if (times[0] != 0.0 || std::signbit(times[0]))
+        return fail(err, FOHIP_E_INVALID_TIME_GRID, 0, "time grid must start at exact canonical 0.0; observed %.17g",
+                    times[0]);
+    if (nodes - 1 > 0xFFFFFFFFull) return fail(err, FOHIP_E_INVALID_TIME_GRID, 0, "time grid has too many nodes");
+    for (size_t i = 1; i < nodes; ++i) {
+        if (!std::isfinite(times[i])) {
+            set_fields(err, -1, (int64_t)i, -1, -1, times[i], 0);
+            return fail(err, FOHIP_E_INVALID_TIME_GRID, 0, "time at node %zu must be finite; observed %.17g", i, times[i]);
+        }
+        if (times[i] <= times[i - 1]) {
+            set_fields(err, -1, (int64_t)i, -1, -1, times[i], times[i - 1]);
+            return fail(err, FOHIP_E_INVALID_TIME_GRID, 0,
+                        "time grid must be strictly increasing; node %zu = %.17g follows node %zu = %.17g", i,
+                        times[i], i - 1, times[i - 1]);
+        }
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_log_discount_curve(const double *times, const double *logd, size_t nodes,
+                                                 fohip_error *err) {
+    int rc = fohip_validate_time_grid(times, nodes, err);
+    if (rc) return rc;
+    // curve.rs:24-62
+    if (logd[0] != 0.0 || std::signbit(logd[0]))
+        return fail(err, FOHIP_E_INVALID_LOG_DISCOUNT_CURVE, 0,
+                    "log discount at the origin must be exact canonical 0.0; observed %.17g", logd[0]);
+    for (size_t i = 0; i < nodes; ++i) {
+        if (!std::isfinite(logd[i])) {
+            set_fields(err, -1, (int64_t)i, -1, -1, logd[i], 0);
+            return fail(err, FOHIP_E_INVALID_LOG_DISCOUNT_CURVE, 0, "log discount at node %zu must be finite; observed %.17g",
+                        i, logd[i]);
+        }
+        if (i == 0) continue;
+        const double interval = logd[i] - logd[i - 1];
+        const double d = std::exp(interval);
+        if (!std::isfinite(d) || d <= 0.0)
+            return fail(err, FOHIP_E_INVALID_LOG_DISCOUNT_CURVE, 0,
+                        "discount from node %zu to %zu is not representable; interval log discount %.17g", i - 1, i,
+                        interval);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_cumulative_yield_curve(const double *times, const double *q, size_t nodes,
+                                                     fohip_error *err) {
+    int rc = fohip_validate_time_grid(times, nodes, err);
+    if (rc) return rc;
+    // yield_curve.rs:28-80
+    if (q[0] != 0.0 || std::signbit(q[0]))
+        return fail(err, FOHIP_E_INVALID_YIELD_CURVE, 0,
+                    "cumulative dividend yield at the origin must be exact canonical 0.0; observed %.17g", q[0]);
+    for (size_t i = 0; i < nodes; ++i) {
+        if (!canonical_zero_or_positive(q[i]))
+            return fail(err, FOHIP_E_INVALID_YIELD_CURVE, 0,
+                        "cumulative dividend yield at node %zu must be finite and nonnegative; observed %.17g", i, q[i]);
+        if (i == 0) continue;
+        if (q[i] < q[i - 1])
+            return fail(err, FOHIP_E_INVALID_YIELD_CURVE, 0,
+                        "cumulative dividend yield decreases from node %zu (%.17g) to node %zu (%.17g)", i - 1, q[i - 1],
+                        i, q[i]);
+        const double payout = std::exp(-(q[i] - q[i - 1]));
+        if (!std::isfinite(payout) || payout <= 0.0)
+            return fail(err, FOHIP_E_INVALID_YIELD_CURVE, 0,
+                        "dividend-yield payout discount from node %zu to %zu is not representable", i - 1, i);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_cash_dividend_amount(double amount, fohip_error *err) {
+    clear_error(err);
+    if (!canonical_zero_or_positive(amount))
+        return fail(err, FOHIP_E_INVALID_CASH_DIVIDEND, 0,
+                    "cash dividend amount must be finite, nonnegative and canonical; observed %.17g", amount);
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_constant_volatility(double v, fohip_error *err) {
+    clear_error(err);
+    if (!std::isfinite(v) || v <= 0.0)
+        return fail(err, FOHIP_E_INVALID_VOLATILITY, 0,
+                    "constant volatility must be finite and strictly positive; observed %.17g", v);
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_event_schedule(const double *times, size_t nodes, const uint32_t *dnodes,
+                                             const double *damounts, size_t dividends, const uint32_t *enodes,
+                                             size_t exercise_count, uint32_t expiry, fohip_error *err) {
+    int rc = fohip_validate_time_grid(times, nodes, err);
+    if (rc) return rc;
+    // event.rs:42-112
+    const uint64_t expected = nodes - 1;
+    if (expected == 0 || expiry != expected)
+        return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "expiry must be the final node %llu; observed %u",
+                    (unsigned long long)expected, expiry);
+    for (size_t i = 0; i < dividends; ++i) {
+        if (dnodes[i] == 0 || dnodes[i] > expiry)
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "cash dividend node %u must lie in 1..=%u", dnodes[i], expiry);
+        if (i > 0 && dnodes[i - 1] == dnodes[i])
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "duplicate cash dividend node %u", dnodes[i]);
+        if (i > 0 && dnodes[i - 1] > dnodes[i])
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "cash dividend nodes are unordered: %u then %u", dnodes[i - 1],
+                        dnodes[i]);
+        rc = fohip_validate_cash_dividend_amount(damounts[i], err);
+        if (rc) return rc;
+    }
+    for (size_t i = 0; i < exercise_count; ++i) {
+        if (enodes[i] >= expiry)
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "exercise node %u must precede expiry %u", enodes[i], expiry);
+        if (i > 0 && enodes[i - 1] == enodes[i])
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "duplicate exercise node %u", enodes[i]);
+        if (i > 0 && enodes[i - 1] > enodes[i])
+            return fail(err, FOHIP_E_INVALID_SCHEDULE, 0, "exercise nodes are unordered: %u then %u", enodes[i - 1],
+                        enodes[i]);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_ragged_offsets(const int64_t *offsets, size_t len, size_t values_len, fohip_error *err) {
+    clear_error(err);
+    // batch/src/view.rs:99-160: at least two offsets, nonnegative and nondecreasing; offsets may
+    // start above zero (a sliced Arrow list) and then address child[offset - offsets[0]].
+    if (len < 2 || !offsets) return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "ragged offsets need at least two entries");
+    for (size_t i = 0; i < len; ++i) {
+        if (offsets[i] < 0)
+            return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "ragged offset %zu is negative: %lld", i, (long long)offsets[i]);
+        if (i > 0 && offsets[i] < offsets[i - 1])
+            return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "ragged offsets decrease at %zu", i);
+    }
+    if (static_cast<uint64_t>(offsets[len - 1] - offsets[0]) != values_len)
+        return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "ragged offsets span %lld child values but %zu are present",
+                    (long long)(offsets[len - 1] - offsets[0]), values_len);
+    return FOHIP_OK;
+}
+
+// ============================================================================== executor
+
+namespace {
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace
+
+struct fohip_executor {
+    int32_t device = 0;
+    int32_t precision = 0;
+    hipStream_t stream = nullptr;
+    char name[256] = {0};
+    fohip_tuning tuning = {0, 0};
+    size_t batch_budget = 0;   // bytes for paths + values per batch
+    std::mutex mu;
+    DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, items_e,
+        rows, eval_out, summary, replay, probe_a, probe_b, probe_c;
+    // host copies from the last grouped call, kept for fohip_grouped_diagnostics
+    std::vector<Model> last_models;
+    std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
+    std::vector<uint32_t> last_row_nodes;   // per group: observation nodes concatenated
+    std::vector<uint32_t> last_row_base;
+    // kernel timing
+    bool profiling = false;
+    struct Pending {
+        const char *name;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<fohip_kernel_stat> stats;
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void collect() {
+        for (auto &p : pending) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+                fohip_kernel_stat *st = nullptr;
+                for (auto &s : stats)
+                    if (std::strncmp(s.name, p.name, sizeof s.name) == 0) st = &s;
+                if (!st) {
+                    fohip_kernel_stat ns;
+                    std::memset(&ns, 0, sizeof ns);
+                    std::snprintf(ns.name, sizeof ns.name, "%s", p.name);
+                    stats.push_back(ns);
+                    st = &stats.back();
+                }
+                st->launches += 1;
+                st->total_ms += ms;
+            }
+            event_pool.push_back(p.start);
+            event_pool.push_back(p.stop);
+        }
+        pending.clear();
+    }
+};
+
+namespace {
+// Brackets one launch with events on the executor's stream when profiling is enabled.
+struct ProfScope {
+    fohip_executor *ex;
+    const char *name;
+    hipEvent_t start = nullptr;
+    ProfScope(fohip_executor *e, const char *n) : ex(e), name(n) {
+        if (ex->profiling) {
+            start = ex->take_event();
+            if (start) (void)hipEventRecord(start, ex->stream);
+        }
+    }
+    ~ProfScope() {
+        if (ex->profiling && start) {
+            hipEvent_t stop = ex->take_event();
+            if (stop) {
+                (void)hipEventRecord(stop, ex->stream);
+                ex->pending.push_back({name, start, stop});
+            }
+        }
+    }
+};
+}  // namespace
+
+#define PROF_LAUNCH(name, expr)          \
+    do {                                 \
+        ProfScope _scope(ex, name);      \
+        HIP_TRY(expr);                   \
+    } while (0)
+
+extern "C" int fohip_abi_version(void) { return FOHIP_ABI_VERSION; }
+
+extern "C" int fohip_device_count(int32_t *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_executor_create(int32_t device, int32_t precision, fohip_executor **out, fohip_error *err) {
+    clear_error(err);
+    if (!out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "executor output pointer is null");
+    *out = nullptr;
+    if (precision != FOHIP_PRECISION_F64 && precision != FOHIP_PRECISION_F32_BULK)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "precision must be FP64 (0) or FP32 bulk (1); observed %d",
+                    precision);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(err, FOHIP_E_NO_DEVICE, 0, "no HIP device is available; the HIP executor has no fallback");
+    if (device < 0 || device >= n)
+        return fail(err, FOHIP_E_NO_DEVICE, 0, "HIP device %d does not exist (%d visible)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    auto *ex = new (std::nothrow) fohip_executor();
+    if (!ex) return fail(err, FOHIP_E_OUT_OF_MEMORY, 0, "executor allocation failed");
+    ex->device = device;
+    ex->precision = precision;
+    std::snprintf(ex->name, sizeof ex->name, "%s (%s)", prop.name, prop.gcnArchName);
+    // Paths + values of one batch of groups stay below a quarter of device memory.
+    ex->batch_budget = std::max<size_t>(prop.totalGlobalMem / 4, size_t(1) << 30);
+    e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ex;
+        return fail(err, FOHIP_E_HIP, 0, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = ex;
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_executor_destroy(fohip_executor *ex) {
+    if (!ex) return FOHIP_OK;
+    (void)hipSetDevice(ex->device);
+    (void)hipStreamSynchronize(ex->stream);
+    for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
+                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t, &ex->items_e, &ex->rows,
+                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c})
+        b->release();
+    for (auto &p : ex->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t e : ex->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ex->stream);
+    delete ex;
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_executor_info(const fohip_executor *ex, int32_t *device, int32_t *precision, char *name,
+                                   size_t cap) {
+    if (!ex) return FOHIP_E_INVALID_ARGUMENT;
+    if (device) *device = ex->device;
+    if (precision) *precision = ex->precision;
+    if (name && cap) std::snprintf(name, cap, "%s", ex->name);
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_set_profiling(fohip_executor *ex, int32_t enabled) {
+    if (!ex) return FOHIP_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ex->mu);
+    ex->profiling = enabled != 0;
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_get_profile(fohip_executor *ex, fohip_kernel_stat *out, size_t cap, size_t *count) {
+    if (!ex) return FOHIP_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ex->mu);
+    if (count) *count = ex->stats.size();
+    for (size_t i = 0; i < ex->stats.size() && i < cap && out; ++i) out[i] = ex->stats[i];
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_reset_profile(fohip_executor *ex) {
+    if (!ex) return FOHIP_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ex->mu);
+    ex->stats.clear();
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_set_tuning(fohip_executor *ex, const fohip_tuning *t) {
+    if (!ex || !t) return FOHIP_E_INVALID_ARGUMENT;
+    if (t->eval_strike_tile > 16) return FOHIP_E_INVALID_ARGUMENT;
+    ex->tuning = *t;
+    return FOHIP_OK;
+}
+
+// ============================================================================ group plans
+
+namespace {
+
+struct GroupPlan {
+    std::vector<double> times, logd, q;
+    std::vector<double> cash;          // per node (0 when absent)
+    std::vector<uint8_t> has_cash;     // per node
+    std::vector<uint32_t> obs_nodes;   // exercise nodes + expiry
+    double S0 = 0.0, sigma = 0.0;
+    int32_t policy = 0;
+    uint32_t strike_begin = 0, strike_end = 0;
+    uint64_t nt = 0, ne = 0;
+    uint64_t train_seed = 0, eval_seed = 0;
+    uint32_t train_domain = 0, eval_domain = 0;
+};
+
+struct RunSpec {
+    FitParams fp;
+    uint32_t greeks = 0;
+    bool f32 = false;
+};
+
+struct RowOut {
+    double price = 0, sd = 0, se = 0;
+    bool has_sd = false;
+    RowSummary sum{};
+    double greek[4] = {0, 0, 0, 0}, greek_se[4] = {0, 0, 0, 0};
+};
+
+struct GroupTables {
+    std::vector<DevGroup> groups;
+    std::vector<double> log_drift, diffusion, dt, sqrt_dt, cash, disc0, disc_step, row_time;
+    std::vector<float> log_drift32, diffusion32;
+    std::vector<uint32_t> node_flags, row_node;
+};
+
+// Host-side discount checks (policy.rs:1081-1098): every factor used by the fit and evaluation
+// must be finite and positive. Returns the offending (from, to, log discount) or false.
+bool check_discount(double ld, double &d) {
+    d = std::exp(ld);
+    return std::isfinite(d) && d > 0.0;
+}
+
+}  // namespace
+
+// Builds step/node/row tables of a list of groups and their HBM offsets.
+static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<double> &strikes, bool f32,
+                        GroupTables &gt, uint64_t &path_elems, uint64_t &value_elems, uint64_t &model_count,
+                        uint64_t &part_t_slots, uint64_t &part_e_slots, uint64_t &count_slots, fohip_error *err,
+                        int64_t &bad_group, int64_t &bad_node) {
+    (void)strikes;
+    (void)f32;
+    bad_group = -1;
+    bad_node = -1;
+    path_elems = value_elems = model_count = part_t_slots = part_e_slots = count_slots = 0;
+    gt = GroupTables();
+    gt.groups.reserve(plans.size());
+    for (size_t gi = 0; gi < plans.size(); ++gi) {
+        const GroupPlan &p = plans[gi];
+        DevGroup g;
+        std::memset(&g, 0, sizeof g);
+        const uint32_t nodes = static_cast<uint32_t>(p.times.size());
+        g.S0 = p.S0;
+        g.sigma = p.sigma;
+        g.train_seed = p.train_seed;
+        g.eval_seed = p.eval_seed;
+        g.train_domain = p.train_domain;
+        g.eval_domain = p.eval_domain;
+        g.nt = static_cast<uint32_t>(p.nt);
+        g.ne = static_cast<uint32_t>(p.ne);
+        g.intervals = nodes - 1;
+        g.obs = static_cast<uint32_t>(p.obs_nodes.size());
+        g.origin_row = p.obs_nodes[0] == 0 ? 1u : 0u;
+        g.fit_rows = g.obs - 1 - g.origin_row;
+        g.step_base = static_cast<uint32_t>(gt.log_drift.size());
+        g.node_base = static_cast<uint32_t>(gt.cash.size());
+        g.row_base = static_cast<uint32_t>(gt.disc0.size());
+        g.strike_begin = p.strike_begin;
+        g.strike_end = p.strike_end;
+        g.policy = p.policy;
+        const uint32_t ns = p.strike_end - p.strike_begin;
+        g.path_base = path_elems;
+        path_elems += static_cast<uint64_t>(g.obs) * g.nt;
+        g.value_base = value_elems;
+        value_elems += static_cast<uint64_t>(ns) * g.nt;
+        g.model_base = static_cast<uint32_t>(model_count);
+        model_count += static_cast<uint64_t>(ns) * g.obs;
+        g.count_base = static_cast<uint32_t>(count_slots);
+        count_slots += static_cast<uint64_t>(ns) * g.obs;
+        g.chunks_t = static_cast<uint32_t>((p.nt + kChunk - 1) / kChunk);
+        g.chunks_e = static_cast<uint32_t>((p.ne + kChunk - 1) / kChunk);
+        g.part_base_t = static_cast<uint32_t>(part_t_slots);
+        part_t_slots += static_cast<uint64_t>(ns) * g.chunks_t;
+        g.part_base_e = static_cast<uint32_t>(part_e_slots);
+        part_e_slots += static_cast<uint64_t>(ns) * g.chunks_e;
+        // step.rs:27-57 transition coefficients
+        const double sigma = p.sigma;
+        for (uint32_t to = 1; to < nodes; ++to) {
+            const uint32_t from = to - 1;
+            const double dt = p.times[to] - p.times[from];
+            const double interval_log_discount = p.logd[to] - p.logd[from];
+            const double interval_yield = p.q[to] - p.q[from];
+            const double variance = std::fma(sigma, sigma, 0.0) * dt;
+            const double log_drift = -interval_log_discount - interval_yield - 0.5 * variance;
+            gt.log_drift.push_back(log_drift);
+            gt.diffusion.push_back(std::sqrt(variance));
+            gt.log_drift32.push_back(static_cast<float>(log_drift));
+            gt.diffusion32.push_back(static_cast<float>(std::sqrt(variance)));
+            gt.dt.push_back(dt);
+            gt.sqrt_dt.push_back(std::sqrt(dt));
+        }
+        const uint32_t terminal = p.obs_nodes.back();
+        size_t oi = 0;
+        for (uint32_t n = 0; n < nodes; ++n) {
+            uint32_t flags = 0;
+            while (oi < p.obs_nodes.size() && p.obs_nodes[oi] < n) ++oi;
+            if (oi < p.obs_nodes.size() && p.obs_nodes[oi] == n && n > 0) flags |= kNodeObserved;
+            if (n == terminal) flags |= kNodeTerminal;
+            if (p.has_cash[n]) flags |= kNodeCash;
+            gt.node_flags.push_back(flags);
+            gt.cash.push_back(p.has_cash[n] ? p.cash[n] : 0.0);
+        }
+        // observation rows: discounts (policy.rs:1049-1098)
+        for (uint32_t r = 0; r < g.obs; ++r) {
+            const uint32_t node = p.obs_nodes[r];
+            double d0 = 0.0, ds = 1.0;
+            if (!check_discount(p.logd[node], d0) && bad_group < 0) {
+                bad_group = static_cast<int64_t>(gi);
+                bad_node = node;
+            }
+            if (r + 1 < g.obs) {
+                const uint32_t later = p.obs_nodes[r + 1];
+                if (!check_discount(p.logd[later] - p.logd[node], ds) && node != 0 && bad_group < 0) {
+                    bad_group = static_cast<int64_t>(gi);
+                    bad_node = node;
+                }
+            }
+            gt.disc0.push_back(d0);
+            gt.disc_step.push_back(ds);
+            gt.row_time.push_back(p.times[node]);
+            gt.row_node.push_back(node);
+        }
+        const uint32_t later_final = g.fit_rows ? p.obs_nodes[g.origin_row] : p.obs_nodes[g.obs - 1];
+        double dfinal = 0.0;
+        if (!check_discount(p.logd[later_final], dfinal) && bad_group < 0) {
+            bad_group = static_cast<int64_t>(gi);
+            bad_node = 0;
+        }
+        g.disc_final = dfinal;
+        gt.groups.push_back(g);
+    }
+    (void)err;
+    return FOHIP_OK;
+}
+
+namespace {
+
+// Rust Display of f64 values in messages uses the shortest round-trip form; %.17g is close enough
+// for diagnostics and is exact.
+const char *regression_message(int code, char *buf, size_t cap, int degree, const RowError &e) {
+    switch (code) {
+        case FOHIP_E_INSUFFICIENT_OBSERVATIONS:
+            std::snprintf(buf, cap, "regression has %lld observations but degree %d requires at least %d",
+                          (long long)e.index, degree, degree + 1);
+            break;
+        case FOHIP_E_DEGENERATE_PREDICTOR:
+            std::snprintf(buf, cap, "degree %d regression predictor scale is degenerate; observed %.17g", degree,
+                          e.observed);
+            break;
+        case FOHIP_E_RANK_DEFICIENT:
+            std::snprintf(buf, cap,
+                          "degree %d polynomial design has numerical rank %lld of %d at rank tolerance %.17g",
+                          degree, (long long)e.index, degree + 1, e.limit);
+            break;
+        case FOHIP_E_CONDITION_LIMIT:
+            std::snprintf(buf, cap,
+                          "degree %d polynomial design condition number %.17g exceeds configured limit %.17g",
+                          degree, e.observed, e.limit);
+            break;
+        case FOHIP_E_JACOBI_NOT_CONVERGED:
+            std::snprintf(buf, cap, "polynomial condition diagnostic did not converge for %lld columns after 32 sweeps",
+                          (long long)e.index);
+            break;
+        default:
+            std::snprintf(buf, cap, "polynomial regression produced non-finite arithmetic at basis degree %lld",
+                          (long long)e.index);
+    }
+    return buf;
+}
+
+}  // namespace
+
+// Runs the full pipeline for a list of groups (processed in memory-bounded batches).
+static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, const std::vector<double> &strikes,
+                      const RunSpec &spec, std::vector<RowOut> &out, std::vector<Model> *models_out,
+                      std::vector<uint32_t> *counts_out, fohip_error *err) {
+    HIP_TRY(hipSetDevice(ex->device));
+    hipStream_t st = ex->stream;
+    const bool f32 = spec.f32;
+    const size_t real = f32 ? sizeof(float) : sizeof(double);
+    out.assign(strikes.size(), RowOut());
+    // memory-bounded batches of consecutive groups
+    std::vector<std::pair<size_t, size_t>> batches;
+    {
+        size_t begin = 0, bytes = 0;
+        for (size_t g = 0; g < all.size(); ++g) {
+            const GroupPlan &p = all[g];
+            const size_t need = (p.obs_nodes.size() + (p.strike_end - p.strike_begin)) * p.nt * real;
+            if (g > begin && bytes + need > ex->batch_budget) {
+                batches.push_back({begin, g});
+                begin = g;
+                bytes = 0;
+            }
+            bytes += need;
+        }
+        batches.push_back({begin, all.size()});
+    }
+    if (models_out) models_out->clear();
+    if (counts_out) counts_out->clear();
+    ex->last_models.clear();
+    for (auto [gb, ge] : batches) {
+        std::vector<GroupPlan> plans(all.begin() + gb, all.begin() + ge);
+        // strike rows of this batch are contiguous: rebase to the batch
+        const uint32_t row0 = plans.front().strike_begin;
+        const uint32_t row1 = plans.back().strike_end;
+        const uint32_t nrows = row1 - row0;
+        std::vector<double> bstrikes(strikes.begin() + row0, strikes.begin() + row1);
+        for (auto &p : plans) {
+            p.strike_begin -= row0;
+            p.strike_end -= row0;
+        }
+        GroupTables gt;
+        uint64_t path_elems, value_elems, model_count, part_t, part_e, count_slots;
+        int64_t bad_group, bad_node;
+        build_tables(plans, bstrikes, f32, gt, path_elems, value_elems, model_count, part_t, part_e, count_slots, err,
+                     bad_group, bad_node);
+        const uint32_t ngroups = static_cast<uint32_t>(plans.size());
+        std::vector<uint32_t> strike_group(nrows);
+        uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
+        for (uint32_t g = 0; g < ngroups; ++g) {
+            for (uint32_t s = plans[g].strike_begin; s < plans[g].strike_end; ++s) strike_group[s] = g;
+            max_strikes = std::max(max_strikes, plans[g].strike_end - plans[g].strike_begin);
+            max_obs = std::max(max_obs, gt.groups[g].obs);
+            max_fit = std::max(max_fit, gt.groups[g].fit_rows);
+        }
+        // work items ordered by fit rows (descending) so each backward step uses a prefix
+        std::vector<uint32_t> order(ngroups);
+        for (uint32_t g = 0; g < ngroups; ++g) order[g] = g;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return gt.groups[a].fit_rows > gt.groups[b].fit_rows; });
+        std::vector<uint2> items_t, items_e;
+        std::vector<uint32_t> rows_sorted;
+        std::vector<uint32_t> step_items(max_fit + 1, 0), step_rows(max_fit + 1, 0);
+        for (uint32_t g : order) {
+            for (uint32_t c = 0; c < gt.groups[g].chunks_t; ++c) items_t.push_back(make_uint2(g, c));
+            for (uint32_t s = plans[g].strike_begin; s < plans[g].strike_end; ++s) rows_sorted.push_back(s);
+        }
+        for (uint32_t g = 0; g < ngroups; ++g)
+            for (uint32_t c = 0; c < gt.groups[g].chunks_e; ++c) items_e.push_back(make_uint2(g, c));
+        for (uint32_t j = 0; j <= max_fit; ++j) {
+            uint32_t ni = 0, nr = 0;
+            for (uint32_t g : order) {
+                if (gt.groups[g].fit_rows > j || j == max_fit) {
+                    ni += gt.groups[g].chunks_t;
+                    nr += plans[g].strike_end - plans[g].strike_begin;
+                }
+            }
+            step_items[j] = ni;
+            step_rows[j] = nr;
+        }
+        // ---- device buffers
+        const size_t tables_bytes_list[] = {
+            gt.groups.size() * sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8,
+            gt.log_drift32.size() * 4, gt.diffusion32.size() * 4, gt.dt.size() * 8, gt.sqrt_dt.size() * 8,
+            gt.cash.size() * 8, gt.node_flags.size() * 4, gt.disc0.size() * 8, gt.disc_step.size() * 8,
+            gt.row_time.size() * 8, gt.row_node.size() * 4, bstrikes.size() * 8, strike_group.size() * 4};
+        size_t tables_bytes = 0;
+        std::vector<size_t> offs;
+        for (size_t b : tables_bytes_list) {
+            offs.push_back(tables_bytes);
+            tables_bytes += (b + 255) & ~size_t(255);
+        }
+        HIP_TRY(ex->tables.ensure(tables_bytes));
+        HIP_TRY(ex->paths.ensure(std::max<uint64_t>(path_elems, 1) * real));
+        HIP_TRY(ex->values.ensure(std::max<uint64_t>(value_elems, 1) * real));
+        HIP_TRY(ex->models.ensure(std::max<uint64_t>(model_count, 1) * sizeof(Model)));
+        HIP_TRY(ex->part_t.ensure(std::max<uint64_t>(part_t, 1) * kPartialStride * sizeof(double)));
+        HIP_TRY(ex->part_e.ensure(std::max<uint64_t>(part_e, 1) * kPartialStride * sizeof(double)));
+        HIP_TRY(ex->counts.ensure(std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t)));
+        HIP_TRY(ex->nosup.ensure(std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t)));
+        HIP_TRY(ex->errors.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowError)));
+        HIP_TRY(ex->fail_t.ensure(ngroups * sizeof(unsigned long long)));
+        HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
+        HIP_TRY(ex->items_t.ensure(std::max<size_t>(items_t.size(), 1) * sizeof(uint2)));
+        HIP_TRY(ex->items_e.ensure(std::max<size_t>(items_e.size(), 1) * sizeof(uint2)));
+        HIP_TRY(ex->rows.ensure(std::max<size_t>(rows_sorted.size(), 1) * sizeof(uint32_t)));
+        HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
+        HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
+        char *tb = ex->tables.as<char>();
+        const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
+                                    gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
+                                    gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
+                                    gt.row_node.data(), bstrikes.data(), strike_group.data()};
+        for (size_t i = 0; i < offs.size(); ++i)
+            if (tables_bytes_list[i]) HIP_TRY(hipMemcpyAsync(tb + offs[i], host_parts[i], tables_bytes_list[i], hipMemcpyHostToDevice, st));
+        DevTables t;
+        t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
+        t.log_drift = reinterpret_cast<const double *>(tb + offs[1]);
+        t.diffusion = reinterpret_cast<const double *>(tb + offs[2]);
+        t.log_drift32 = reinterpret_cast<const float *>(tb + offs[3]);
+        t.diffusion32 = reinterpret_cast<const float *>(tb + offs[4]);
+        t.dt = reinterpret_cast<const double *>(tb + offs[5]);
+        t.sqrt_dt = reinterpret_cast<const double *>(tb + offs[6]);
+        t.cash = reinterpret_cast<const double *>(tb + offs[7]);
+        t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[8]);
+        t.disc0 = reinterpret_cast<const double *>(tb + offs[9]);
+        t.disc_step = reinterpret_cast<const double *>(tb + offs[10]);
+        t.row_time = reinterpret_cast<const double *>(tb + offs[11]);
+        t.row_node = reinterpret_cast<const uint32_t *>(tb + offs[12]);
+        t.strikes = reinterpret_cast<const double *>(tb + offs[13]);
+        t.strike_group = reinterpret_cast<const uint32_t *>(tb + offs[14]);
+        HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items_t.data(), items_t.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ex->items_e.ptr, items_e.data(), items_e.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ex->rows.ptr, rows_sorted.data(), rows_sorted.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemsetAsync(ex->models.ptr, 0, model_count * sizeof(Model), st));
+        HIP_TRY(hipMemsetAsync(ex->counts.ptr, 0, count_slots * sizeof(uint32_t), st));
+        HIP_TRY(hipMemsetAsync(ex->nosup.ptr, 0, count_slots * sizeof(uint32_t), st));
+        HIP_TRY(hipMemsetAsync(ex->errors.ptr, 0xFF, nrows * sizeof(RowError), st));
+        HIP_TRY(hipMemsetAsync(ex->fail_t.ptr, 0xFF, ngroups * sizeof(unsigned long long), st));
+        HIP_TRY(hipMemsetAsync(ex->fail_e.ptr, 0xFF, ngroups * sizeof(unsigned long long), st));
+        const uint2 *it = ex->items_t.as<uint2>();
+        const uint32_t *rows = ex->rows.as<uint32_t>();
+        Model *models = ex->models.as<Model>();
+        RowError *errors = ex->errors.as<RowError>();
+        const FitParams &fp = spec.fp;
+        // ---- training simulation
+        PROF_LAUNCH("k_simulate", launch_simulate(st, f32, t, it, static_cast<uint32_t>(items_t.size()), ex->paths.ptr, 0,
+                                ex->fail_t.as<unsigned long long>()));
+        // ---- backward induction
+        for (uint32_t j = 0; j < max_fit; ++j) {
+            const uint32_t ni = step_items[j], nr = step_rows[j];
+            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, f32, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
+                                       ex->part_t.as<double>(), errors));
+            PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
+            if (f32) {
+                PROF_LAUNCH("k_fit_gram", launch_fit_gram(st, f32, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
+                                        ex->part_t.as<double>(), errors));
+                PROF_LAUNCH("k_fit_reduce_gram", launch_fit_reduce_gram(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
+            } else {
+                for (int k = 0; k <= fp.degree; ++k) {
+                    PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, f32, t, fp, j, k, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr,
+                                               models, ex->part_t.as<double>(), errors));
+                    PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
+                }
+            }
+        }
+        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
+                                 ex->values.ptr, models, ex->part_t.as<double>(), errors));
+        PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));
+        // ---- evaluation
+        const bool greeks = spec.greeks != 0;
+        const uint32_t tile = eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
+        PROF_LAUNCH("k_evaluate", launch_evaluate(st, f32, greeks, tile, t, fp, ex->items_e.as<uint2>(),
+                                static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
+                                ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
+                                ex->fail_e.as<unsigned long long>()));
+        PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
+        PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
+                                   ex->summary.as<RowSummary>()));
+        // ---- results
+        std::vector<unsigned long long> fail_t(ngroups), fail_e(ngroups);
+        std::vector<RowError> row_err(nrows);
+        std::vector<double> eval_out(static_cast<size_t>(nrows) * 16);
+        std::vector<RowSummary> summary(nrows);
+        HIP_TRY(hipMemcpyAsync(fail_t.data(), ex->fail_t.ptr, ngroups * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(fail_e.data(), ex->fail_e.ptr, ngroups * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(row_err.data(), ex->errors.ptr, nrows * sizeof(RowError), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(eval_out.data(), ex->eval_out.ptr, eval_out.size() * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(summary.data(), ex->summary.ptr, nrows * sizeof(RowSummary), hipMemcpyDeviceToHost, st));
+        std::vector<Model> models_h;
+        std::vector<uint32_t> counts_h;
+        if (models_out) {
+            models_h.resize(model_count);
+            HIP_TRY(hipMemcpyAsync(models_h.data(), ex->models.ptr, model_count * sizeof(Model), hipMemcpyDeviceToHost, st));
+        }
+        if (counts_out) {
+            counts_h.resize(count_slots);
+            HIP_TRY(hipMemcpyAsync(counts_h.data(), ex->counts.ptr, count_slots * 4, hipMemcpyDeviceToHost, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipGetLastError());
+        if (ex->profiling) ex->collect();
+        // ---- failures, in the reference's processing order: groups in order; within a group the
+        // training simulation, then strike rows' fits in row order, then the evaluation simulation,
+        // then strike rows' evaluations.
+        for (uint32_t g = 0; g < ngroups; ++g) {
+            const DevGroup &G = gt.groups[g];
+            const GroupPlan &P = plans[g];
+            const int64_t first_row = static_cast<int64_t>(row0 + P.strike_begin);
+            auto sim_failure = [&](int evaluation, unsigned long long path) -> int {
+                std::vector<double> info(4, 0.0);
+                (void)ex->replay.ensure(4 * sizeof(double));
+                HIP_TRY(launch_replay_path(st, t, g, evaluation, static_cast<uint32_t>(path), ex->replay.as<double>()));
+                HIP_TRY(hipMemcpyAsync(info.data(), ex->replay.ptr, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                const int phase = evaluation ? FOHIP_PHASE_EVALUATION_SIMULATION : FOHIP_PHASE_TRAINING_SIMULATION;
+                const char *role = evaluation ? "evaluation" : "training";
+                const uint32_t interval = static_cast<uint32_t>(info[0] < 0 ? 0 : info[0]);
+                set_fields(err, first_row, interval + 1, static_cast<int64_t>(path), interval, info[1], info[2]);
+                if (info[3] != 0.0)
+                    return fail(err, FOHIP_E_NORMAL_REJECTION_LIMIT, phase,
+                                "LSM %s simulation failed: random-normal rejection limit reached at interval %u, "
+                                "path %llu; no variate or fallback was produced",
+                                role, interval, path);
+                return fail(err, FOHIP_E_NON_FINITE_TRANSITION, phase,
+                            "LSM %s simulation failed: GBM path transition from node %u to %u at path %llu produced "
+                            "non-finite spot from state %.17g and variate %.17g",
+                            role, interval, interval + 1, path, info[1], info[2]);
+            };
+            if (fail_t[g] != kNoError) return sim_failure(0, fail_t[g]);
+            if (bad_group == static_cast<int64_t>(g)) {
+                set_fields(err, first_row, bad_node, -1, -1, 0, 0);
+                return fail(err, FOHIP_E_UNREPRESENTABLE_DISCOUNT, FOHIP_PHASE_FIT,
+                            "discount factor to observation node %lld is not finite and positive", (long long)bad_node);
+            }
+            for (uint32_t s = P.strike_begin; s < P.strike_end; ++s) {
+                const RowError &e = row_err[s];
+                if (e.key == kNoError) continue;
+                const uint32_t step = static_cast<uint32_t>(e.key >> 44);
+                if (step >= kEvalStepBase) continue;
+                const uint32_t stage = static_cast<uint32_t>((e.key >> 36) & 0xFF);
+                const uint64_t path = e.key & ((1ull << 36) - 1);
+                const int64_t row = static_cast<int64_t>(row0 + s);
+                uint32_t node = 0;
+                if (step < G.fit_rows) node = P.obs_nodes[G.obs - 2 - step];
+                if (stage == kStageRegression) {
+                    char msg[256];
+                    set_fields(err, row, node, -1, e.index, e.observed, e.limit);
+                    regression_message(e.code, msg, sizeof msg, fp.degree, e);
+                    return fail(err, e.code, FOHIP_PHASE_FIT, "continuation regression failed at exercise node %u: %s",
+                                node, msg);
+                }
+                set_fields(err, row, node, static_cast<int64_t>(path), -1, 0, 0);
+                if (stage == kStageApply)
+                    return fail(err, FOHIP_E_NON_FINITE_PREDICTION, FOHIP_PHASE_FIT,
+                                "continuation prediction failed at exercise node %u, path %llu", node,
+                                (unsigned long long)path);
+                if (stage == kStagePredictor)
+                    return fail(err, FOHIP_E_NON_FINITE_PREDICTOR, FOHIP_PHASE_FIT,
+                                "continuation regression failed at exercise node %u: regression predictor must be "
+                                "finite at path %llu",
+                                node, (unsigned long long)path);
+                return fail(err, FOHIP_E_NON_FINITE_DISCOUNTED_PAYOFF, FOHIP_PHASE_FIT,
+                            "LSM evaluation produced non-finite discounted payoff at path %llu", (unsigned long long)path);
+            }
+            if (fail_e[g] != kNoError) return sim_failure(1, fail_e[g]);
+            for (uint32_t s = P.strike_begin; s < P.strike_end; ++s) {
+                const RowError &e = row_err[s];
+                const RowSummary &rs = summary[s];
+                const int64_t row = static_cast<int64_t>(row0 + s);
+                int64_t err_row_obs = -1;
+                uint32_t stage = 0;
+                uint64_t path = 0;
+                if (e.key != kNoError && (e.key >> 44) >= kEvalStepBase) {
+                    err_row_obs = static_cast<int64_t>((e.key >> 44) - kEvalStepBase);
+                    stage = static_cast<uint32_t>((e.key >> 36) & 0xFF);
+                    path = e.key & ((1ull << 36) - 1);
+                }
+                if (rs.nosupport_row >= 0 && (err_row_obs < 0 || rs.nosupport_row <= err_row_obs)) {
+                    const uint32_t node = P.obs_nodes[rs.nosupport_row];
+                    set_fields(err, row, node, -1, rs.nosupport_count, 0, 0);
+                    return fail(err, FOHIP_E_OUTSIDE_TRAINING_SUPPORT, FOHIP_PHASE_EVALUATION,
+                                "fixed policy has no training support at exercise node %u, but %u active evaluation "
+                                "paths are in the money",
+                                node, rs.nosupport_count);
+                }
+                if (err_row_obs >= 0) {
+                    const uint32_t node = P.obs_nodes[err_row_obs];
+                    set_fields(err, row, node, static_cast<int64_t>(path), -1, 0, 0);
+                    if (stage == kStageEvalPrediction)
+                        return fail(err, FOHIP_E_NON_FINITE_PREDICTION, FOHIP_PHASE_EVALUATION,
+                                    "continuation prediction failed at exercise node %u, path %llu", node,
+                                    (unsigned long long)path);
+                    return fail(err, FOHIP_E_NON_FINITE_DISCOUNTED_PAYOFF, FOHIP_PHASE_EVALUATION,
+                                "LSM evaluation produced non-finite discounted payoff at path %llu",
+                                (unsigned long long)path);
+                }
+            }
+        }
+        // ---- row outputs
+        for (uint32_t s = 0; s < nrows; ++s) {
+            const uint32_t g = strike_group[s];
+            const GroupPlan &P = plans[g];
+            RowOut &o = out[row0 + s];
+            o.sum = summary[s];
+            const double n = static_cast<double>(P.ne);
+            if (summary[s].origin_fitted && summary[s].origin_exercised) {
+                // policy.rs:672-682: deterministic origin exercise
+                o.price = std::max(P.S0 - bstrikes[s], 0.0);
+                o.sd = o.se = 0.0;
+                o.has_sd = true;
+                o.sum.early = static_cast<uint32_t>(P.ne);
+                o.sum.expiry = 0;
+                o.greek[0] = P.S0 > bstrikes[s] ? 1.0 : 0.0;
+                continue;
+            }
+            const double *e = &eval_out[static_cast<size_t>(s) * 16];
+            o.price = e[0];
+            if (P.ne > 1) {
+                o.sd = std::sqrt(e[1] / (n - 1.0));
+                o.se = o.sd / std::sqrt(n);
+                o.has_sd = true;
+            }
+            for (int a = 0; a < 4; ++a) {
+                o.greek[a] = e[3 + a];
+                o.greek_se[a] = e[7 + a];
+            }
+        }
+        if (models_out) {
+            // append with strike rows in batch order: models are [strike][obs] per group
+            models_out->insert(models_out->end(), models_h.begin(), models_h.end());
+        }
+        if (counts_out) counts_out->insert(counts_out->end(), counts_h.begin(), counts_h.end());
+        // keep the last batch's layout for diagnostics
+        ex->last_models = models_h;
+    }
+    return FOHIP_OK;
+}
+
+// ============================================================================ single call
+
+static int parse_regression(uint32_t degree, double rank_tol, double cond_limit, int32_t tie, int32_t support,
+                            FitParams &fp, fohip_error *err) {
+    // regression.rs:23-90
+    if (degree > 4)
+        return fail(err, FOHIP_E_UNSUPPORTED_DEGREE, 0, "polynomial degree %u exceeds the supported maximum 4", degree);
+    if (!std::isfinite(rank_tol) || rank_tol <= 0.0 || rank_tol >= 1.0)
+        return fail(err, FOHIP_E_INVALID_REGRESSION_SPEC, 0,
+                    "rank tolerance must be finite and strictly between 0 and 1; observed %.17g", rank_tol);
+    const double maximum = 1.0 / rank_tol;
+    if (!std::isfinite(cond_limit) || cond_limit < 1.0 || cond_limit > maximum)
+        return fail(err, FOHIP_E_INVALID_REGRESSION_SPEC, 0,
+                    "condition limit must be finite and in [1, %.17g]; observed %.17g", maximum, cond_limit);
+    if (tie != 0 && tie != 1) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown exercise tie rule %d", tie);
+    if (support != 0 && support != 1)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown insufficient-support policy %d", support);
+    fp.degree = static_cast<int32_t>(degree);
+    fp.tie = tie;
+    fp.support = support;
+    fp.pad = 0;
+    fp.rank_tol = rank_tol;
+    fp.cond_limit = cond_limit;
+    return FOHIP_OK;
+}
+
+static bool same_grid(const double *a, size_t na, const double *b, size_t nb) {
+    if (na != nb) return false;
+    for (size_t i = 0; i < na; ++i) {
+        uint64_t x, y;
+        std::memcpy(&x, &a[i], 8);
+        std::memcpy(&y, &b[i], 8);
+        if (x != y) return false;
+    }
+    return true;
+}
+
+extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request *rq, fohip_call_result *res,
+                                    fohip_error *err) {
+    clear_error(err);
+    if (!ex || !rq || !res) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null executor, request or result");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    int rc;
+    // lib.rs:1170-1248 parse order: financial model, regression spec, estimator
+    if ((rc = fohip_validate_event_schedule(rq->schedule_times, rq->schedule_nodes, rq->dividend_nodes,
+                                            rq->dividend_amounts, rq->dividends, rq->exercise_nodes,
+                                            rq->exercise_count, rq->expiry_node, err)))
+        return rc;
+    if ((rc = fohip_validate_log_discount_curve(rq->curve_times, rq->log_discounts, rq->curve_nodes, err))) return rc;
+    if ((rc = fohip_validate_cumulative_yield_curve(rq->yield_times, rq->cumulative_yields, rq->yield_nodes, err)))
+        return rc;
+    if ((rc = fohip_validate_constant_volatility(rq->volatility, err))) return rc;
+    if (rq->dividend_policy != 0 && rq->dividend_policy != 1)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown dividend policy %d", rq->dividend_policy);
+    if (!same_grid(rq->schedule_times, rq->schedule_nodes, rq->curve_times, rq->curve_nodes))
+        return fail(err, FOHIP_E_TIME_GRID_MISMATCH, 0, "discount curve and event schedule must use the identical time grid");
+    if (!same_grid(rq->schedule_times, rq->schedule_nodes, rq->yield_times, rq->yield_nodes))
+        return fail(err, FOHIP_E_TIME_GRID_MISMATCH, 0,
+                    "continuous dividend-yield curve and event schedule must use the identical time grid");
+    FitParams fp;
+    if ((rc = parse_regression(rq->polynomial_degree, rq->rank_tolerance, rq->condition_limit, rq->tie_rule,
+                               rq->insufficient_support, fp, err)))
+        return rc;
+    if (!std::isfinite(rq->strike) || rq->strike <= 0.0)
+        return fail(err, FOHIP_E_INVALID_STRIKE, 0, "call strike must be finite and strictly positive; observed %.17g",
+                    rq->strike);
+    // policy.rs:306-336
+    if (rq->training_domain == rq->evaluation_domain)
+        return fail(err, FOHIP_E_RANDOM_DOMAIN_COLLISION, 0,
+                    "LSM training and evaluation random domains must differ; both use domain %u", rq->training_domain);
+    if (!canonical_zero_or_positive(rq->initial_spot))
+        return fail(err, FOHIP_E_INVALID_INITIAL_SPOT, FOHIP_PHASE_TRAINING_SIMULATION,
+                    "LSM training simulation failed: initial spot must be finite, nonnegative, and canonical; observed "
+                    "%.17g",
+                    rq->initial_spot);
+    if (rq->training_paths == 0 || rq->evaluation_paths == 0)
+        return fail(err, FOHIP_E_EMPTY_PATH_SET, rq->training_paths == 0 ? 1 : 3,
+                    "LSM %s simulation failed: simulation path count must be positive",
+                    rq->training_paths == 0 ? "training" : "evaluation");
+    if (rq->training_paths > 0xFFFFFFFFull || rq->evaluation_paths > 0xFFFFFFFFull)
+        return fail(err, FOHIP_E_PATH_NAMESPACE_OVERFLOW, 0, "path count exceeds the 32-bit path namespace");
+    if (rq->training_paths >= (1ull << 36) / 16 || rq->evaluation_paths >= (1ull << 36) / 16)
+        return fail(err, FOHIP_E_UNSUPPORTED, 0, "path count exceeds this engine's per-group limit");
+    GroupPlan p;
+    p.times.assign(rq->schedule_times, rq->schedule_times + rq->schedule_nodes);
+    p.logd.assign(rq->log_discounts, rq->log_discounts + rq->curve_nodes);
+    p.q.assign(rq->cumulative_yields, rq->cumulative_yields + rq->yield_nodes);
+    p.cash.assign(p.times.size(), 0.0);
+    p.has_cash.assign(p.times.size(), 0);
+    for (size_t i = 0; i < rq->dividends; ++i) {
+        p.cash[rq->dividend_nodes[i]] = rq->dividend_amounts[i];
+        p.has_cash[rq->dividend_nodes[i]] = 1;
+    }
+    p.obs_nodes.assign(rq->exercise_nodes, rq->exercise_nodes + rq->exercise_count);
+    p.obs_nodes.push_back(rq->expiry_node);
+    p.S0 = rq->initial_spot;
+    p.sigma = rq->volatility;
+    p.policy = rq->dividend_policy;
+    p.strike_begin = 0;
+    p.strike_end = 1;
+    p.nt = rq->training_paths;
+    p.ne = rq->evaluation_paths;
+    p.train_seed = rq->training_seed;
+    p.eval_seed = rq->evaluation_seed;
+    p.train_domain = rq->training_domain;
+    p.eval_domain = rq->evaluation_domain;
+    RunSpec spec;
+    spec.fp = fp;
+    spec.greeks = rq->greeks;
+    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    std::vector<RowOut> out;
+    std::vector<Model> models;
+    std::vector<uint32_t> counts;
+    std::vector<GroupPlan> plans{p};
+    std::vector<double> strikes{rq->strike};
+    rc = run_groups(ex, plans, strikes, spec, out, &models, &counts, err);
+    if (rc) return rc;
+    const RowOut &o = out[0];
+    const uint32_t obs = static_cast<uint32_t>(p.obs_nodes.size());
+    res->price = o.price;
+    res->paths = static_cast<uint32_t>(p.ne);
+    res->has_standard_error = o.has_sd ? 1 : 0;
+    res->sample_standard_deviation = o.sd;
+    res->standard_error = o.se;
+    res->observation_rows = obs;
+    const bool origin_exercise = o.sum.origin_fitted && o.sum.origin_exercised;
+    for (uint32_t r = 0; r < obs; ++r) {
+        if (res->observation_nodes) res->observation_nodes[r] = p.obs_nodes[r];
+        if (res->exercise_counts) res->exercise_counts[r] = origin_exercise ? (r == 0 ? res->paths : 0) : counts[r];
+    }
+    for (int a = 0; a < 4; ++a) {
+        res->greeks[a] = o.greek[a];
+        res->greek_standard_errors[a] = o.greek_se[a];
+    }
+    if (res->diagnostics) {
+        for (uint32_t r = 0; r + 1 < obs; ++r) {
+            const Model &m = models[r];
+            fohip_node_diagnostic &d = res->diagnostics[r];
+            std::memset(&d, 0, sizeof d);
+            d.node = p.obs_nodes[r];
+            switch (m.kind) {
+                case kRuleOrigin:
+                    d.kind = 0;
+                    d.exercise = m.exercise;
+                    d.estimated_continuation = m.estimate;
+                    break;
+                case kRuleNoSupport:
+                    d.kind = 1;
+                    break;
+                case kRuleContinue:
+                    d.kind = 2;
+                    d.in_the_money_paths = m.itm;
+                    d.columns = static_cast<uint32_t>(fp.degree + 1);
+                    break;
+                default:
+                    d.kind = 3;
+                    d.in_the_money_paths = m.itm;
+                    d.observations = m.itm;
+                    d.columns = static_cast<uint32_t>(fp.degree + 1);
+                    d.rank = static_cast<uint32_t>(m.rank);
+                    d.condition_number = m.condition;
+                    d.predictor_mean = m.mean;
+                    d.predictor_scale = m.scale;
+                    for (int k = 0; k < 4; ++k) {
+                        d.alpha[k] = m.alpha[k];
+                        d.beta[k] = m.beta[k];
+                    }
+                    for (int k = 0; k < 5; ++k) d.coefficients[k] = m.coef[k];
+            }
+        }
+    }
+    return FOHIP_OK;
+}
+
+// =========================================================================== grouped batch
+
+namespace {
+
+int total_cmp(double a, double b) { return a < b ? -1 : (a > b ? 1 : 0); }
+
+int64_t search_time(const std::vector<double> &times, double t) {
+    auto it = std::lower_bound(times.begin(), times.end(), t, [](double a, double b) { return total_cmp(a, b) < 0; });
+    if (it != times.end() && total_cmp(*it, t) == 0) return it - times.begin();
+    return -1;
+}
+
+}  // namespace
+
+static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<GroupPlan> &plans,
+                               std::vector<double> &strikes, fohip_error *err) {
+    const size_t groups = rq->groups;
+    // input.rs:617-723
+    if (groups == 0 || rq->rows == 0) return fail(err, FOHIP_E_EMPTY_BATCH, 0, "batch must contain at least one contract");
+    if (!rq->group_offsets || !rq->initial_spots || !rq->maturities || !rq->rates || !rq->volatilities || !rq->strikes)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null batch column");
+    // validate_group_offsets (input.rs:761-796)
+    if (rq->group_offsets[0] != 0)
+        return fail(err, FOHIP_E_GROUP_OFFSETS, 0, "group offsets must start at zero; observed %u", rq->group_offsets[0]);
+    for (size_t g = 0; g < groups; ++g)
+        if (rq->group_offsets[g] >= rq->group_offsets[g + 1])
+            return fail(err, FOHIP_E_GROUP_OFFSETS, 0, "group %zu has an empty or decreasing range %u..%u", g,
+                        rq->group_offsets[g], rq->group_offsets[g + 1]);
+    if (rq->group_offsets[groups] != rq->rows)
+        return fail(err, FOHIP_E_GROUP_OFFSETS, 0, "final group offset %u must equal the strike row count %zu",
+                    rq->group_offsets[groups], rq->rows);
+    for (size_t r = 0; r < rq->rows; ++r)
+        if (!std::isfinite(rq->strikes[r]) || rq->strikes[r] <= 0.0)
+            return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0, "strikes[%zu] must be finite and strictly positive; observed %.17g",
+                        r, rq->strikes[r]);
+    if (rq->dividend_model != 0 && rq->dividend_model != 1)
+        return fail(err, FOHIP_E_UNSUPPORTED, 0, "unsupported dividend model %d", rq->dividend_model);
+    if (rq->dividend_model == 1) {
+        if (!rq->cash_offsets) return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "cash-dividend offsets are missing");
+        int rc = fohip_validate_ragged_offsets(rq->cash_offsets, groups + 1, rq->cash_events, err);
+        if (rc) return rc;
+        if (rq->maturity_boundary != 0 && rq->maturity_boundary != 1)
+            return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "unsupported cash maturity boundary");
+        if (rq->cash_policy != 0 && rq->cash_policy != 1)
+            return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown dividend policy %d", rq->cash_policy);
+    }
+    for (size_t g = 0; g < groups; ++g) {
+        const double S0 = rq->initial_spots[g], T = rq->maturities[g], r = rq->rates[g], v = rq->volatilities[g];
+        if (!canonical_zero_or_positive(S0))
+            return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0, "initial_spots[%zu] must be finite and nonnegative; observed %.17g",
+                        g, S0);
+        if (!std::isfinite(T) || T <= 0.0)
+            return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0, "maturities[%zu] must be finite and strictly positive; observed %.17g",
+                        g, T);
+        if (!std::isfinite(r))
+            return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0, "rates[%zu] must be finite; observed %.17g", g, r);
+        if (!std::isfinite(v) || v <= 0.0)
+            return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0,
+                        "volatilities[%zu] must be finite and strictly positive; observed %.17g", g, v);
+        if (rq->dividend_model == 0) {
+            const double q = rq->dividend_yields[g];
+            if (!canonical_zero_or_positive(q))
+                return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0,
+                            "continuous_dividend_yields[%zu] must be finite and nonnegative; observed %.17g", g, q);
+        } else {
+            // validate_cash_schedule (input.rs:725-759)
+            const int64_t cb = rq->cash_offsets[0];
+            for (int64_t e = rq->cash_offsets[g] - cb; e < rq->cash_offsets[g + 1] - cb; ++e) {
+                const double t = rq->cash_times[e];
+                if (!std::isfinite(t) || t <= 0.0)
+                    return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "cash dividend time must be finite and positive");
+                if (t > T)
+                    return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "row %zu cash event %lld at %.17g is after maturity %.17g",
+                                g, (long long)(e - (rq->cash_offsets[g] - cb)), t, T);
+                if (t == T && rq->maturity_boundary == FOHIP_BOUNDARY_STRICTLY_BEFORE)
+                    return fail(err, FOHIP_E_CASH_SCHEDULE, 0,
+                                "row %zu cash event %lld lies at excluded maturity %.17g", g,
+                                (long long)(e - (rq->cash_offsets[g] - cb)), T);
+                if (e > rq->cash_offsets[g] - cb && rq->cash_times[e - 1] >= t)
+                    return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "row %zu cash event times must be strictly increasing", g);
+                int rc = fohip_validate_cash_dividend_amount(rq->cash_amounts[e], err);
+                if (rc) return rc;
+            }
+        }
+    }
+    // method-level checks (lib.rs:1302-1400, input.rs:201-238)
+    if (rq->training_paths == 0 || rq->training_paths > 0xFFFFFFFFull)
+        return fail(err, FOHIP_E_INVALID_PATH_COUNT, 0, "training path count must be in [1, 4294967295]");
+    if (rq->evaluation_paths < 2 || rq->evaluation_paths > 0xFFFFFFFFull)
+        return fail(err, FOHIP_E_INSUFFICIENT_EVALUATION_PATHS, 0,
+                    "batched LSM requires at least two independent evaluation paths");
+    {
+        const uint64_t te = static_cast<uint64_t>(rq->training_domain_start) + groups;
+        const uint64_t ee = static_cast<uint64_t>(rq->evaluation_domain_start) + groups;
+        if (te > (1ull << 32) || ee > (1ull << 32))
+            return fail(err, FOHIP_E_RANDOM_DOMAIN_RANGE, 0, "random domain range overflows the 32-bit domain lane");
+        if (rq->training_domain_start < ee && rq->evaluation_domain_start < te)
+            return fail(err, FOHIP_E_RANDOM_DOMAIN_RANGE, 0,
+                        "training domains %u..%llu and evaluation domains %u..%llu overlap", rq->training_domain_start,
+                        (unsigned long long)te, rq->evaluation_domain_start, (unsigned long long)ee);
+    }
+    if (rq->mesh_kind == FOHIP_MESH_UNIFORM && rq->base_intervals == 0)
+        return fail(err, FOHIP_E_EXERCISE_MESH, 0, "uniform exercise mesh needs at least one base interval");
+    if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES && rq->dividend_model != 1)
+        return fail(err, FOHIP_E_EXERCISE_MESH, 0, "CashDividendExerciseMesh requires cash-dividend inputs");
+    if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
+        if (!rq->exercise_offsets) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "explicit exercise offsets are missing");
+        int rc = fohip_validate_ragged_offsets(rq->exercise_offsets, groups + 1, rq->exercise_events, err);
+        if (rc) return rc;
+    }
+    if (rq->mesh_kind < 0 || rq->mesh_kind > 2) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "unknown exercise mesh");
+    strikes.assign(rq->strikes, rq->strikes + rq->rows);
+    plans.resize(groups);
+    for (size_t g = 0; g < groups; ++g) {
+        GroupPlan &p = plans[g];
+        const double T = rq->maturities[g], rate = rq->rates[g];
+        std::vector<double> cash_t, cash_a;
+        if (rq->dividend_model == 1) {
+            const int64_t cb = rq->cash_offsets[0];
+            cash_t.assign(rq->cash_times + (rq->cash_offsets[g] - cb), rq->cash_times + (rq->cash_offsets[g + 1] - cb));
+            cash_a.assign(rq->cash_amounts + (rq->cash_offsets[g] - cb), rq->cash_amounts + (rq->cash_offsets[g + 1] - cb));
+        }
+        std::vector<double> ex_t;
+        if (rq->mesh_kind == FOHIP_MESH_EXPLICIT)
+            ex_t.assign(rq->exercise_times + (rq->exercise_offsets[g] - rq->exercise_offsets[0]),
+                        rq->exercise_times + (rq->exercise_offsets[g + 1] - rq->exercise_offsets[0]));
+        if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES) {
+            // validate_cash_dividend_exercise_mesh_row (pricing.rs:397-419)
+            if (rate < 0.0)
+                return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu has negative rate %.17g for the cash-dividend mesh",
+                            (size_t)rq->group_offsets[g], rate);
+            for (double t : cash_t)
+                if (t == T)
+                    return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu has terminal cash for the cash-dividend mesh",
+                                (size_t)rq->group_offsets[g]);
+        }
+        if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
+            for (size_t e = 0; e < ex_t.size(); ++e) {
+                if (!std::isfinite(ex_t[e]) || ex_t[e] <= 0.0 || (e > 0 && ex_t[e] <= ex_t[e - 1]))
+                    return fail(err, FOHIP_E_EXERCISE_MESH, 0, "explicit exercise times must be positive and strictly increasing");
+                if (ex_t[e] >= T)
+                    return fail(err, FOHIP_E_EXERCISE_MESH, 0,
+                                "row %zu explicit exercise event %zu at %.17g is at or after maturity %.17g",
+                                (size_t)rq->group_offsets[g], e, ex_t[e], T);
+            }
+        }
+        // exact_union_times / exact_explicit_union_times (pricing.rs:548-695)
+        std::vector<double> &times = p.times;
+        times.clear();
+        if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
+            times.push_back(0.0);
+            size_t e = 0, c = 0;
+            while (e < ex_t.size() || c < cash_t.size()) {
+                double next;
+                if (e < ex_t.size() && c < cash_t.size()) {
+                    const int k = total_cmp(ex_t[e], cash_t[c]);
+                    if (k < 0) {
+                        next = ex_t[e++];
+                    } else if (k == 0) {
+                        next = ex_t[e++];
+                        ++c;
+                    } else {
+                        next = cash_t[c++];
+                    }
+                } else if (e < ex_t.size()) {
+                    next = ex_t[e++];
+                } else {
+                    next = cash_t[c++];
+                }
+                if (next < T) times.push_back(next);
+            }
+            times.push_back(T);
+        } else {
+            const uint32_t base = rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES ? 1u : rq->base_intervals;
+            int64_t node = 0;
+            size_t event = 0;
+            while (node >= 0 || event < cash_t.size()) {
+                const bool have_base = node >= 0;
+                double bt = 0.0;
+                if (have_base)
+                    bt = static_cast<uint32_t>(node) == base
+                             ? T
+                             : T * (static_cast<double>(static_cast<uint32_t>(node)) / static_cast<double>(base));
+                auto advance = [&]() { node = static_cast<uint32_t>(node) == base ? -1 : node + 1; };
+                if (have_base && event < cash_t.size()) {
+                    const int k = total_cmp(bt, cash_t[event]);
+                    if (k < 0) {
+                        times.push_back(bt);
+                        advance();
+                    } else if (k == 0) {
+                        times.push_back(bt);
+                        advance();
+                        ++event;
+                    } else {
+                        times.push_back(cash_t[event++]);
+                    }
+                } else if (have_base) {
+                    times.push_back(bt);
+                    advance();
+                } else {
+                    times.push_back(cash_t[event++]);
+                }
+            }
+        }
+        const size_t group_row = rq->group_offsets[g];
+        {
+            int rc = fohip_validate_time_grid(times.data(), times.size(), err);
+            if (rc) {
+                err->row = static_cast<int64_t>(group_row);
+                return rc;
+            }
+            if (times.size() < 2) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu mesh has no interval", group_row);
+        }
+        // build_row_curves (pricing.rs:862-891)
+        const double yield = rq->dividend_model == 0 ? rq->dividend_yields[g] : 0.0;
+        p.logd.resize(times.size());
+        p.q.resize(times.size());
+        for (size_t i = 0; i < times.size(); ++i) {
+            p.logd[i] = i == 0 ? 0.0 : -rate * times[i];
+            p.q[i] = i == 0 ? 0.0 : yield * times[i];
+        }
+        {
+            int rc = fohip_validate_log_discount_curve(times.data(), p.logd.data(), times.size(), err);
+            if (!rc) rc = fohip_validate_cumulative_yield_curve(times.data(), p.q.data(), times.size(), err);
+            if (rc) {
+                err->row = static_cast<int64_t>(group_row);
+                return rc;
+            }
+        }
+        // build_row_schedule (pricing.rs:893-965)
+        p.cash.assign(times.size(), 0.0);
+        p.has_cash.assign(times.size(), 0);
+        for (size_t e = 0; e < cash_t.size(); ++e) {
+            const int64_t idx = search_time(times, cash_t[e]);
+            if (idx < 0) return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "row %zu cash event %zu has no grid node", group_row, e);
+            p.cash[idx] = cash_a[e];
+            p.has_cash[idx] = 1;
+        }
+        p.obs_nodes.clear();
+        if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
+            if (rq->origin_exercise == FOHIP_ORIGIN_INCLUDED) p.obs_nodes.push_back(0);
+            for (size_t e = 0; e < ex_t.size(); ++e) {
+                const int64_t idx = search_time(times, ex_t[e]);
+                if (idx < 0) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu exercise event %zu has no grid node", group_row, e);
+                p.obs_nodes.push_back(static_cast<uint32_t>(idx));
+            }
+        } else {
+            const uint32_t start =
+                (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES || rq->origin_exercise == FOHIP_ORIGIN_EXCLUDED) ? 1u : 0u;
+            for (uint32_t n = start; n + 1 < times.size(); ++n) p.obs_nodes.push_back(n);
+        }
+        p.obs_nodes.push_back(static_cast<uint32_t>(times.size() - 1));
+        p.S0 = rq->initial_spots[g];
+        p.sigma = rq->volatilities[g];
+        p.policy = rq->dividend_model == 1 ? rq->cash_policy : 0;
+        p.strike_begin = rq->group_offsets[g];
+        p.strike_end = rq->group_offsets[g + 1];
+        p.nt = rq->training_paths;
+        p.ne = rq->evaluation_paths;
+        p.train_seed = rq->training_seed;
+        p.eval_seed = rq->evaluation_seed;
+        p.train_domain = rq->training_domain_start + static_cast<uint32_t>(g);
+        p.eval_domain = rq->evaluation_domain_start + static_cast<uint32_t>(g);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_price_lsm_grouped(fohip_executor *ex, const fohip_grouped_request *rq, fohip_batch_result *res,
+                                       fohip_error *err) {
+    clear_error(err);
+    if (!ex || !rq || !res) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null executor, request or result");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    FitParams fp;
+    int rc = parse_regression(rq->polynomial_degree, rq->rank_tolerance, rq->condition_limit, rq->tie_rule,
+                              rq->insufficient_support, fp, err);
+    if (rc) return rc;
+    std::vector<GroupPlan> plans;
+    std::vector<double> strikes;
+    rc = build_grouped_plans(rq, plans, strikes, err);
+    if (rc) return rc;
+    RunSpec spec;
+    spec.fp = fp;
+    spec.greeks = rq->greeks;
+    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    std::vector<RowOut> out;
+    rc = run_groups(ex, plans, strikes, spec, out, nullptr, nullptr, err);
+    if (rc) return rc;
+    for (size_t r = 0; r < out.size(); ++r) {
+        const RowOut &o = out[r];
+        if (o.sum.early + static_cast<uint64_t>(o.sum.expiry) != rq->evaluation_paths) {
+            set_fields(err, static_cast<int64_t>(r), -1, -1, -1, 0, 0);
+            return fail(err, FOHIP_E_STOPPING_COUNT_MISMATCH, FOHIP_PHASE_EVALUATION,
+                        "row %zu stopping counts %llu do not partition %llu evaluation paths", r,
+                        (unsigned long long)(o.sum.early + static_cast<uint64_t>(o.sum.expiry)),
+                        (unsigned long long)rq->evaluation_paths);
+        }
+    }
+    for (size_t r = 0; r < out.size(); ++r) {
+        const RowOut &o = out[r];
+        if (res->prices) res->prices[r] = o.price;
+        if (res->sample_standard_deviations) res->sample_standard_deviations[r] = o.sd;
+        if (res->standard_errors) res->standard_errors[r] = o.se;
+        if (res->origin_decisions_fitted) res->origin_decisions_fitted[r] = o.sum.origin_fitted;
+        if (res->origin_exercised) res->origin_exercised[r] = o.sum.origin_exercised;
+        if (res->regression_nodes) res->regression_nodes[r] = o.sum.regression_nodes;
+        if (res->no_training_support_nodes) res->no_training_support_nodes[r] = o.sum.no_support_nodes;
+        if (res->insufficient_exercise_support_nodes) res->insufficient_exercise_support_nodes[r] = o.sum.insufficient_nodes;
+        if (res->minimum_in_the_money_training_paths) res->minimum_in_the_money_training_paths[r] = o.sum.min_itm;
+        if (res->maximum_regression_condition_numbers) res->maximum_regression_condition_numbers[r] = o.sum.max_condition;
+        if (res->early_exercise_paths) res->early_exercise_paths[r] = o.sum.early;
+        if (res->expiry_paths) res->expiry_paths[r] = o.sum.expiry;
+        double *cols[4] = {res->delta, res->vega, res->rho, res->dividend_yield};
+        double *secols[4] = {res->delta_se, res->vega_se, res->rho_se, res->dividend_yield_se};
+        for (int a = 0; a < 4; ++a) {
+            if (cols[a]) cols[a][r] = (rq->greeks & (1u << a)) ? o.greek[a] : std::nan("");
+            if (secols[a]) secols[a][r] = (rq->greeks & (1u << a)) ? o.greek_se[a] : std::nan("");
+        }
+    }
+    // keep the per-group layout of the last call for fohip_grouped_diagnostics
+    ex->last_obs.clear();
+    ex->last_strike_begin.clear();
+    ex->last_strike_end.clear();
+    for (const auto &p : plans) {
+        ex->last_obs.push_back(static_cast<uint32_t>(p.obs_nodes.size()));
+        ex->last_strike_begin.push_back(p.strike_begin);
+        ex->last_strike_end.push_back(p.strike_end);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_grouped_diagnostics(fohip_executor *ex, const fohip_grouped_request *rq, uint32_t group,
+                                         fohip_node_diagnostic *out, size_t capacity, uint32_t *exercise_rows,
+                                         fohip_error *err) {
+    clear_error(err);
+    if (!ex || !rq || !out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    FitParams fp;
+    int rc = parse_regression(rq->polynomial_degree, rq->rank_tolerance, rq->condition_limit, rq->tie_rule,
+                              rq->insufficient_support, fp, err);
+    if (rc) return rc;
+    std::vector<GroupPlan> plans;
+    std::vector<double> strikes;
+    rc = build_grouped_plans(rq, plans, strikes, err);
+    if (rc) return rc;
+    if (group >= plans.size()) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "group %u out of range", group);
+    // rerun the single group (identical domains and arithmetic) and return its fitted rules
+    GroupPlan p = plans[group];
+    const uint32_t ns = p.strike_end - p.strike_begin;
+    std::vector<double> gstrikes(strikes.begin() + p.strike_begin, strikes.begin() + p.strike_end);
+    p.strike_begin = 0;
+    p.strike_end = ns;
+    RunSpec spec;
+    spec.fp = fp;
+    spec.greeks = 0;
+    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    std::vector<RowOut> o;
+    std::vector<Model> models;
+    std::vector<GroupPlan> one{p};
+    rc = run_groups(ex, one, gstrikes, spec, o, &models, nullptr, err);
+    if (rc) return rc;
+    const uint32_t obs = static_cast<uint32_t>(p.obs_nodes.size());
+    const size_t need = static_cast<size_t>(ns) * (obs - 1);
+    if (exercise_rows) *exercise_rows = obs - 1;
+    if (capacity < need) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "diagnostics capacity %zu < %zu", capacity, need);
+    for (uint32_t s = 0; s < ns; ++s) {
+        for (uint32_t r = 0; r + 1 < obs; ++r) {
+            const Model &m = models[static_cast<size_t>(s) * obs + r];
+            fohip_node_diagnostic &d = out[static_cast<size_t>(s) * (obs - 1) + r];
+            std::memset(&d, 0, sizeof d);
+            d.node = p.obs_nodes[r];
+            d.kind = m.kind == kRuleOrigin ? 0 : m.kind == kRuleNoSupport ? 1 : m.kind == kRuleContinue ? 2 : 3;
+            d.exercise = m.exercise;
+            d.estimated_continuation = m.estimate;
+            d.in_the_money_paths = m.itm;
+            d.observations = m.itm;
+            d.columns = static_cast<uint32_t>(fp.degree + 1);
+            d.rank = static_cast<uint32_t>(m.rank);
+            d.condition_number = m.condition;
+            d.predictor_mean = m.mean;
+            d.predictor_scale = m.scale;
+            for (int k = 0; k < 4; ++k) {
+                d.alpha[k] = m.alpha[k];
+                d.beta[k] = m.beta[k];
+            }
+            for (int k = 0; k < 5; ++k) d.coefficients[k] = m.coef[k];
+        }
+    }
+    return FOHIP_OK;
+}
+
+// ======================================================================== path export/probes
+
+extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_plan *pl, double S0, uint64_t paths,
+                                         uint64_t seed, uint32_t domain, double *out, fohip_error *err) {
+    clear_error(err);
+    if (!ex || !pl || !out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    int rc = fohip_validate_event_schedule(pl->times, pl->nodes, pl->dividend_nodes, pl->dividend_amounts, pl->dividends,
+                                           pl->exercise_nodes, pl->exercise_count, pl->expiry_node, err);
+    if (!rc) rc = fohip_validate_log_discount_curve(pl->times, pl->log_discounts, pl->nodes, err);
+    if (!rc) rc = fohip_validate_cumulative_yield_curve(pl->times, pl->cumulative_yields, pl->nodes, err);
+    if (!rc) rc = fohip_validate_constant_volatility(pl->volatility, err);
+    if (rc) return rc;
+    if (!canonical_zero_or_positive(S0))
+        return fail(err, FOHIP_E_INVALID_INITIAL_SPOT, 0, "initial spot must be finite, nonnegative, and canonical; observed %.17g", S0);
+    if (paths == 0) return fail(err, FOHIP_E_EMPTY_PATH_SET, 0, "simulation path count must be positive");
+    if (paths > 0xFFFFFFFFull) return fail(err, FOHIP_E_PATH_NAMESPACE_OVERFLOW, 0, "path count exceeds the 32-bit path namespace");
+    HIP_TRY(hipSetDevice(ex->device));
+    GroupPlan p;
+    p.times.assign(pl->times, pl->times + pl->nodes);
+    p.logd.assign(pl->log_discounts, pl->log_discounts + pl->nodes);
+    p.q.assign(pl->cumulative_yields, pl->cumulative_yields + pl->nodes);
+    p.cash.assign(pl->nodes, 0.0);
+    p.has_cash.assign(pl->nodes, 0);
+    for (size_t i = 0; i < pl->dividends; ++i) {
+        p.cash[pl->dividend_nodes[i]] = pl->dividend_amounts[i];
+        p.has_cash[pl->dividend_nodes[i]] = 1;
+    }
+    p.obs_nodes.assign(pl->exercise_nodes, pl->exercise_nodes + pl->exercise_count);
+    p.obs_nodes.push_back(pl->expiry_node);
+    p.S0 = S0;
+    p.sigma = pl->volatility;
+    p.policy = pl->dividend_policy;
+    p.strike_begin = 0;
+    p.strike_end = 0;
+    p.nt = paths;
+    p.ne = 0;
+    p.train_seed = seed;
+    p.train_domain = domain;
+    std::vector<GroupPlan> plans{p};
+    std::vector<double> none;
+    GroupTables gt;
+    uint64_t path_elems, value_elems, model_count, part_t, part_e, count_slots;
+    int64_t bad_group, bad_node;
+    build_tables(plans, none, false, gt, path_elems, value_elems, model_count, part_t, part_e, count_slots, err,
+                 bad_group, bad_node);
+    const bool f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    const size_t real = f32 ? 4 : 8;
+    const size_t tb_bytes[] = {sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8, gt.log_drift32.size() * 4,
+                               gt.diffusion32.size() * 4, gt.cash.size() * 8, gt.node_flags.size() * 4};
+    const void *tb_src[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
+                            gt.diffusion32.data(), gt.cash.data(), gt.node_flags.data()};
+    size_t total = 0;
+    size_t offs[7];
+    for (int i = 0; i < 7; ++i) {
+        offs[i] = total;
+        total += (tb_bytes[i] + 255) & ~size_t(255);
+    }
+    HIP_TRY(ex->tables.ensure(total));
+    HIP_TRY(ex->paths.ensure(path_elems * real));
+    HIP_TRY(ex->fail_t.ensure(8));
+    std::vector<uint2> items;
+    for (uint32_t c = 0; c < gt.groups[0].chunks_t; ++c) items.push_back(make_uint2(0, c));
+    HIP_TRY(ex->items_t.ensure(items.size() * sizeof(uint2)));
+    char *tb = ex->tables.as<char>();
+    hipStream_t st = ex->stream;
+    for (int i = 0; i < 7; ++i) HIP_TRY(hipMemcpyAsync(tb + offs[i], tb_src[i], tb_bytes[i], hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(ex->fail_t.ptr, 0xFF, 8, st));
+    DevTables t;
+    std::memset(&t, 0, sizeof t);
+    t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
+    t.log_drift = reinterpret_cast<const double *>(tb + offs[1]);
+    t.diffusion = reinterpret_cast<const double *>(tb + offs[2]);
+    t.log_drift32 = reinterpret_cast<const float *>(tb + offs[3]);
+    t.diffusion32 = reinterpret_cast<const float *>(tb + offs[4]);
+    t.cash = reinterpret_cast<const double *>(tb + offs[5]);
+    t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[6]);
+    HIP_TRY(launch_simulate(st, f32, t, ex->items_t.as<uint2>(), static_cast<uint32_t>(items.size()), ex->paths.ptr, 0,
+                            ex->fail_t.as<unsigned long long>()));
+    unsigned long long failed = 0;
+    HIP_TRY(hipMemcpyAsync(&failed, ex->fail_t.ptr, 8, hipMemcpyDeviceToHost, st));
+    if (f32) {
+        std::vector<float> tmp(path_elems);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), ex->paths.ptr, path_elems * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < path_elems; ++i) out[i] = tmp[i];
+    } else {
+        HIP_TRY(hipMemcpyAsync(out, ex->paths.ptr, path_elems * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (failed != kNoError) {
+        set_fields(err, -1, -1, static_cast<int64_t>(failed), -1, 0, 0);
+        return fail(err, FOHIP_E_NON_FINITE_TRANSITION, FOHIP_PHASE_TRAINING_SIMULATION,
+                    "GBM path transition at path %llu produced a non-finite spot", failed);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_philox_blocks(fohip_executor *ex, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                   uint32_t blocks, uint32_t *out, fohip_error *err) {
+    clear_error(err);
+    if (!ex || !out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    HIP_TRY(hipSetDevice(ex->device));
+    const size_t bytes = static_cast<size_t>(paths) * blocks * 16;
+    if (bytes == 0) return FOHIP_OK;
+    HIP_TRY(ex->probe_a.ensure(bytes));
+    HIP_TRY(launch_philox_blocks(ex->stream, seed, domain, path0, paths, blocks, ex->probe_a.as<uint32_t>()));
+    HIP_TRY(hipMemcpyAsync(out, ex->probe_a.ptr, bytes, hipMemcpyDeviceToHost, ex->stream));
+    HIP_TRY(hipStreamSynchronize(ex->stream));
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_stream_normals(fohip_executor *ex, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                    uint32_t count, double *out, uint32_t *draws, fohip_error *err) {
+    clear_error(err);
+    if (!ex || !out || !draws) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    HIP_TRY(hipSetDevice(ex->device));
+    const size_t n = static_cast<size_t>(paths) * count;
+    if (n == 0) return FOHIP_OK;
+    HIP_TRY(ex->probe_a.ensure(n * 8));
+    HIP_TRY(ex->probe_b.ensure(n * 4));
+    HIP_TRY(ex->probe_c.ensure(sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ex->probe_c.ptr, 0, sizeof(int), ex->stream));
+    HIP_TRY(launch_stream_normals(ex->stream, seed, domain, path0, paths, count, ex->probe_a.as<double>(),
+                                  ex->probe_b.as<uint32_t>(), ex->probe_c.as<int>()));
+    int failed = 0;
+    HIP_TRY(hipMemcpyAsync(out, ex->probe_a.ptr, n * 8, hipMemcpyDeviceToHost, ex->stream));
+    HIP_TRY(hipMemcpyAsync(draws, ex->probe_b.ptr, n * 4, hipMemcpyDeviceToHost, ex->stream));
+    HIP_TRY(hipMemcpyAsync(&failed, ex->probe_c.ptr, sizeof(int), hipMemcpyDeviceToHost, ex->stream));
+    HIP_TRY(hipStreamSynchronize(ex->stream));
+    if (failed) return fail(err, FOHIP_E_NORMAL_REJECTION_LIMIT, 0, "random-normal rejection limit reached");
+    return FOHIP_OK;
+}

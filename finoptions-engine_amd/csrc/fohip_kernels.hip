@@ -1,0 +1,1280 @@
+// CDNA4 kernels of the LSM hot path: seeded simulation, backward-induction fit passes,
+// deterministic chunk reductions, and the fused regenerate-and-evaluate pass (with pathwise Greeks).
+//
+// Work decomposition: a work item is (group, 1024-path chunk). Fit and evaluation passes launch
+// 256-thread blocks over (work item, strike or strike tile); every thread owns paths
+// chunk*1024 + threadIdx.x + 256*k, k = 0..3, so HBM rows are read fully coalesced. All reductions
+// go thread -> wave (down-shuffle tree) -> block (fixed order) -> chunk list (fixed order in a
+// one-block-per-strike reduce kernel), which makes results independent of timing and of launch
+// tuning.
+#include <hip/hip_runtime.h>
+
+#include "fohip_device.hpp"
+#include "fohip_kernels.hpp"
+
+namespace fohip {
+
+#include "zig_tables.inc"
+
+__device__ const double g_zig_x[257] = FOHIP_ZIG_X_INIT;
+__device__ const double g_zig_f[257] = FOHIP_ZIG_F_INIT;
+
+__device__ __forceinline__ void load_zig(double *sx, double *sf) {
+    for (int i = threadIdx.x; i < 257; i += blockDim.x) {
+        sx[i] = g_zig_x[i];
+        sf[i] = g_zig_f[i];
+    }
+    __syncthreads();
+}
+
+// Path-level failures only lower the row's key; the failure kind is a function of the key's stage
+// and is decoded on the host, so racing writers cannot leave a key/code mismatch.
+__device__ __forceinline__ void record_error(RowError *e, unsigned long long key) {
+    atomicMin(&e->key, key);
+}
+
+// ------------------------------------------------------------------------ random-stream probes
+
+__global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                uint32_t blocks, uint32_t *out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<uint64_t>(paths) * blocks) return;
+    const uint32_t p = static_cast<uint32_t>(i / blocks), b = static_cast<uint32_t>(i % blocks);
+    const uint4 w = philox4x32_10(b, 0u, path0 + p, domain, static_cast<uint32_t>(seed),
+                                  static_cast<uint32_t>(seed >> 32));
+    out[i * 4 + 0] = w.x;
+    out[i * 4 + 1] = w.y;
+    out[i * 4 + 2] = w.z;
+    out[i * 4 + 3] = w.w;
+}
+
+__global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                 uint32_t count, double *out, uint32_t *draws, int *fail) {
+    __shared__ double sx[257], sf[257];
+    load_zig(sx, sf);
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= paths) return;
+    PathStream s;
+    s.init(seed, domain, path0 + p);
+    for (uint32_t i = 0; i < count; ++i) {
+        double z = 0.0;
+        uint32_t d = 0;
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, d)) atomicExch(fail, 1);
+        out[static_cast<uint64_t>(p) * count + i] = z;
+        draws[static_cast<uint64_t>(p) * count + i] = d;
+    }
+}
+
+// ------------------------------------------------------------------------------ simulation
+
+template <typename Real> struct StepCoef;
+template <> struct StepCoef<double> {
+    __device__ static double drift(const DevTables &t, uint32_t i) { return t.log_drift[i]; }
+    __device__ static double diff(const DevTables &t, uint32_t i) { return t.diffusion[i]; }
+};
+template <> struct StepCoef<float> {
+    __device__ static float drift(const DevTables &t, uint32_t i) { return t.log_drift32[i]; }
+    __device__ static float diff(const DevTables &t, uint32_t i) { return t.diffusion32[i]; }
+};
+
+// seeded.rs:399-459: one thread per path; for each interval draw a normal, advance, then apply a
+// terminal cash event before recording expiry, or record a pre-event exercise state before a
+// same-node cash event. Output is the time-major training matrix.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *__restrict__ items,
+                                                     Real *__restrict__ paths_out, int evaluation,
+                                                     unsigned long long *__restrict__ fail) {
+    __shared__ double sx[257], sf[257];
+    load_zig(sx, sf);
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t n = evaluation ? g.ne : g.nt;
+    const uint32_t p = item.y * kChunk + blockIdx.y * kBlock + threadIdx.x;
+    if (p >= n) return;
+    PathStream s;
+    s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p);
+    Real *out = paths_out + g.path_base + p;
+    Real spot = static_cast<Real>(g.S0);
+    uint32_t row = 0;
+    if (g.origin_row) {
+        out[0] = spot;
+        row = 1;
+    }
+    bool ok = true;
+    for (uint32_t i = 0; i < g.intervals; ++i) {
+        double z;
+        uint32_t draws = 0;
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
+            ok = false;
+            break;
+        }
+        Real growth;
+        const Real next = gbm_advance(spot, StepCoef<Real>::diff(t, g.step_base + i),
+                                      StepCoef<Real>::drift(t, g.step_base + i), static_cast<Real>(z), growth);
+        if (!isfinite(next)) {
+            ok = false;
+            break;
+        }
+        spot = next;
+        const uint32_t node = g.node_base + i + 1;
+        const uint32_t flags = t.node_flags[node];
+        const bool cash = flags & kNodeCash;
+        const bool terminal = flags & kNodeTerminal;
+        if (cash && terminal) spot = apply_cash<Real>(g.policy, static_cast<Real>(t.cash[node]), spot);
+        if (flags & kNodeObserved) {
+            out[static_cast<uint64_t>(row) * n] = spot;
+            row += 1;
+        }
+        if (cash && !terminal) spot = apply_cash<Real>(g.policy, static_cast<Real>(t.cash[node]), spot);
+    }
+    if (!ok) atomicMin(fail + item.x, static_cast<unsigned long long>(p));
+}
+
+// ------------------------------------------------------------------------------- fit passes
+
+__device__ __forceinline__ bool row_dead(const RowError *err, uint32_t s) {
+    return err[s].key != kNoError;
+}
+
+template <typename Real>
+__device__ __forceinline__ Real intrinsic(Real spot, Real strike) {
+    const Real v = spot - strike;
+    return v > Real(0) ? v : Real(0);
+}
+
+// Step j of backward induction over exercise row r = obs - 2 - j:
+//   j == 0: cashflow = terminal intrinsic; j > 0: apply the fitted rule of row r + 1
+//   (policy.rs:570-601), then discount from node(r + 1) to node(r) (policy.rs:1049-1063),
+//   store, and accumulate Welford moments of moneyness S/K over in-the-money paths
+//   (regression.rs:279-297).
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams fp, uint32_t step,
+                                                        const uint2 *__restrict__ items,
+                                                        const Real *__restrict__ paths,
+                                                        Real *__restrict__ values,
+                                                        const Model *__restrict__ models,
+                                                        double *__restrict__ partials,
+                                                        RowError *__restrict__ errors) {
+    __shared__ double scratch[3 * kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t local = blockIdx.y;
+    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
+    const uint32_t srow = g.strike_begin + local;
+    if (row_dead(errors, srow)) return;
+    const uint32_t r = g.obs - 2 - step;
+    const double Kd = t.strikes[srow];
+    const Real K = static_cast<Real>(Kd);
+    const uint32_t n = g.nt;
+    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const Real *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
+    Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const Model &m_next = models[g.model_base + local * g.obs + r + 1];
+    const bool apply = step > 0 && m_next.kind == kRuleRegression;
+    const Real disc = static_cast<Real>(t.disc_step[g.row_base + r]);
+    Moments acc = {0.0, 0.0, 0.0};
+    for (int k = 0; k < kPathsPerThread; ++k) {
+        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
+        if (p >= n) break;
+        Real v;
+        if (step == 0) {
+            v = intrinsic(S_next[p], K);
+        } else {
+            v = V[p];
+            if (apply) {
+                const Real s1 = S_next[p];
+                if (s1 > K) {
+                    const Real x = s1 / K;
+                    const Real cont = predict(m_next, fp.degree, x);
+                    if (!isfinite(cont)) {
+                        record_error(&errors[srow], error_key(step - 1, kStageApply, p));
+                    } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
+                        v = s1 - K;
+                    }
+                }
+            }
+        }
+        v *= disc;
+        if (!isfinite(v))
+            record_error(&errors[srow], error_key(step, kStageDiscount, p));
+        V[p] = v;
+        const Real s0 = S_r[p];
+        if (s0 > K) {
+            const Real x = s0 / K;
+            if (!isfinite(x))
+                record_error(&errors[srow], error_key(step, kStagePredictor, p));
+            welford_push(acc, static_cast<double>(x));
+        }
+    }
+    acc = block_chan(acc, scratch);
+    if (threadIdx.x == 0) {
+        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        out[0] = acc.n;
+        out[1] = acc.mean;
+        out[2] = acc.m2;
+    }
+}
+
+__device__ void fail_row(RowError *e, uint32_t step, int32_t code, int64_t index, double observed,
+                         double limit) {
+    const unsigned long long key = error_key(step, kStageRegression, 0);
+    if (key < e->key) {
+        e->key = key;
+        e->code = code;
+        e->index = index;
+        e->observed = observed;
+        e->limit = limit;
+    }
+}
+
+// Support decision and predictor location/scale of one (strike, row) (policy.rs:480-529,
+// regression.rs:315-325).
+__global__ __launch_bounds__(kBlock) void k_fit_reduce_welford(DevTables t, FitParams fp, uint32_t step,
+                                                               const uint32_t *__restrict__ rows,
+                                                               const double *__restrict__ partials,
+                                                               Model *__restrict__ models,
+                                                               RowError *__restrict__ errors) {
+    __shared__ double scratch[3 * kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    if (step >= g.fit_rows || row_dead(errors, srow)) return;
+    const uint32_t local = srow - g.strike_begin;
+    const uint32_t r = g.obs - 2 - step;
+    Moments acc = {0.0, 0.0, 0.0};
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
+        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        acc = chan_merge(acc, Moments{in[0], in[1], in[2]});
+    }
+    acc = block_chan(acc, scratch);
+    if (threadIdx.x != 0) return;
+    Model &m = models[g.model_base + local * g.obs + r];
+    const uint32_t count = static_cast<uint32_t>(acc.n);
+    const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
+    m.itm = count;
+    m.rank = 0;
+    m.condition = 0.0;
+    if (count == 0) {
+        m.kind = fp.support == 1 ? kRuleContinue : kRuleNoSupport;
+        return;
+    }
+    if (count < required) {
+        if (fp.support == 1) {
+            m.kind = kRuleContinue;
+        } else {
+            m.kind = kRuleNone;
+            fail_row(&errors[srow], step, FOHIP_E_INSUFFICIENT_OBSERVATIONS, count, required, fp.degree);
+        }
+        return;
+    }
+    double mean = 0.0, scale = 1.0;
+    if (fp.degree != 0) {
+        scale = sqrt(acc.m2 / acc.n);
+        if (!isfinite(acc.mean) || !isfinite(scale) || scale <= 0.0) {
+            m.kind = kRuleNone;
+            fail_row(&errors[srow], step, FOHIP_E_DEGENERATE_PREDICTOR, fp.degree, scale, 0.0);
+            return;
+        }
+        mean = acc.mean;
+    }
+    m.mean = mean;
+    m.scale = scale;
+    m.prev_norm = 1.0;
+    for (int k = 0; k < kMaxDegree; ++k) m.alpha[k] = m.beta[k] = 0.0;
+    for (int k = 0; k <= kMaxDegree; ++k) m.coef[k] = m.norm[k] = 0.0;
+    m.kind = kRulePending;
+}
+
+// Moments of the basis vector p_k (runtime/src/lib.rs:84-95, 141-158): p_k is rebuilt from the
+// standardized predictor with the recurrence coefficients fitted so far, so no basis vector is
+// stored between passes.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_fit_moments(DevTables t, FitParams fp, uint32_t step, int k,
+                                                        const uint2 *__restrict__ items,
+                                                        const Real *__restrict__ paths,
+                                                        const Real *__restrict__ values,
+                                                        const Model *__restrict__ models,
+                                                        double *__restrict__ partials,
+                                                        const RowError *__restrict__ errors) {
+    __shared__ double scratch[3 * kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t local = blockIdx.y;
+    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
+    const uint32_t srow = g.strike_begin + local;
+    const uint32_t r = g.obs - 2 - step;
+    const Model &m = models[g.model_base + local * g.obs + r];
+    if (row_dead(errors, srow) || m.kind != kRulePending) return;
+    const double K = t.strikes[srow];
+    const uint32_t n = g.nt;
+    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    double sums[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < kPathsPerThread; ++q) {
+        const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
+        if (p >= n) break;
+        const double s0 = static_cast<double>(S_r[p]);
+        if (!(s0 > K)) continue;
+        const double x = s0 / K;
+        const double z = fp.degree == 0 ? 0.0 : (x - m.mean) / m.scale;
+        const double y = static_cast<double>(V[p]);
+        double previous = 0.0, current = 1.0;
+        for (int j = 0; j < k; ++j) {
+            const double next = fma(z - m.alpha[j], current, -m.beta[j] * previous);
+            previous = current;
+            current = next;
+        }
+        const double sq = current * current;
+        sums[0] += sq;
+        sums[1] = fma(y, current, sums[1]);
+        sums[2] = fma(z, sq, sums[2]);
+    }
+    block_sum<3>(sums, scratch);
+    if (threadIdx.x == 0) {
+        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        out[0] = sums[0];
+        out[1] = sums[1];
+        out[2] = sums[2];
+    }
+}
+
+// ---- 5x5 conditioning diagnostic (regression.rs:540-577, 711-850), one thread per fit
+
+__device__ int orthonormal_transform(int degree, const Model &m, double *tr) {
+    double previous[5] = {0, 0, 0, 0, 0}, current[5] = {1, 0, 0, 0, 0};
+    for (int i = 0; i < 25; ++i) tr[i] = 0.0;
+    for (int k = 0; k <= degree; ++k) {
+        const double inverse_norm = 1.0 / sqrt(m.norm[k]);
+        if (!isfinite(inverse_norm)) return k + 1;
+        for (int j = 0; j <= k; ++j) tr[k * 5 + j] = current[j] * inverse_norm;
+        if (k == degree) break;
+        double next[5] = {0, 0, 0, 0, 0};
+        for (int j = 0; j <= k; ++j) {
+            next[j + 1] += current[j];
+            next[j] -= m.alpha[k] * current[j];
+            next[j] -= m.beta[k] * previous[j];
+        }
+        for (int j = 0; j < 5; ++j)
+            if (!isfinite(next[j])) return k + 2;
+        for (int j = 0; j < 5; ++j) {
+            previous[j] = current[j];
+            current[j] = next[j];
+        }
+    }
+    return 0;
+}
+
+__device__ int jacobi_singular_values(double *mat, int columns, double *sv) {
+    const double tol = 16.0 * kEpsilon;
+    bool converged = false;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        converged = true;
+        for (int left = 0; left < columns - 1; ++left) {
+            for (int right = left + 1; right < columns; ++right) {
+                double ln = 0.0, rn = 0.0, cross = 0.0;
+                for (int row = 0; row < columns; ++row) {
+                    const double lv = mat[left * 5 + row], rv = mat[right * 5 + row];
+                    ln = fma(lv, lv, ln);
+                    rn = fma(rv, rv, rn);
+                    cross = fma(lv, rv, cross);
+                }
+                if (!isfinite(ln) || !isfinite(rn) || !isfinite(cross) || ln <= 0.0 || rn <= 0.0) return 1 + right;
+                if (fabs(cross) <= tol * sqrt(ln * rn)) continue;
+                converged = false;
+                const double zeta = (rn - ln) / (2.0 * cross);
+                double tangent;
+                if (zeta == 0.0) {
+                    tangent = 1.0;
+                } else {
+                    tangent = copysign(1.0, zeta) / (fabs(zeta) + hypot(zeta, 1.0));
+                }
+                const double cosine = 1.0 / sqrt(fma(tangent, tangent, 1.0));
+                const double sine = cosine * tangent;
+                for (int row = 0; row < columns; ++row) {
+                    const double lv = mat[left * 5 + row], rv = mat[right * 5 + row];
+                    mat[left * 5 + row] = fma(cosine, lv, -sine * rv);
+                    mat[right * 5 + row] = fma(sine, lv, cosine * rv);
+                }
+            }
+        }
+        if (converged) break;
+    }
+    if (!converged) return -1;
+    for (int c = 0; c < 5; ++c) sv[c] = 0.0;
+    for (int c = 0; c < columns; ++c) {
+        double sq = 0.0;
+        for (int row = 0; row < columns; ++row) {
+            const double v = mat[c * 5 + row];
+            sq += v * v;
+        }
+        sv[c] = sqrt(sq);
+    }
+    for (int i = 1; i < columns; ++i) {   // descending insertion sort
+        const double v = sv[i];
+        int j = i - 1;
+        while (j >= 0 && sv[j] < v) {
+            sv[j + 1] = sv[j];
+            --j;
+        }
+        sv[j + 1] = v;
+    }
+    return 0;
+}
+
+// Rank gate, condition number and acceptance of a completed fit.
+__device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, uint32_t step) {
+    double tr[25], sv[5];
+    const int columns = fp.degree + 1;
+    const int bad = orthonormal_transform(fp.degree, m, tr);
+    if (bad) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_NON_FINITE_FIT, bad - 1, 0.0, 0.0);
+        return;
+    }
+    const int jr = jacobi_singular_values(tr, columns, sv);
+    if (jr < 0) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_JACOBI_NOT_CONVERGED, columns, 0.0, 0.0);
+        return;
+    }
+    if (jr > 0) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_NON_FINITE_FIT, jr - 1, 0.0, 0.0);
+        return;
+    }
+    const double largest = sv[0], threshold = fp.rank_tol * largest;
+    int rank = 0;
+    for (int c = 0; c < columns; ++c) rank += sv[c] > threshold ? 1 : 0;
+    if (rank < columns) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_RANK_DEFICIENT, rank, columns, fp.rank_tol);
+        return;
+    }
+    const double condition = largest / sv[columns - 1];
+    if (!isfinite(condition)) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_NON_FINITE_FIT, fp.degree, 0.0, 0.0);
+        return;
+    }
+    if (condition > fp.cond_limit) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_CONDITION_LIMIT, fp.degree, condition, fp.cond_limit);
+        return;
+    }
+    m.rank = rank;
+    m.condition = condition;
+    m.kind = kRuleRegression;
+}
+
+// One Stieltjes step (regression.rs:448-514) from the reduced moments of p_k.
+__device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint32_t step, int k,
+                               double norm, double projection, double weighted) {
+    if (!isfinite(norm) || !isfinite(projection) || !isfinite(weighted)) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_NON_FINITE_FIT, k, 0.0, 0.0);
+        return;
+    }
+    if (norm <= 0.0 || sqrt(norm / static_cast<double>(m.itm)) <= fp.rank_tol) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_RANK_DEFICIENT, k, fp.degree + 1, fp.rank_tol);
+        return;
+    }
+    m.norm[k] = norm;
+    m.coef[k] = projection / norm;
+    if (!isfinite(m.coef[k])) {
+        m.kind = kRuleNone;
+        fail_row(e, step, FOHIP_E_NON_FINITE_FIT, k, 0.0, 0.0);
+        return;
+    }
+    if (k < fp.degree) {
+        m.alpha[k] = weighted / norm;
+        m.beta[k] = k == 0 ? 0.0 : norm / m.prev_norm;
+        if (!isfinite(m.alpha[k]) || !isfinite(m.beta[k])) {
+            m.kind = kRuleNone;
+            fail_row(e, step, FOHIP_E_NON_FINITE_FIT, k, 0.0, 0.0);
+            return;
+        }
+        m.prev_norm = norm;
+    } else {
+        diagnose_and_accept(m, fp, e, step);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitParams fp, uint32_t step, int k,
+                                                               const uint32_t *__restrict__ rows,
+                                                               const double *__restrict__ partials,
+                                                               Model *__restrict__ models,
+                                                               RowError *__restrict__ errors) {
+    __shared__ double scratch[3 * kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    if (step >= g.fit_rows) return;
+    const uint32_t local = srow - g.strike_begin;
+    const uint32_t r = g.obs - 2 - step;
+    Model &m = models[g.model_base + local * g.obs + r];
+    if (row_dead(errors, srow) || m.kind != kRulePending) return;
+    double sums[3] = {0.0, 0.0, 0.0};
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
+        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        sums[0] += in[0];
+        sums[1] += in[1];
+        sums[2] += in[2];
+    }
+    block_sum<3>(sums, scratch);
+    if (threadIdx.x == 0) stieltjes_step(m, fp, &errors[srow], step, k, sums[0], sums[1], sums[2]);
+}
+
+// FP32-bulk fit: one pass accumulating, in FP64, the power sums sum z^j (j <= 2d) and
+// sum y z^j (j <= d) of the standardized FP32 predictor (the Hankel Gram matrix and right-hand
+// side of the normal equations).
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_fit_gram(DevTables t, FitParams fp, uint32_t step,
+                                                     const uint2 *__restrict__ items,
+                                                     const Real *__restrict__ paths,
+                                                     const Real *__restrict__ values,
+                                                     const Model *__restrict__ models,
+                                                     double *__restrict__ partials,
+                                                     const RowError *__restrict__ errors) {
+    __shared__ double scratch[14 * kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t local = blockIdx.y;
+    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
+    const uint32_t srow = g.strike_begin + local;
+    const uint32_t r = g.obs - 2 - step;
+    const Model &m = models[g.model_base + local * g.obs + r];
+    if (row_dead(errors, srow) || m.kind != kRulePending) return;
+    const Real K = static_cast<Real>(t.strikes[srow]);
+    const Real mean = static_cast<Real>(m.mean), inv_scale = Real(1) / static_cast<Real>(m.scale);
+    const uint32_t n = g.nt;
+    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    double acc[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    for (int q = 0; q < kPathsPerThread; ++q) {
+        const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
+        if (p >= n) break;
+        const Real s0 = S_r[p];
+        if (!(s0 > K)) continue;
+        const Real z = fp.degree == 0 ? Real(0) : (s0 / K - mean) * inv_scale;
+        const Real y = V[p];
+        Real pw = Real(1);
+#pragma unroll
+        for (int j = 0; j <= 8; ++j) {
+            acc[j] += static_cast<double>(pw);
+            if (j <= 4) acc[9 + j] += static_cast<double>(y * pw);
+            pw *= z;
+        }
+    }
+    block_sum<14>(acc, scratch);
+    if (threadIdx.x == 0) {
+        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+#pragma unroll
+        for (int i = 0; i < 14; ++i) out[i] = acc[i];
+    }
+}
+
+// Stieltjes recurrence from the power sums: the monic orthogonal basis p_k is carried as monomial
+// coefficients, norms and alpha use the Hankel moments, projections the right-hand side. This is
+// the LDL^T factorization of the Gram matrix in orthogonal-polynomial form, so the fitted rule has
+// exactly the representation (alpha, beta, coef) of the FP64 method.
+__global__ __launch_bounds__(kBlock) void k_fit_reduce_gram(DevTables t, FitParams fp, uint32_t step,
+                                                            const uint32_t *__restrict__ rows,
+                                                            const double *__restrict__ partials,
+                                                            Model *__restrict__ models,
+                                                            RowError *__restrict__ errors) {
+    __shared__ double scratch[14 * kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    if (step >= g.fit_rows) return;
+    const uint32_t local = srow - g.strike_begin;
+    const uint32_t r = g.obs - 2 - step;
+    Model &m = models[g.model_base + local * g.obs + r];
+    if (row_dead(errors, srow) || m.kind != kRulePending) return;
+    double acc[14];
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
+        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        for (int i = 0; i < 14; ++i) acc[i] += in[i];
+    }
+    block_sum<14>(acc, scratch);
+    if (threadIdx.x != 0) return;
+    const double *mu = acc;        // sum z^j, j = 0..8
+    const double *nu = acc + 9;    // sum y z^j, j = 0..4
+    double a_prev[5] = {0, 0, 0, 0, 0}, a_cur[5] = {1, 0, 0, 0, 0};
+    for (int k = 0; k <= fp.degree; ++k) {
+        double norm = 0.0, weighted = 0.0, projection = 0.0;
+        for (int i = 0; i <= k; ++i) {
+            projection += a_cur[i] * nu[i];
+            for (int j = 0; j <= k; ++j) {
+                norm += a_cur[i] * a_cur[j] * mu[i + j];
+                if (k < fp.degree) weighted += a_cur[i] * a_cur[j] * mu[i + j + 1];
+            }
+        }
+        stieltjes_step(m, fp, &errors[srow], step, k, norm, projection, weighted);
+        if (m.kind != kRulePending && m.kind != kRuleRegression) return;
+        if (k < fp.degree) {
+            double a_next[5] = {0, 0, 0, 0, 0};
+            for (int i = 0; i <= k; ++i) {
+                a_next[i + 1] += a_cur[i];
+                a_next[i] -= m.alpha[k] * a_cur[i];
+                a_next[i] -= m.beta[k] * a_prev[i];
+            }
+            for (int i = 0; i < 5; ++i) {
+                a_prev[i] = a_cur[i];
+                a_cur[i] = a_next[i];
+            }
+        }
+    }
+}
+
+// After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
+// cashflows for the valuation-origin decision (policy.rs:267-281).
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp, const uint2 *__restrict__ items,
+                                                      const Real *__restrict__ paths,
+                                                      const Real *__restrict__ values,
+                                                      const Model *__restrict__ models,
+                                                      double *__restrict__ partials,
+                                                      RowError *__restrict__ errors) {
+    __shared__ double scratch[kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t local = blockIdx.y;
+    if (g.strike_begin + local >= g.strike_end) return;
+    const uint32_t srow = g.strike_begin + local;
+    if (row_dead(errors, srow)) return;
+    const double Kd = t.strikes[srow];
+    const Real K = static_cast<Real>(Kd);
+    const uint32_t n = g.nt;
+    const uint32_t last_row = g.origin_row;   // smallest positive-time exercise row
+    const Real *S_last = paths + g.path_base + static_cast<uint64_t>(g.fit_rows ? last_row : g.obs - 1) * n;
+    const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const Model &m = models[g.model_base + local * g.obs + last_row];
+    const bool apply = g.fit_rows > 0 && m.kind == kRuleRegression;
+    const Real disc = static_cast<Real>(g.disc_final);
+    double sum[1] = {0.0};
+    for (int k = 0; k < kPathsPerThread; ++k) {
+        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
+        if (p >= n) break;
+        Real v;
+        if (g.fit_rows == 0) {
+            v = intrinsic(S_last[p], K);
+        } else {
+            v = V[p];
+            if (apply) {
+                const Real s1 = S_last[p];
+                if (s1 > K) {
+                    const Real cont = predict(m, fp.degree, s1 / K);
+                    if (!isfinite(cont)) {
+                        record_error(&errors[srow], error_key(g.fit_rows - 1, kStageApply, p));
+                    } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
+                        v = s1 - K;
+                    }
+                }
+            }
+        }
+        v *= disc;
+        if (!isfinite(v))
+            record_error(&errors[srow], error_key(g.fit_rows, kStageDiscount, p));
+        sum[0] += static_cast<double>(v);
+    }
+    block_sum<1>(sum, scratch);
+    if (threadIdx.x == 0 && g.origin_row) {
+        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        out[0] = sum[0];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitParams fp,
+                                                              const uint32_t *__restrict__ rows,
+                                                              const double *__restrict__ partials,
+                                                              Model *__restrict__ models,
+                                                              RowError *__restrict__ errors) {
+    __shared__ double scratch[kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    if (!g.origin_row || row_dead(errors, srow)) return;
+    const uint32_t local = srow - g.strike_begin;
+    double sum[1] = {0.0};
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock)
+        sum[0] += partials[static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride];
+    block_sum<1>(sum, scratch);
+    if (threadIdx.x != 0) return;
+    Model &m = models[g.model_base + local * g.obs];
+    const double estimate = sum[0] / static_cast<double>(g.nt);
+    const double K = t.strikes[srow];
+    const double intrinsic0 = g.S0 - K > 0.0 ? g.S0 - K : 0.0;
+    m.kind = kRuleOrigin;
+    m.estimate = estimate;
+    m.exercise = should_exercise(fp.tie, intrinsic0, estimate) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------------------- evaluation
+
+// Fused regenerate-and-evaluate (policy.rs:642-736): each thread regenerates its evaluation
+// paths from the Philox counters, applies every strike's frozen rule in row order, and reduces
+// discounted payoffs per strike. With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along
+// the same transitions and the stopping row of each strike is held fixed (pathwise estimator).
+template <typename Real, int TS, bool GREEKS>
+__global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
+                                                     const Model *__restrict__ models,
+                                                     double *__restrict__ partials,
+                                                     uint32_t *__restrict__ counts,
+                                                     uint32_t *__restrict__ nosupport,
+                                                     RowError *__restrict__ errors,
+                                                     unsigned long long *__restrict__ sim_fail,
+                                                     int use_lds_hist) {
+    extern __shared__ uint32_t hist[];   // [TS][obs] when use_lds_hist
+    __shared__ double sx[257], sf[257];
+    constexpr int NACC = GREEKS ? 10 : 2;
+    __shared__ double scratch[NACC * TS * kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t s_first = g.strike_begin + blockIdx.y * TS;
+    if (s_first >= g.strike_end) return;
+    const int ns = static_cast<int>(min(static_cast<uint32_t>(TS), g.strike_end - s_first));
+    const uint32_t obs = g.obs;
+    load_zig(sx, sf);
+    if (use_lds_hist) {
+        for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
+        __syncthreads();
+    }
+    Real K[TS];
+    bool live[TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i) {
+        const uint32_t srow = s_first + i;
+        live[i] = i < ns;
+        K[i] = Real(1);
+        if (live[i]) {
+            K[i] = static_cast<Real>(t.strikes[srow]);
+            const Model &m0 = models[g.model_base + (srow - g.strike_begin) * obs];
+            if (g.origin_row && m0.kind == kRuleOrigin && m0.exercise) live[i] = false;
+            if (errors[srow].key != kNoError) live[i] = false;
+        }
+    }
+    double acc[TS][NACC];
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) acc[i][a] = 0.0;
+    const uint32_t n = g.ne;
+    const uint32_t last = obs - 1;
+    for (int q = 0; q < kPathsPerThread; ++q) {
+        const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
+        const bool valid = p < n;
+        PathStream s;
+        s.init(g.eval_seed, g.eval_domain, p);
+        Real spot = static_cast<Real>(g.S0);
+        Real tan[4] = {Real(1), Real(0), Real(0), Real(0)};
+        uint32_t active = 0;
+#pragma unroll
+        for (int i = 0; i < TS; ++i) active |= (live[i] && valid) ? (1u << i) : 0u;
+        Real pay[TS];
+        Real dpay[TS][4];
+        uint32_t stop[TS];
+#pragma unroll
+        for (int i = 0; i < TS; ++i) {
+            pay[i] = Real(0);
+            stop[i] = last;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) dpay[i][a] = Real(0);
+        }
+        uint32_t row = g.origin_row;   // origin row holds no regression rule
+        bool ok = true;
+        for (uint32_t iv = 0; iv < g.intervals && valid; ++iv) {
+            double z;
+            uint32_t draws = 0;
+            if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
+                ok = false;
+                break;
+            }
+            const uint32_t step_idx = g.step_base + iv;
+            Real growth;
+            const Real before = spot;
+            const Real next = gbm_advance(spot, StepCoef<Real>::diff(t, step_idx), StepCoef<Real>::drift(t, step_idx),
+                                          static_cast<Real>(z), growth);
+            if (!isfinite(next)) {
+                ok = false;
+                break;
+            }
+            spot = next;
+            if (GREEKS) {
+                if (before == Real(0)) {
+                    tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
+                } else {
+                    const Real dt = static_cast<Real>(t.dt[step_idx]);
+                    const Real sdt = static_cast<Real>(t.sqrt_dt[step_idx]);
+                    tan[0] = tan[0] * growth;
+                    tan[1] = tan[1] * growth + next * (sdt * static_cast<Real>(z) - static_cast<Real>(g.sigma) * dt);
+                    tan[2] = tan[2] * growth + next * dt;
+                    tan[3] = tan[3] * growth - next * dt;
+                }
+            }
+            const uint32_t node = g.node_base + iv + 1;
+            const uint32_t flags = t.node_flags[node];
+            const bool cash = flags & kNodeCash;
+            const bool terminal = flags & kNodeTerminal;
+            if (cash && terminal) {
+                const Real amount = static_cast<Real>(t.cash[node]);
+                if (GREEKS && g.policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
+                spot = apply_cash<Real>(g.policy, amount, spot);
+            }
+            if (flags & kNodeObserved) {
+                if (row < last) {
+                    const double disc = t.disc0[g.row_base + row];
+#pragma unroll
+                    for (int i = 0; i < TS; ++i) {
+                        if (!(active & (1u << i)) || !(spot > K[i])) continue;
+                        const Model &m = models[g.model_base + (s_first + i - g.strike_begin) * obs + row];
+                        if (m.kind == kRuleRegression) {
+                            const Real intr = spot - K[i];
+                            const Real cont = predict(m, fp.degree, spot / K[i]);
+                            if (!isfinite(cont)) {
+                                record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
+                                active &= ~(1u << i);
+                                continue;
+                            }
+                            if (should_exercise(fp.tie, static_cast<double>(intr), static_cast<double>(cont))) {
+                                pay[i] = intr * static_cast<Real>(disc);
+                                stop[i] = row;
+                                active &= ~(1u << i);
+                                if (GREEKS) {
+                                    const Real dd = static_cast<Real>(disc);
+                                    dpay[i][0] = tan[0] * dd;
+                                    dpay[i][1] = tan[1] * dd;
+                                    dpay[i][2] = tan[2] * dd - static_cast<Real>(t.row_time[g.row_base + row]) * pay[i];
+                                    dpay[i][3] = tan[3] * dd;
+                                }
+                            }
+                        } else if (m.kind == kRuleNoSupport) {
+                            atomicAdd(&nosupport[g.count_base + (s_first + i - g.strike_begin) * obs + row], 1u);
+                        }
+                    }
+                }
+                row += 1;
+            }
+            if (cash && !terminal) {
+                const Real amount = static_cast<Real>(t.cash[node]);
+                if (GREEKS && g.policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
+                spot = apply_cash<Real>(g.policy, amount, spot);
+            }
+        }
+        if (!ok) {
+            atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
+            active = 0;
+        }
+        if (valid && ok) {
+            const double disc = t.disc0[g.row_base + last];
+#pragma unroll
+            for (int i = 0; i < TS; ++i) {
+                if (!(active & (1u << i))) continue;
+                const Real intr = intrinsic(spot, K[i]);
+                pay[i] = intr * static_cast<Real>(disc);
+                stop[i] = last;
+                if (GREEKS && spot > K[i]) {
+                    const Real dd = static_cast<Real>(disc);
+                    dpay[i][0] = tan[0] * dd;
+                    dpay[i][1] = tan[1] * dd;
+                    dpay[i][2] = tan[2] * dd - static_cast<Real>(t.row_time[g.row_base + last]) * pay[i];
+                    dpay[i][3] = tan[3] * dd;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < TS; ++i) {
+                if (!live[i]) continue;
+                const double v = static_cast<double>(pay[i]);
+                if (!isfinite(v))
+                    record_error(&errors[s_first + i], error_key(kEvalStepBase + stop[i], kStageEvalPayoff, p));
+                acc[i][0] += v;
+                acc[i][1] += v * v;
+                if (GREEKS) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const double d = static_cast<double>(dpay[i][a]);
+                        acc[i][2 + a] += d;
+                        acc[i][6 + a] += d * d;
+                    }
+                }
+                if (use_lds_hist)
+                    atomicAdd(&hist[i * obs + stop[i]], 1u);
+                else
+                    atomicAdd(&counts[g.count_base + (s_first + i - g.strike_begin) * obs + stop[i]], 1u);
+            }
+        }
+    }
+    double flat[TS * NACC];
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) flat[i * NACC + a] = acc[i][a];
+    block_sum<TS * NACC>(flat, scratch);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < ns; ++i) {
+            double *out = partials + static_cast<uint64_t>(g.part_base_e + (s_first + i - g.strike_begin) * g.chunks_e + item.y) * kPartialStride;
+            for (int a = 0; a < NACC; ++a) out[a] = flat[i * NACC + a];
+        }
+    }
+    if (use_lds_hist) {
+        __syncthreads();
+        for (uint32_t idx = threadIdx.x; idx < static_cast<uint32_t>(ns) * obs; idx += kBlock) {
+            const uint32_t c = hist[idx];
+            if (c) {
+                const uint32_t i = idx / obs, rr = idx % obs;
+                atomicAdd(&counts[g.count_base + (s_first + i - g.strike_begin) * obs + rr], c);
+            }
+        }
+    }
+}
+
+// Per strike: exact per-chunk (count, mean, M2) from the chunk sums, merged in chunk order
+// (policy.rs:891-926 computes the same population moments sequentially).
+__global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint32_t *__restrict__ rows,
+                                                        const double *__restrict__ partials, int greeks,
+                                                        double *__restrict__ out) {
+    __shared__ double scratch[8 * kWaves];
+    __shared__ double mscratch[3 * kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    const uint32_t local = srow - g.strike_begin;
+    Moments acc = {0.0, 0.0, 0.0};
+    double gs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c = threadIdx.x; c < g.chunks_e; c += kBlock) {
+        const double *in = partials + static_cast<uint64_t>(g.part_base_e + local * g.chunks_e + c) * kPartialStride;
+        const double nc = static_cast<double>(min(static_cast<uint32_t>(kChunk), g.ne - c * kChunk));
+        const double mean = in[0] / nc;
+        const double m2 = fmax(in[1] - in[0] * mean, 0.0);
+        acc = chan_merge(acc, Moments{nc, mean, m2});
+        if (greeks)
+            for (int a = 0; a < 8; ++a) gs[a] += in[2 + a];
+    }
+    acc = block_chan(acc, mscratch);
+    block_sum<8>(gs, scratch);
+    if (threadIdx.x == 0) {
+        double *o = out + static_cast<uint64_t>(srow) * 16;
+        const double nn = static_cast<double>(g.ne);
+        o[0] = acc.mean;
+        o[1] = acc.m2;
+        o[2] = nn;
+        for (int a = 0; a < 4; ++a) {
+            const double mean = gs[a] / nn;
+            const double var = fmax((gs[4 + a] - gs[a] * mean) / (nn - 1.0), 0.0);
+            o[3 + a] = mean;
+            o[7 + a] = sqrt(var / nn);
+        }
+    }
+}
+
+
+// Per strike row: the 12-column summary of try_from_parts (batch/pricing.rs:711-812) computed
+// from the fitted rules and stopping counts, plus the first evaluation row whose declared
+// missing-support rule met an active in-the-money path (policy.rs:684-703).
+__global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model *__restrict__ models,
+                                                        const uint32_t *__restrict__ counts,
+                                                        const uint32_t *__restrict__ nosupport,
+                                                        uint32_t rows, RowSummary *__restrict__ out) {
+    const uint32_t srow = blockIdx.x * blockDim.x + threadIdx.x;
+    if (srow >= rows) return;
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    const uint32_t local = srow - g.strike_begin;
+    RowSummary rs;
+    rs.origin_fitted = 0;
+    rs.origin_exercised = 0;
+    rs.regression_nodes = 0;
+    rs.no_support_nodes = 0;
+    rs.insufficient_nodes = 0;
+    rs.min_itm = 0xFFFFFFFFu;
+    rs.max_condition = 0.0;
+    rs.early = 0;
+    rs.expiry = 0;
+    rs.nosupport_row = -1;
+    rs.nosupport_count = 0;
+    rs.origin_estimate = 0.0;
+    for (uint32_t r = 0; r + 1 < g.obs; ++r) {
+        const Model &m = models[g.model_base + local * g.obs + r];
+        switch (m.kind) {
+            case kRuleOrigin:
+                rs.origin_fitted = 1;
+                rs.origin_exercised = m.exercise ? 1 : 0;
+                rs.origin_estimate = m.estimate;
+                break;
+            case kRuleNoSupport:
+                rs.no_support_nodes += 1;
+                break;
+            case kRuleContinue:
+                rs.insufficient_nodes += 1;
+                break;
+            case kRuleRegression:
+                rs.regression_nodes += 1;
+                rs.min_itm = min(rs.min_itm, m.itm);
+                rs.max_condition = fmax(rs.max_condition, m.condition);
+                break;
+            default:
+                break;
+        }
+        rs.early += counts[g.count_base + local * g.obs + r];
+        if (rs.nosupport_row < 0 && nosupport[g.count_base + local * g.obs + r] > 0) {
+            rs.nosupport_row = static_cast<int32_t>(r);
+            rs.nosupport_count = nosupport[g.count_base + local * g.obs + r];
+        }
+    }
+    if (rs.regression_nodes == 0) rs.min_itm = 0;
+    rs.expiry = counts[g.count_base + local * g.obs + g.obs - 1];
+    out[srow] = rs;
+}
+
+// Serial replay of one failed path (the reference replays the lowest failing index serially to
+// report a deterministic error, seeded.rs:155-186): out = {interval, spot, normal, kind}.
+__global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint32_t path, double *out) {
+    __shared__ double sx[257], sf[257];
+    load_zig(sx, sf);
+    if (threadIdx.x != 0) return;
+    const DevGroup &g = t.groups[group];
+    PathStream s;
+    s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, path);
+    double spot = g.S0;
+    out[0] = -1.0;
+    for (uint32_t i = 0; i < g.intervals; ++i) {
+        double z;
+        uint32_t draws = 0;
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
+            out[0] = i;
+            out[1] = spot;
+            out[2] = 0.0;
+            out[3] = 1.0;
+            return;
+        }
+        double growth;
+        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth);
+        if (!isfinite(next)) {
+            out[0] = i;
+            out[1] = spot;
+            out[2] = z;
+            out[3] = 0.0;
+            return;
+        }
+        spot = next;
+        const uint32_t node = g.node_base + i + 1;
+        if (t.node_flags[node] & kNodeCash) spot = apply_cash<double>(g.policy, t.cash[node], spot);
+    }
+}
+
+// ------------------------------------------------------------------------------ launchers
+
+namespace {
+inline dim3 grid2(uint32_t x, uint32_t y) { return dim3(x, y, 1); }
+}
+
+hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
+                                uint32_t paths, uint32_t blocks, uint32_t *out) {
+    const uint64_t total = static_cast<uint64_t>(paths) * blocks;
+    if (total == 0) return hipSuccess;
+    const uint32_t nb = static_cast<uint32_t>((total + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_philox_blocks, dim3(nb), dim3(kBlock), 0, st, seed, domain, path0, paths, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
+                                 uint32_t paths, uint32_t count, double *out, uint32_t *draws,
+                                 int *fail) {
+    if (paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_normals, dim3((paths + kBlock - 1) / kBlock), dim3(kBlock), 0, st, seed, domain,
+                       path0, paths, count, out, draws, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const uint2 *items,
+                           uint32_t nitems, void *paths, int evaluation, unsigned long long *fail) {
+    if (nitems == 0) return hipSuccess;
+    const dim3 grid = grid2(nitems, kChunk / kBlock);
+    if (f32)
+        hipLaunchKernelGGL(k_simulate<float>, grid, dim3(kBlock), 0, st, t, items, static_cast<float *>(paths),
+                           evaluation, fail);
+    else
+        hipLaunchKernelGGL(k_simulate<double>, grid, dim3(kBlock), 0, st, t, items, static_cast<double *>(paths),
+                           evaluation, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_prepare(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                              uint32_t step, const uint2 *items, uint32_t nitems,
+                              uint32_t max_strikes, const void *paths, void *values,
+                              const Model *models, double *partials, RowError *errors) {
+    if (nitems == 0) return hipSuccess;
+    const dim3 grid = grid2(nitems, max_strikes);
+    if (f32)
+        hipLaunchKernelGGL(k_fit_prepare<float>, grid, dim3(kBlock), 0, st, t, fp, step, items,
+                           static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
+    else
+        hipLaunchKernelGGL(k_fit_prepare<double>, grid, dim3(kBlock), 0, st, t, fp, step, items,
+                           static_cast<const double *>(paths), static_cast<double *>(values), models, partials,
+                           errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                     uint32_t step, const uint32_t *rows, uint32_t nrows,
+                                     const double *partials, Model *models, RowError *errors) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_reduce_welford, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
+                       errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_moments(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                              uint32_t step, int k, const uint2 *items, uint32_t nitems,
+                              uint32_t max_strikes, const void *paths, const void *values,
+                              const Model *models, double *partials, const RowError *errors) {
+    if (nitems == 0) return hipSuccess;
+    const dim3 grid = grid2(nitems, max_strikes);
+    if (f32)
+        hipLaunchKernelGGL(k_fit_moments<float>, grid, dim3(kBlock), 0, st, t, fp, step, k, items,
+                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
+                           errors);
+    else
+        hipLaunchKernelGGL(k_fit_moments<double>, grid, dim3(kBlock), 0, st, t, fp, step, k, items,
+                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
+                           partials, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                     uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
+                                     const double *partials, Model *models, RowError *errors) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_reduce_moments, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, k, rows, partials,
+                       models, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_gram(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                           uint32_t step, const uint2 *items, uint32_t nitems,
+                           uint32_t max_strikes, const void *paths, const void *values,
+                           const Model *models, double *partials, const RowError *errors) {
+    if (nitems == 0) return hipSuccess;
+    const dim3 grid = grid2(nitems, max_strikes);
+    if (f32)
+        hipLaunchKernelGGL(k_fit_gram<float>, grid, dim3(kBlock), 0, st, t, fp, step, items,
+                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
+                           errors);
+    else
+        hipLaunchKernelGGL(k_fit_gram<double>, grid, dim3(kBlock), 0, st, t, fp, step, items,
+                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
+                           partials, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                  uint32_t step, const uint32_t *rows, uint32_t nrows,
+                                  const double *partials, Model *models, RowError *errors) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_reduce_gram, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
+                       errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                            const uint2 *items, uint32_t nitems, uint32_t max_strikes,
+                            const void *paths, const void *values, const Model *models,
+                            double *partials, RowError *errors) {
+    if (nitems == 0) return hipSuccess;
+    const dim3 grid = grid2(nitems, max_strikes);
+    if (f32)
+        hipLaunchKernelGGL(k_fit_final<float>, grid, dim3(kBlock), 0, st, t, fp, items,
+                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
+                           errors);
+    else
+        hipLaunchKernelGGL(k_fit_final<double>, grid, dim3(kBlock), 0, st, t, fp, items,
+                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
+                           partials, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                    const uint32_t *rows, uint32_t nrows, const double *partials,
+                                    Model *models, RowError *errors) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_reduce_origin, dim3(nrows), dim3(kBlock), 0, st, t, fp, rows, partials, models, errors);
+    return hipGetLastError();
+}
+
+uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
+    const uint32_t options_plain[] = {1, 4, 16};
+    const uint32_t options_greeks[] = {1, 4, 8};
+    const uint32_t *opts = greeks ? options_greeks : options_plain;
+    uint32_t want = requested ? requested : (max_strikes <= 1 ? 1 : (greeks ? 8 : 16));
+    uint32_t tile = opts[0];
+    for (int i = 0; i < 3; ++i)
+        if (opts[i] <= want) tile = opts[i];
+    return tile;
+}
+
+template <typename Real, int TS, bool G>
+static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
+                                uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
+                                double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
+                                unsigned long long *sim_fail) {
+    const uint32_t tiles = (max_strikes + TS - 1) / TS;
+    const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
+    const int use_lds = hist_bytes <= 32768 ? 1 : 0;
+    hipLaunchKernelGGL((k_evaluate<Real, TS, G>), dim3(nitems, tiles), dim3(kBlock), use_lds ? hist_bytes : 0, st,
+                       t, fp, items, models, partials, counts, nosupport, errors, sim_fail, use_lds);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, const DevTables &t,
+                                const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
+                                uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
+                                uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
+#define FOHIP_EVAL_CASE(TSV, GV)                                                                              \
+    if (tile == TSV && greeks == GV)                                                                          \
+        return launch_eval_t<Real, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, \
+                                            counts, nosupport, errors, sim_fail);
+    FOHIP_EVAL_CASE(1, false)
+    FOHIP_EVAL_CASE(4, false)
+    FOHIP_EVAL_CASE(16, false)
+    FOHIP_EVAL_CASE(1, true)
+    FOHIP_EVAL_CASE(4, true)
+    FOHIP_EVAL_CASE(8, true)
+#undef FOHIP_EVAL_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
+                           const DevTables &t, const FitParams &fp, const uint2 *items,
+                           uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
+                           const Model *models, double *partials, uint32_t *counts,
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
+    if (nitems == 0) return hipSuccess;
+    if (f32)
+        return launch_eval_r<float>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
+                                    counts, nosupport, errors, sim_fail);
+    return launch_eval_r<double>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
+                                 counts, nosupport, errors, sim_fail);
+}
+
+hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models,
+                              const uint32_t *counts, const uint32_t *nosupport, uint32_t rows,
+                              RowSummary *out) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_summary, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, models, counts,
+                       nosupport, rows, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_replay_path(hipStream_t st, const DevTables &t, uint32_t group, int evaluation,
+                              uint32_t path, double *out) {
+    hipLaunchKernelGGL(k_replay_path, dim3(1), dim3(64), 0, st, t, group, evaluation, path, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_reduce(hipStream_t st, const DevTables &t, const uint32_t *rows,
+                              uint32_t nrows, const double *partials, int greeks, double *out) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_reduce, dim3(nrows), dim3(kBlock), 0, st, t, rows, partials, greeks, out);
+    return hipGetLastError();
+}
+
+}  // namespace fohip

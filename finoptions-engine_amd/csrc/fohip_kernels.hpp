@@ -1,0 +1,73 @@
+// Host-callable launchers of the LSM kernels (implemented in fohip_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fohip.h"
+#include "fohip_layout.hpp"
+
+namespace fohip {
+
+struct FitParams {
+    int32_t degree;
+    int32_t tie;        // 0 strictly greater, 1 greater or equal
+    int32_t support;    // 0 fail, 1 continue
+    int32_t pad;
+    double rank_tol;
+    double cond_limit;
+};
+
+hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
+                                uint32_t paths, uint32_t blocks, uint32_t *out);
+hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
+                                 uint32_t paths, uint32_t count, double *out, uint32_t *draws,
+                                 int *fail);
+hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const uint2 *items,
+                           uint32_t nitems, void *paths, int evaluation, unsigned long long *fail);
+hipError_t launch_fit_prepare(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                              uint32_t step, const uint2 *items, uint32_t nitems,
+                              uint32_t max_strikes, const void *paths, void *values,
+                              const Model *models, double *partials, RowError *errors);
+hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                     uint32_t step, const uint32_t *rows, uint32_t nrows,
+                                     const double *partials, Model *models, RowError *errors);
+hipError_t launch_fit_moments(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                              uint32_t step, int k, const uint2 *items, uint32_t nitems,
+                              uint32_t max_strikes, const void *paths, const void *values,
+                              const Model *models, double *partials, const RowError *errors);
+hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                     uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
+                                     const double *partials, Model *models, RowError *errors);
+hipError_t launch_fit_gram(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                           uint32_t step, const uint2 *items, uint32_t nitems,
+                           uint32_t max_strikes, const void *paths, const void *values,
+                           const Model *models, double *partials, const RowError *errors);
+hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                  uint32_t step, const uint32_t *rows, uint32_t nrows,
+                                  const double *partials, Model *models, RowError *errors);
+hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                            const uint2 *items, uint32_t nitems, uint32_t max_strikes,
+                            const void *paths, const void *values, const Model *models,
+                            double *partials, RowError *errors);
+hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
+                                    const uint32_t *rows, uint32_t nrows, const double *partials,
+                                    Model *models, RowError *errors);
+hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
+                           const DevTables &t, const FitParams &fp, const uint2 *items,
+                           uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
+                           const Model *models, double *partials, uint32_t *counts,
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail);
+hipError_t launch_eval_reduce(hipStream_t st, const DevTables &t, const uint32_t *rows,
+                              uint32_t nrows, const double *partials, int greeks, double *out);
+
+hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models,
+                              const uint32_t *counts, const uint32_t *nosupport, uint32_t rows,
+                              RowSummary *out);
+hipError_t launch_replay_path(hipStream_t st, const DevTables &t, uint32_t group, int evaluation,
+                              uint32_t path, double *out);
+
+// Strike tiles available to the evaluation kernel for a given Greeks setting.
+uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested);
+
+}  // namespace fohip

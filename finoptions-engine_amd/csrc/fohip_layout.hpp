@@ -1,0 +1,130 @@
+// HBM data layout shared by the host engine and the kernels.
+//
+// One "group" is one simulated market state (a single call is a group with one strike). Per group:
+//   training paths   time-major [obs][nt] Real at path_base (coalesced per observation row)
+//   values           [strikes][nt] Real at value_base: the backward-induction cashflow per strike
+//   models           [strikes][obs] Model at model_base: fitted rule per (strike, observation row)
+//   partials         [strikes][chunks] x kPartialStride doubles: per-1024-path chunk reductions
+// Step, node and row tables are concatenated over groups and indexed by the *_base fields.
+#pragma once
+
+#include <stdint.h>
+
+namespace fohip {
+
+constexpr int kChunk = 1024;            // paths per reduction chunk (fixes every reduction tree)
+constexpr int kPathsPerThread = 4;      // kChunk / threads per block
+constexpr int kPartialStride = 16;      // doubles per chunk partial
+constexpr int kMaxDegree = 4;
+
+// Rule kinds of one (strike, observation row).
+// kRuleNone is zero so a cleared model buffer starts with no rule.
+enum RuleKind : int32_t {
+    kRuleNone = 0,            // no rule (terminal row, failed fit, or not yet fitted)
+    kRuleOrigin = 1,          // common valuation-origin decision
+    kRuleNoSupport = 2,       // no in-the-money training state (declared Fail policy)
+    kRuleContinue = 3,        // declared continuation without regression
+    kRuleRegression = 4,      // fitted continuation regression
+    kRulePending = 5          // regression being fitted
+};
+
+struct Model {
+    double mean, scale;
+    double alpha[kMaxDegree], beta[kMaxDegree], coef[kMaxDegree + 1], norm[kMaxDegree + 1];
+    double condition;
+    double prev_norm;
+    double estimate;          // origin decision: estimated continuation
+    uint32_t itm;             // in-the-money training paths
+    int32_t kind;
+    int32_t rank;
+    int32_t exercise;         // origin decision
+};
+
+struct DevGroup {
+    double S0;
+    double sigma;             // constant volatility (pathwise vega)
+    double disc_final;        // exp(logD[first positive exercise node or expiry])
+    uint64_t train_seed, eval_seed;
+    uint64_t path_base;       // element offset of the training matrix
+    uint64_t value_base;      // element offset of the first strike's value column
+    uint32_t train_domain, eval_domain;
+    uint32_t nt, ne;          // training / evaluation paths
+    uint32_t intervals;       // transitions (normals per path)
+    uint32_t obs;             // observation rows: exercise rows + expiry
+    uint32_t origin_row;      // 1 when observation row 0 is the valuation origin
+    uint32_t fit_rows;        // positive-time exercise rows = obs - 1 - origin_row
+    uint32_t step_base;       // interval tables
+    uint32_t node_base;       // node tables (intervals + 1 entries)
+    uint32_t row_base;        // observation-row tables
+    uint32_t strike_begin, strike_end;
+    uint32_t model_base;      // Model index of (strike_begin, row 0)
+    uint32_t chunks_t, chunks_e;
+    uint32_t part_base_t;     // partial slot of (strike_begin, chunk 0), training
+    uint32_t part_base_e;     // evaluation partial slot base
+    uint32_t count_base;      // exercise-count slot of (strike_begin, row 0)
+    int32_t policy;           // cash-dividend policy
+    uint32_t pad;
+};
+
+// Node flags
+constexpr uint32_t kNodeObserved = 1u;
+constexpr uint32_t kNodeTerminal = 2u;
+constexpr uint32_t kNodeCash = 4u;
+
+struct DevTables {
+    const DevGroup *groups;
+    const double *log_drift;      // [interval]
+    const double *diffusion;      // [interval]
+    const float *log_drift32;
+    const float *diffusion32;
+    const double *dt;             // [interval]
+    const double *sqrt_dt;        // [interval]
+    const double *cash;           // [node] amount (0 when absent)
+    const uint32_t *node_flags;   // [node]
+    const double *disc0;          // [row] exp(logD[node_row])
+    const double *disc_step;      // [row] exp(logD[node_{row+1}] - logD[node_row])
+    const double *row_time;       // [row] t(node_row)
+    const uint32_t *row_node;     // [row]
+    const double *strikes;        // [strike row]
+    const uint32_t *strike_group; // [strike row]
+};
+
+// Per-strike-row failure record: the smallest key wins (earliest failure in processing order).
+struct RowError {
+    unsigned long long key;       // (step << 44) | (stage << 36) | path  (path < 2^36)
+    int32_t code;
+    int32_t pad;
+    double observed, limit;
+    int64_t index;
+};
+
+constexpr unsigned long long kNoError = ~0ull;
+constexpr uint32_t kEvalStepBase = 0x80000u;   // evaluation failures: step field = base + row
+
+struct RowSummary {
+    double max_condition;
+    double origin_estimate;
+    uint32_t regression_nodes, no_support_nodes, insufficient_nodes, min_itm;
+    uint32_t early, expiry;
+    int32_t nosupport_row;
+    uint32_t nosupport_count;
+    uint8_t origin_fitted, origin_exercised;
+    uint8_t pad[6];
+};
+
+__host__ __device__ inline unsigned long long error_key(uint32_t step, uint32_t stage, uint64_t path) {
+    return (static_cast<unsigned long long>(step) << 44) | (static_cast<unsigned long long>(stage) << 36) |
+           (path & ((1ull << 36) - 1));
+}
+
+// stages inside one backward step, in the reference's processing order
+enum Stage : uint32_t {
+    kStageDiscount = 1,       // discount_values: non-finite cashflow
+    kStagePredictor = 2,      // non-finite moneyness
+    kStageRegression = 3,     // regression / support failure (reduce kernels)
+    kStageApply = 4,          // training-rule prediction failure
+    kStageEvalPrediction = 5, // evaluation: non-finite continuation prediction
+    kStageEvalPayoff = 6      // evaluation: non-finite discounted payoff
+};
+
+}  // namespace fohip

@@ -1,0 +1,85 @@
+"""Arrow-style ragged cash-dividend schedules consumed by batched LSM pricing.
+
+Mirrors ``RaggedCashDividends`` and ``MaturityBoundary`` of python/finoptions/dividends.py
+(validation rules of crates/finoptions-batch/src/view.rs:48-254).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from enum import Enum
+from typing import cast
+
+import numpy as np
+import numpy.typing as npt
+
+from ._lib import FohipError, InputError, lib, ptr, raise_for
+
+Float64Vector = npt.NDArray[np.float64]
+Int64Vector = npt.NDArray[np.int64]
+
+
+def validate_vector(value: object, *, field: str, dtype: str) -> None:
+    """Require a 1-D C-contiguous NumPy array of an exact dtype (no implicit copies)."""
+    if type(value) is not np.ndarray:
+        raise InputError(f"{field} must be a plain numpy.ndarray")
+    array = cast("npt.NDArray[np.generic]", value)
+    expected = np.dtype(dtype)
+    if array.dtype != expected:
+        raise InputError(f"{field} must have dtype {expected}; observed {array.dtype}")
+    if array.ndim != 1:
+        raise InputError(f"{field} must be one-dimensional; observed shape {array.shape}")
+    if not array.flags.c_contiguous:
+        raise InputError(f"{field} must be C-contiguous; implicit copies are forbidden")
+
+
+class MaturityBoundary(str, Enum):
+    """Whether an event exactly at option maturity belongs to a model input."""
+
+    STRICTLY_BEFORE = "strictly_before"
+    AT_OR_BEFORE = "at_or_before"
+
+
+@dataclass(frozen=True, slots=True)
+class RaggedCashDividends:
+    """Borrowable Arrow-LargeList-style offsets, times, and exact cash amounts."""
+
+    offsets: Int64Vector
+    times: Float64Vector
+    amounts: Float64Vector
+
+    def __post_init__(self) -> None:
+        validate_vector(self.offsets, field="offsets", dtype="int64")
+        validate_vector(self.times, field="times", dtype="float64")
+        validate_vector(self.amounts, field="amounts", dtype="float64")
+        if self.times.size != self.amounts.size:
+            raise InputError(
+                f"cash-dividend times ({self.times.size}) and amounts ({self.amounts.size}) differ")
+        err = FohipError()
+        code = lib().fohip_validate_ragged_offsets(ptr(self.offsets, C.c_int64), self.offsets.size,
+                                                   self.times.size, C.byref(err))
+        raise_for(code, err)
+        base = int(self.offsets[0])
+        for row in range(self.rows):
+            a, b = int(self.offsets[row]) - base, int(self.offsets[row + 1]) - base
+            t = self.times[a:b]
+            if t.size and (not np.all(np.isfinite(t)) or np.any(t <= 0.0)):
+                raise InputError(f"row {row} cash-dividend times must be finite and positive")
+            if t.size > 1 and np.any(np.diff(t) < 0.0):
+                raise InputError(f"row {row} cash-dividend times must be nondecreasing")
+            amt = self.amounts[a:b]
+            if amt.size and (not np.all(np.isfinite(amt)) or np.any(amt < 0.0)
+                             or np.any((amt == 0.0) & np.signbit(amt))):
+                raise InputError(f"row {row} cash-dividend amounts must be finite and nonnegative")
+
+    @property
+    def rows(self) -> int:
+        return int(self.offsets.size - 1)
+
+    @property
+    def events(self) -> int:
+        return int(self.times.size)
+
+
+__all__ = ["MaturityBoundary", "RaggedCashDividends", "validate_vector"]

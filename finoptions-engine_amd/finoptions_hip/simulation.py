@@ -1,0 +1,269 @@
+"""Explicit HIP execution resource, random streams and seeded path export.
+
+``HipExecutor`` is the MI355X counterpart of the reference ``CpuExecutor``
+(python/finoptions/simulation.py:40-62): an explicit, immutable resource naming one device and one
+numerical method. Construction fails when the device is absent; nothing falls back to the CPU.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+from dataclasses import dataclass, field
+from enum import Enum
+from itertools import pairwise
+from typing import Final
+
+import numpy as np
+import numpy.typing as npt
+
+from ._lib import (
+    ExecutionError,
+    FohipError,
+    FohipKernelStat,
+    FohipPathPlan,
+    FohipTuning,
+    InputError,
+    SimulationError,
+    lib,
+    ptr,
+    raise_for,
+)
+from .model import (
+    ConstantVolatility,
+    CumulativeDividendYieldCurve,
+    DividendPolicy,
+    EventSchedule,
+    LogDiscountCurve,
+)
+
+Float64Matrix = npt.NDArray[np.float64]
+_MAX_SEED: Final = (2**64) - 1
+_MAX_DOMAIN: Final = (2**32) - 1
+_MAX_PATHS: Final = (2**32) - 1
+
+
+class HipPrecision(str, Enum):
+    """Numerical method of a HIP executor.
+
+    ``F64``: every path, payoff, regression pass and reduction in binary64 (the plain method that
+    matches the reference CPU path). ``F32_BULK``: paths, payoffs and basis evaluation in binary32;
+    moment/Gram accumulation, the per-node solve and price/SE reductions in binary64.
+    """
+
+    F64 = "f64"
+    F32_BULK = "f32_bulk"
+
+
+_PRECISION_CODE = {HipPrecision.F64: 0, HipPrecision.F32_BULK: 1}
+
+
+class _Handle:
+    """Owns one native executor; destroyed with its Python owner."""
+
+    def __init__(self, device: int, precision: HipPrecision) -> None:
+        handle = C.c_void_p()
+        err = FohipError()
+        code = lib().fohip_executor_create(device, _PRECISION_CODE[precision], C.byref(handle),
+                                           C.byref(err))
+        raise_for(code, err, lsm=False)
+        self.ptr = handle
+        self._finalizer = weakref.finalize(self, lib().fohip_executor_destroy, handle)
+
+
+@dataclass(frozen=True, slots=True)
+class HipExecutor:
+    """Reusable MI355X resource with an exact device and numerical method.
+
+    Parameters mirror ``CpuExecutor(threads, math_backend)``: ``device`` is the HIP device index and
+    ``precision`` the explicit :class:`HipPrecision`. Neither is inferred or substituted.
+    """
+
+    device: int
+    precision: HipPrecision = HipPrecision.F64
+    _handle: _Handle = field(init=False, repr=False, compare=False)
+
+    def __post_init__(self) -> None:
+        if type(self.device) is not int or self.device < 0:
+            msg = f"device must be a nonnegative plain int; observed {self.device!r}"
+            raise InputError(msg)
+        if type(self.precision) is not HipPrecision:
+            msg = f"precision must be HipPrecision; observed {type(self.precision).__name__}"
+            raise InputError(msg)
+        object.__setattr__(self, "_handle", _Handle(self.device, self.precision))
+
+    @property
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        lib().fohip_executor_info(self._handle.ptr, None, None, buf, 256)
+        return buf.value.decode()
+
+    def profile(self, enabled: bool) -> None:
+        """Enable per-kernel HIP-event timing on this executor's stream (resets the counters)."""
+        lib().fohip_reset_profile(self._handle.ptr)
+        lib().fohip_set_profiling(self._handle.ptr, 1 if enabled else 0)
+
+    def kernel_stats(self) -> dict[str, tuple[int, float]]:
+        """Per-kernel (launches, total milliseconds) since profiling was enabled."""
+        n = C.c_size_t(0)
+        lib().fohip_get_profile(self._handle.ptr, None, 0, C.byref(n))
+        buf = (FohipKernelStat * max(n.value, 1))()
+        lib().fohip_get_profile(self._handle.ptr, buf, n.value, C.byref(n))
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms))
+                for i in range(n.value)}
+
+    def tune(self, *, eval_strike_tile: int = 0) -> None:
+        """Set launch parameters; results are bitwise invariant to them."""
+        t = FohipTuning(int(eval_strike_tile), 0)
+        if lib().fohip_set_tuning(self._handle.ptr, C.byref(t)) != 0:
+            raise InputError(f"unsupported tuning eval_strike_tile={eval_strike_tile}")
+
+
+class CpuExecutor:
+    """The reference's CPU resource is not provided by this package.
+
+    Pricing requests name their executor explicitly; this package runs only on HIP devices and
+    never substitutes one resource for another.
+    """
+
+    def __init__(self, *args: object, **kwargs: object) -> None:
+        msg = ("finoptions_hip executes on HIP devices only; use HipExecutor(device=..., "
+               "precision=...). No CPU executor or fallback exists in this package.")
+        raise ExecutionError(msg)
+
+
+@dataclass(frozen=True, slots=True)
+class RandomStream:
+    """Explicit reproducible Philox namespace for one statistically distinct path set."""
+
+    seed: int
+    domain: int
+
+    def __post_init__(self) -> None:
+        if type(self.seed) is not int or not 0 <= self.seed <= _MAX_SEED:
+            msg = f"seed must be a plain int in [0, {_MAX_SEED}]; observed {self.seed!r}"
+            raise InputError(msg)
+        if type(self.domain) is not int or not 0 <= self.domain <= _MAX_DOMAIN:
+            msg = f"domain must be a plain int in [0, {_MAX_DOMAIN}]; observed {self.domain!r}"
+            raise InputError(msg)
+
+
+@dataclass(frozen=True, slots=True)
+class GbmPaths:
+    """Owned time-major pre-event states at exercise nodes and expiry."""
+
+    observation_nodes: tuple[int, ...]
+    states: Float64Matrix
+
+    def __post_init__(self) -> None:
+        if (type(self.observation_nodes) is not tuple or not self.observation_nodes
+                or any(type(n) is not int or n < 0 or n > _MAX_DOMAIN
+                       for n in self.observation_nodes)):
+            raise InputError("observation_nodes must be an immutable tuple[int, ...]")
+        if any(right <= left for left, right in pairwise(self.observation_nodes)):
+            raise InputError("observation_nodes must be strictly increasing")
+        if type(self.states) is not np.ndarray or self.states.dtype != np.float64:
+            raise InputError("states must be a float64 numpy.ndarray")
+        if self.states.ndim != 2 or self.states.shape[0] != len(self.observation_nodes):
+            raise InputError(
+                f"states must have shape [observation_nodes, paths]; observed {self.states.shape!r}")
+
+    @property
+    def path_count(self) -> int:
+        return self.states.shape[1]
+
+    def values_at(self, node: int) -> npt.NDArray[np.float64]:
+        if type(node) is not int:
+            raise InputError("node must be a plain int")
+        return self.states[self.observation_nodes.index(node)]
+
+
+def simulate_gbm_seeded(  # noqa: PLR0913 - financial inputs remain independently typed
+    *,
+    initial_spot: float,
+    schedule: EventSchedule,
+    curve: LogDiscountCurve,
+    dividend_yield_curve: CumulativeDividendYieldCurve,
+    volatility: ConstantVolatility,
+    dividend_policy: DividendPolicy,
+    paths: int,
+    random_stream: RandomStream,
+    executor: HipExecutor,
+) -> GbmPaths:
+    """Generate seeded GBM paths on the device and return the time-major observation matrix.
+
+    Mirrors the reference ``simulate_gbm_seeded`` (python/finoptions/simulation.py:262-332).
+    """
+    if type(initial_spot) is not float:
+        raise InputError("initial_spot must be a plain float")
+    for name, value, cls in (("schedule", schedule, EventSchedule), ("curve", curve, LogDiscountCurve),
+                             ("dividend_yield_curve", dividend_yield_curve,
+                              CumulativeDividendYieldCurve),
+                             ("volatility", volatility, ConstantVolatility),
+                             ("dividend_policy", dividend_policy, DividendPolicy),
+                             ("random_stream", random_stream, RandomStream),
+                             ("executor", executor, HipExecutor)):
+        if type(value) is not cls:
+            raise InputError(f"{name} must be {cls.__name__}; observed {type(value).__name__}")
+    if type(paths) is not int or not 1 <= paths <= _MAX_PATHS:
+        raise InputError(f"paths must be a plain int in [1, {_MAX_PATHS}]; observed {paths!r}")
+    if schedule.times != curve.times or schedule.times != dividend_yield_curve.times:
+        raise SimulationError("discount curve and event schedule must use the identical time grid")
+    times = np.ascontiguousarray(schedule.times, dtype=np.float64)
+    logd = np.ascontiguousarray(curve.log_discounts, dtype=np.float64)
+    q = np.ascontiguousarray(dividend_yield_curve.cumulative_yields, dtype=np.float64)
+    dn = np.ascontiguousarray([d.node for d in schedule.dividends] or [0], dtype=np.uint32)
+    da = np.ascontiguousarray([d.amount for d in schedule.dividends] or [0.0], dtype=np.float64)
+    en = np.ascontiguousarray(list(schedule.exercise_nodes) or [0], dtype=np.uint32)
+    plan = FohipPathPlan(ptr(times, C.c_double), times.size, ptr(logd, C.c_double),
+                         ptr(q, C.c_double), ptr(dn, C.c_uint32), ptr(da, C.c_double),
+                         len(schedule.dividends), ptr(en, C.c_uint32), len(schedule.exercise_nodes),
+                         schedule.expiry_node, volatility.annualized_decimal,
+                         0 if dividend_policy is DividendPolicy.LIQUIDATOR else 1)
+    obs = len(schedule.exercise_nodes) + 1
+    out = np.empty((obs, paths), dtype=np.float64)
+    err = FohipError()
+    code = lib().fohip_simulate_gbm_seeded(executor._handle.ptr, C.byref(plan), initial_spot,  # noqa: SLF001
+                                           paths, random_stream.seed, random_stream.domain,
+                                           ptr(out, C.c_double), C.byref(err))
+    raise_for(code, err, lsm=False)
+    out.flags.writeable = False
+    return GbmPaths(observation_nodes=tuple(schedule.exercise_nodes) + (schedule.expiry_node,),
+                    states=out)
+
+
+def philox_blocks(executor: HipExecutor, *, seed: int, domain: int, path0: int, paths: int,
+                  blocks: int) -> np.ndarray:
+    """Raw device Philox4x32-10 words, shape [paths, blocks, 4] (parity probe)."""
+    out = np.empty((paths, blocks, 4), dtype=np.uint32)
+    err = FohipError()
+    code = lib().fohip_philox_blocks(executor._handle.ptr, seed, domain, path0, paths, blocks,  # noqa: SLF001
+                                     ptr(out, C.c_uint32), C.byref(err))
+    raise_for(code, err, lsm=False)
+    return out
+
+
+def stream_normals(executor: HipExecutor, *, seed: int, domain: int, path0: int, paths: int,
+                   count: int) -> tuple[np.ndarray, np.ndarray]:
+    """Device ZIGNOR normals and the u64 draws each consumed, shape [paths, count]."""
+    out = np.empty((paths, count), dtype=np.float64)
+    draws = np.empty((paths, count), dtype=np.uint32)
+    err = FohipError()
+    code = lib().fohip_stream_normals(executor._handle.ptr, seed, domain, path0, paths, count,  # noqa: SLF001
+                                      ptr(out, C.c_double), ptr(draws, C.c_uint32), C.byref(err))
+    raise_for(code, err, lsm=False)
+    return out, draws
+
+
+__all__ = [
+    "CpuExecutor",
+    "ExecutionError",
+    "GbmPaths",
+    "HipExecutor",
+    "HipPrecision",
+    "RandomStream",
+    "SimulationError",
+    "philox_blocks",
+    "simulate_gbm_seeded",
+    "stream_normals",
+]

@@ -1,0 +1,338 @@
+/*
+ * fohip: C ABI of the MI355X-native Longstaff-Schwartz engine (libfohip.so).
+ *
+ * Plain C types only (pointers + sizes, structs of pointers); no torch or HIP types cross this
+ * boundary. Every entry point returns 0 on success or a FOHIP_E_* code and fills `fohip_error`.
+ * Failures are atomic: no partial result is reported as success, and no other precision, device or
+ * CPU path is substituted.
+ *
+ * Reference interfaces replaced (jorgensandhaug/finoptions-engine):
+ *   fohip_executor_create/destroy  finoptions-runtime CpuExecutor::try_new
+ *                                  (crates/finoptions-runtime/src/lib.rs:341-389); Python
+ *                                  CpuExecutor (python/finoptions/simulation.py:40-62)
+ *   fohip_price_lsm_call           _native.price_lsm_call_seeded_f64
+ *                                  (crates/finoptions-python/src/lib.rs:1170-1248) ->
+ *                                  LsmCallEstimator::price_seeded (crates/finoptions-lsm/src/policy.rs:306)
+ *   fohip_price_lsm_grouped        _native.price_lsm_call_grouped_batch_{continuous_yield,
+ *                                  cash_dividends}_f64 (crates/finoptions-python/src/lib.rs:1885-2060)
+ *                                  -> GroupedBatchLsmCallInputs::price (batch/pricing.rs:422)
+ *   fohip_simulate_gbm_seeded      _native.simulate_gbm_seeded_f64 (lib.rs:1020-1060) ->
+ *                                  GbmPathPlan::simulate_seeded_with (crates/finoptions-sim/src/executor.rs:34)
+ *   fohip_philox_blocks            philox4x32_10 (crates/finoptions-sim/src/random/philox.rs:32) over
+ *                                  PhiloxPathStream counters (philox.rs:64-86); parity probe
+ *   fohip_stream_normals           sample_standard_normal (crates/finoptions-sim/src/random/normal.rs:96);
+ *                                  parity probe
+ *   fohip_validate_*               _native.validate_event_schedule / validate_log_discount_curve /
+ *                                  validate_cumulative_dividend_yield_curve / validate_constant_volatility
+ *                                  / validate_cash_dividend_amount (lib.rs:400-740)
+ */
+#ifndef FOHIP_H
+#define FOHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOHIP_ABI_VERSION 1
+
+/* ---- error codes (the numbering is shared with the CPU oracle) ---- */
+enum {
+    FOHIP_OK = 0,
+    /* 1xx: financial / request validation (reference InputError) */
+    FOHIP_E_INVALID_ARGUMENT = 100,
+    FOHIP_E_INVALID_INITIAL_SPOT = 101,
+    FOHIP_E_INVALID_STRIKE = 102,
+    FOHIP_E_INVALID_VOLATILITY = 103,
+    FOHIP_E_INVALID_TIME_GRID = 104,
+    FOHIP_E_INVALID_LOG_DISCOUNT_CURVE = 105,
+    FOHIP_E_INVALID_YIELD_CURVE = 106,
+    FOHIP_E_INVALID_SCHEDULE = 107,
+    FOHIP_E_INVALID_CASH_DIVIDEND = 108,
+    FOHIP_E_TIME_GRID_MISMATCH = 109,
+    FOHIP_E_INVALID_PATH_COUNT = 110,
+    FOHIP_E_INVALID_REGRESSION_SPEC = 111,
+    FOHIP_E_UNSUPPORTED_DEGREE = 112,
+    /* 2xx: simulation (reference SimulationError) */
+    FOHIP_E_NON_FINITE_TRANSITION = 200,
+    FOHIP_E_NORMAL_REJECTION_LIMIT = 201,
+    FOHIP_E_PATH_NAMESPACE_OVERFLOW = 202,
+    FOHIP_E_EMPTY_PATH_SET = 203,
+    /* 3xx: regression at an exercise node (reference RegressionError) */
+    FOHIP_E_INSUFFICIENT_OBSERVATIONS = 300,
+    FOHIP_E_DEGENERATE_PREDICTOR = 301,
+    FOHIP_E_RANK_DEFICIENT = 302,
+    FOHIP_E_CONDITION_LIMIT = 303,
+    FOHIP_E_NON_FINITE_FIT = 304,
+    FOHIP_E_JACOBI_NOT_CONVERGED = 305,
+    FOHIP_E_NON_FINITE_PREDICTOR = 306,
+    FOHIP_E_NON_FINITE_RESPONSE = 307,
+    FOHIP_E_NON_FINITE_PREDICTION = 308,
+    /* 4xx: policy fitting / evaluation (reference LsmError) */
+    FOHIP_E_RANDOM_DOMAIN_COLLISION = 400,
+    FOHIP_E_NON_FINITE_DISCOUNTED_PAYOFF = 401,
+    FOHIP_E_OUTSIDE_TRAINING_SUPPORT = 402,
+    FOHIP_E_UNREPRESENTABLE_DISCOUNT = 403,
+    FOHIP_E_PATH_COUNT_OUTSIDE_DOMAIN = 404,
+    FOHIP_E_STOPPING_COUNT_MISMATCH = 405,
+    /* 5xx: batch request validation (reference BatchLsmError -> InputError) */
+    FOHIP_E_EMPTY_BATCH = 500,
+    FOHIP_E_ROW_COUNT_MISMATCH = 501,
+    FOHIP_E_GROUP_OFFSETS = 502,
+    FOHIP_E_INVALID_ROW_VALUE = 503,
+    FOHIP_E_RANDOM_DOMAIN_RANGE = 504,
+    FOHIP_E_CASH_SCHEDULE = 505,
+    FOHIP_E_EXERCISE_MESH = 506,
+    FOHIP_E_INSUFFICIENT_EVALUATION_PATHS = 507,
+    /* 9xx: execution (reference ExecutionError) */
+    FOHIP_E_NO_DEVICE = 900,
+    FOHIP_E_HIP = 901,
+    FOHIP_E_OUT_OF_MEMORY = 902,
+    FOHIP_E_UNSUPPORTED = 903,
+    FOHIP_E_INTERNAL = 904
+};
+
+/* phase of a failure inside one pricing request */
+enum {
+    FOHIP_PHASE_NONE = 0,
+    FOHIP_PHASE_TRAINING_SIMULATION = 1,
+    FOHIP_PHASE_FIT = 2,
+    FOHIP_PHASE_EVALUATION_SIMULATION = 3,
+    FOHIP_PHASE_EVALUATION = 4
+};
+
+typedef struct {
+    int32_t code;
+    int32_t phase;
+    int64_t row;       /* strike row of a batch, or -1 */
+    int64_t node;      /* exercise / transition node, or -1 */
+    int64_t path;      /* path index, or -1 */
+    int64_t index;     /* code-specific integer (interval, rank, observed count, ...) */
+    double observed;   /* code-specific value */
+    double limit;      /* code-specific value */
+    char message[512];
+} fohip_error;
+
+/* ---- executor ---- */
+enum { FOHIP_PRECISION_F64 = 0, FOHIP_PRECISION_F32_BULK = 1 };
+
+typedef struct fohip_executor fohip_executor;
+
+/* Binds one HIP device explicitly. Fails with FOHIP_E_NO_DEVICE when the device does not exist;
+ * there is no CPU fallback. `precision` selects plain FP64 or FP32 bulk (FP32 paths, payoff and
+ * basis; FP64 Gram/solve and price/SE reductions). */
+int fohip_executor_create(int32_t device, int32_t precision, fohip_executor **out,
+                          fohip_error *err);
+int fohip_executor_destroy(fohip_executor *executor);
+int fohip_executor_info(const fohip_executor *executor, int32_t *device, int32_t *precision,
+                        char *device_name, size_t name_capacity);
+int fohip_device_count(int32_t *count);
+int fohip_abi_version(void);
+
+/* Launch parameters an autotuner may change. Results are bitwise invariant to them: reduction
+ * trees are fixed by 1024-path chunks, not by these values. Zero fields keep defaults. */
+typedef struct {
+    uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16) */
+    uint32_t sim_waves_per_eu;      /* occupancy hint for simulation launches (0 = default) */
+} fohip_tuning;
+int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
+
+/* Per-kernel timing with HIP events recorded on the executor's stream around every launch
+ * (off by default; it adds one event pair per launch). Statistics accumulate until reset. */
+typedef struct {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+} fohip_kernel_stat;
+int fohip_set_profiling(fohip_executor *executor, int32_t enabled);
+int fohip_get_profile(fohip_executor *executor, fohip_kernel_stat *out, size_t capacity,
+                      size_t *count);
+int fohip_reset_profile(fohip_executor *executor);
+
+/* ---- Greeks (pathwise, frozen stopping policy) ---- */
+enum {
+    FOHIP_GREEK_DELTA = 1u << 0,
+    FOHIP_GREEK_VEGA = 1u << 1,
+    FOHIP_GREEK_RHO = 1u << 2,
+    FOHIP_GREEK_DIVIDEND_YIELD = 1u << 3
+};
+
+/* ---- single scheduled call: price_lsm_call ---- */
+enum { FOHIP_DIVIDEND_LIQUIDATOR = 0, FOHIP_DIVIDEND_SURVIVOR = 1 };
+enum { FOHIP_TIE_STRICTLY_GREATER = 0, FOHIP_TIE_GREATER_OR_EQUAL = 1 };
+enum { FOHIP_SUPPORT_FAIL = 0, FOHIP_SUPPORT_CONTINUE = 1 };
+
+typedef struct {
+    double initial_spot;
+    double strike;
+    const double *schedule_times;       size_t schedule_nodes;
+    const double *curve_times;          const double *log_discounts;      size_t curve_nodes;
+    const double *yield_times;          const double *cumulative_yields;  size_t yield_nodes;
+    const uint32_t *dividend_nodes;     const double *dividend_amounts;   size_t dividends;
+    const uint32_t *exercise_nodes;     size_t exercise_count;
+    uint32_t expiry_node;
+    double volatility;
+    int32_t dividend_policy;
+    uint64_t training_paths, evaluation_paths;
+    uint64_t training_seed;   uint32_t training_domain;
+    uint64_t evaluation_seed; uint32_t evaluation_domain;
+    uint32_t polynomial_degree;
+    double rank_tolerance, condition_limit;
+    int32_t tie_rule;
+    int32_t insufficient_support;
+    uint32_t greeks;                    /* FOHIP_GREEK_* mask; 0 = price only */
+} fohip_call_request;
+
+/* one exercise node's fit (kind: 0 origin decision, 1 no ITM paths, 2 insufficient support,
+ * 3 regression) -- the 11 fields of NativeExerciseDiagnostic (lib.rs:1060-1072) */
+typedef struct {
+    uint32_t node;
+    int32_t kind;
+    int32_t exercise;
+    double estimated_continuation;
+    uint32_t in_the_money_paths;
+    uint32_t observations;
+    uint32_t columns;
+    uint32_t rank;
+    double condition_number;
+    double predictor_mean;
+    double predictor_scale;
+    /* fitted orthogonal recurrence (not part of the reference tuple; parity evidence) */
+    double alpha[4], beta[4], coefficients[5];
+} fohip_node_diagnostic;
+
+typedef struct {
+    double price;
+    uint32_t paths;
+    int32_t has_standard_error;         /* 0 when paths == 1 (reference returns None) */
+    double sample_standard_deviation;
+    double standard_error;
+    uint32_t observation_rows;          /* = exercise_count + 1 */
+    uint32_t *observation_nodes;        /* [exercise_count + 1], caller buffer */
+    uint32_t *exercise_counts;          /* [exercise_count + 1], caller buffer */
+    fohip_node_diagnostic *diagnostics; /* [exercise_count], caller buffer */
+    double greeks[4];                   /* delta, vega, rho, dividend_yield (when requested) */
+    double greek_standard_errors[4];
+} fohip_call_result;
+
+int fohip_price_lsm_call(fohip_executor *executor, const fohip_call_request *request,
+                         fohip_call_result *result, fohip_error *err);
+
+/* ---- group-addressed strikes: price_lsm_call_grouped_batch ---- */
+enum { FOHIP_DIVIDENDS_CONTINUOUS_YIELD = 0, FOHIP_DIVIDENDS_CASH = 1 };
+enum { FOHIP_MESH_UNIFORM = 0, FOHIP_MESH_CASH_DIVIDEND_DATES = 1, FOHIP_MESH_EXPLICIT = 2 };
+enum { FOHIP_ORIGIN_EXCLUDED = 0, FOHIP_ORIGIN_INCLUDED = 1 };
+enum { FOHIP_BOUNDARY_STRICTLY_BEFORE = 0, FOHIP_BOUNDARY_AT_OR_BEFORE = 1 };
+
+typedef struct {
+    const uint32_t *group_offsets;      /* [groups + 1], uint32 strike-row boundaries */
+    size_t groups;
+    const double *initial_spots;        /* [groups] */
+    const double *maturities;           /* [groups] */
+    const double *rates;                /* [groups] constant continuously compounded */
+    const double *volatilities;         /* [groups] */
+    const double *strikes;              /* [rows] */
+    size_t rows;
+    int32_t dividend_model;             /* FOHIP_DIVIDENDS_* */
+    const double *dividend_yields;      /* [groups] (continuous yield) */
+    const int64_t *cash_offsets;        /* [groups + 1] Arrow-style offsets (cash) */
+    const double *cash_times;
+    const double *cash_amounts;
+    size_t cash_events;                 /* length of cash_times / cash_amounts */
+    int32_t cash_policy;                /* FOHIP_DIVIDEND_* */
+    int32_t maturity_boundary;          /* FOHIP_BOUNDARY_* */
+    int32_t mesh_kind;                  /* FOHIP_MESH_* */
+    uint32_t base_intervals;            /* uniform mesh */
+    int32_t origin_exercise;            /* FOHIP_ORIGIN_* (uniform and explicit meshes) */
+    const int64_t *exercise_offsets;    /* [groups + 1] (explicit mesh) */
+    const double *exercise_times;
+    size_t exercise_events;
+    uint64_t training_paths, evaluation_paths;
+    uint64_t training_seed, evaluation_seed;
+    uint32_t training_domain_start, evaluation_domain_start;
+    uint32_t polynomial_degree;
+    double rank_tolerance, condition_limit;
+    int32_t tie_rule;
+    int32_t insufficient_support;
+    uint32_t greeks;                    /* FOHIP_GREEK_* mask */
+} fohip_grouped_request;
+
+/* The 12 row-ordered columns of NativeBatchLsmResult (lib.rs:1417-1430) plus optional Greek
+ * columns (null pointers skip them). All caller-owned, length `rows`. */
+typedef struct {
+    double *prices;
+    double *sample_standard_deviations;
+    double *standard_errors;
+    uint8_t *origin_decisions_fitted;
+    uint8_t *origin_exercised;
+    uint32_t *regression_nodes;
+    uint32_t *no_training_support_nodes;
+    uint32_t *insufficient_exercise_support_nodes;
+    uint32_t *minimum_in_the_money_training_paths;
+    double *maximum_regression_condition_numbers;
+    uint32_t *early_exercise_paths;
+    uint32_t *expiry_paths;
+    double *delta, *vega, *rho, *dividend_yield;
+    double *delta_se, *vega_se, *rho_se, *dividend_yield_se;
+} fohip_batch_result;
+
+int fohip_price_lsm_grouped(fohip_executor *executor, const fohip_grouped_request *request,
+                            fohip_batch_result *result, fohip_error *err);
+
+/* Fitted rules of every (strike row, exercise row) of one group of `request` (diagnostic and
+ * parity evidence; the group is priced again with identical streams). `out` receives
+ * strikes x exercise_rows entries, strike-major; `exercise_rows` receives the row count. */
+int fohip_grouped_diagnostics(fohip_executor *executor, const fohip_grouped_request *request,
+                              uint32_t group, fohip_node_diagnostic *out, size_t capacity,
+                              uint32_t *exercise_rows, fohip_error *err);
+
+/* ---- path export and random-stream parity probes ---- */
+typedef struct {
+    const double *times;               size_t nodes;
+    const double *log_discounts;
+    const double *cumulative_yields;
+    const uint32_t *dividend_nodes;    const double *dividend_amounts;   size_t dividends;
+    const uint32_t *exercise_nodes;    size_t exercise_count;
+    uint32_t expiry_node;
+    double volatility;
+    int32_t dividend_policy;
+} fohip_path_plan;
+
+/* Time-major [exercise_count + 1, paths] states, written to a host buffer `out` (FP64) or, for
+ * an FP32-bulk executor, converted from FP32 paths. */
+int fohip_simulate_gbm_seeded(fohip_executor *executor, const fohip_path_plan *plan,
+                              double initial_spot, uint64_t paths, uint64_t seed, uint32_t domain,
+                              double *out, fohip_error *err);
+
+/* Raw Philox4x32-10 output words: out[(p * blocks + b) * 4 + w] for path path0 + p, block b. */
+int fohip_philox_blocks(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,
+                        uint32_t paths, uint32_t blocks, uint32_t *out, fohip_error *err);
+
+/* ZIGNOR normals of paths path0..path0+paths-1: out[p * count + i]; draws[p * count + i] is the
+ * number of u64 draws normal i consumed (1 on the fast path). */
+int fohip_stream_normals(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,
+                         uint32_t paths, uint32_t count, double *out, uint32_t *draws,
+                         fohip_error *err);
+
+/* ---- host-side validation of the financial inputs (no device needed) ---- */
+int fohip_validate_time_grid(const double *times, size_t nodes, fohip_error *err);
+int fohip_validate_log_discount_curve(const double *times, const double *log_discounts,
+                                      size_t nodes, fohip_error *err);
+int fohip_validate_cumulative_yield_curve(const double *times, const double *cumulative_yields,
+                                          size_t nodes, fohip_error *err);
+int fohip_validate_event_schedule(const double *times, size_t nodes,
+                                  const uint32_t *dividend_nodes, const double *dividend_amounts,
+                                  size_t dividends, const uint32_t *exercise_nodes,
+                                  size_t exercise_count, uint32_t expiry_node, fohip_error *err);
+int fohip_validate_constant_volatility(double volatility, fohip_error *err);
+int fohip_validate_cash_dividend_amount(double amount, fohip_error *err);
+int fohip_validate_ragged_offsets(const int64_t *offsets, size_t offsets_len, size_t values_len,
+                                  fohip_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOHIP_H */

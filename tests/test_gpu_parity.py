@@ -1,0 +1,272 @@
+"""GPU parity of the HIP engine against the CPU oracle on identical seeded inputs.
+
+Bars: Philox words bit-exact; ZIGNOR normals bit-exact wherever the accept decision needs no
+transcendental (every normal drawn with one u64); FP64 paths within a few ulps; plain FP64 prices,
+standard errors, regression coefficients within 1e-10 relative; stopping counts equal. Flips of
+exercise decisions at ties are counted and reported.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import (
+    c1_inputs,
+    c1_plan,
+    c2_columns,
+    hip_groups,
+    hip_method,
+    oracle_grouped,
+    rel_err,
+)
+
+pytestmark = pytest.mark.gpu
+
+PRICE_TOL = 1e-10
+
+
+def test_philox_words_are_bit_exact(hip_f64, oracle, golden):
+    import finoptions_hip.simulation as sim
+
+    for kat in golden["philox_kats"]:
+        if kat["counter"][1] != 0:
+            continue
+        seed = kat["key"][0] | (kat["key"][1] << 32)
+        words = sim.philox_blocks(hip_f64, seed=seed, domain=kat["counter"][3],
+                                  path0=kat["counter"][2], paths=1, blocks=1)
+        assert words[0, 0].tolist() == kat["expected"]
+    for seed, domain in ((7, 1), (0x0123456789ABCDEF, 0xFFFFFFFF), (2**64 - 1, 0)):
+        words = sim.philox_blocks(hip_f64, seed=seed, domain=domain, path0=1000, paths=64,
+                                  blocks=16)
+        for p in (0, 1, 31, 63):
+            for b in (0, 1, 15):
+                ref = oracle.philox([b, 0, 1000 + p, domain], [seed & 0xFFFFFFFF, seed >> 32])
+                assert words[p, b].tolist() == ref.tolist()
+
+
+def test_normals_match_the_oracle_stream(hip_f64, oracle, golden):
+    import finoptions_hip.simulation as sim
+
+    z = golden["zignor_sequence"]
+    values, draws = sim.stream_normals(hip_f64, seed=z["seed"], domain=z["domain"],
+                                       path0=z["path"], paths=1, count=len(z["bits"]))
+    assert [f"0x{v:016x}" for v in values[0].view(np.uint64)] == z["bits"]
+    paths, count = 512, 96
+    dev, dev_draws = sim.stream_normals(hip_f64, seed=11, domain=1, path0=0, paths=paths,
+                                        count=count)
+    flips = 0
+    ulp_diff = 0
+    for p in range(paths):
+        ref, ref_draws = oracle.normals(11, 1, p, count)
+        same_draws = np.array_equal(ref_draws, dev_draws[p])
+        if not same_draws:
+            flips += 1       # a wedge/tail accept decision went the other way (exp/log ulp)
+            continue
+        one = ref_draws == 1
+        assert np.array_equal(ref[one].view(np.uint64), dev[p][one].view(np.uint64))
+        rest = ~one
+        ulp_diff += int(np.sum(ref[rest].view(np.uint64) != dev[p][rest].view(np.uint64)))
+        np.testing.assert_allclose(dev[p][rest], ref[rest], rtol=1e-14)
+    print(f"normal stream: {paths * count} normals, decision flips on {flips} paths, "
+          f"{ulp_diff} multi-draw normals differing in the last bits")
+    assert flips <= 1
+
+
+def test_fp64_paths_match_the_oracle(hip_f64, oracle):
+    import finoptions_hip as fh
+
+    inp = c1_inputs()
+    paths = fh.simulate_gbm_seeded(initial_spot=100.0, schedule=inp["schedule"], curve=inp["curve"],
+                                   dividend_yield_curve=inp["dividend_yield_curve"],
+                                   volatility=inp["volatility"],
+                                   dividend_policy=inp["dividend_policy"], paths=65_536,
+                                   random_stream=fh.RandomStream(seed=7, domain=1),
+                                   executor=hip_f64)
+    ref = oracle.simulate_seeded(c1_plan(), 100.0, 65_536, 7, 1)
+    assert paths.states.shape == ref.shape
+    err = rel_err(paths.states, ref)
+    exact = float(np.mean(paths.states == ref))
+    print(f"FP64 paths: max rel err {err:.3e}, bit-identical fraction {exact:.4f}")
+    assert err < 1e-13
+
+
+def test_c1_plain_fp64_matches_the_oracle(hip_f64, oracle):
+    import finoptions_hip as fh
+
+    method = fh.Lsm(training_paths=65_536, evaluation_paths=65_536,
+                    training_stream=fh.RandomStream(seed=7, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=7, domain=2),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.FAIL)
+    res = fh.price_lsm_call(**c1_inputs(), executor=hip_f64, method=method)
+    ref = oracle.price_lsm_call(c1_plan(), 100.0, 100.0, 65_536, 65_536, 7, 1, 7, 2)
+    print(f"C1 FP64: price {res.price!r} oracle {ref['price']!r} "
+          f"rel {abs(res.price - ref['price']) / ref['price']:.2e}")
+    assert rel_err(res.price, ref["price"]) < PRICE_TOL
+    assert rel_err(res.sample_standard_deviation, ref["sd"]) < PRICE_TOL
+    assert rel_err(res.standard_error, ref["se"]) < PRICE_TOL
+    assert res.observation_nodes == tuple(ref["observation_nodes"])
+    assert sum(res.exercise_counts) == 65_536
+    flips = sum(abs(a - b) for a, b in zip(res.exercise_counts, ref["exercise_counts"])) // 2
+    assert flips == 0
+    d = res.fit_diagnostics[0]
+    model = ref["fits"][0]["model"]
+    assert isinstance(d, fh.RegressionFitDiagnostics)
+    assert d.in_the_money_paths == ref["fits"][0]["itm_paths"]
+    assert rel_err(d.predictor_mean, model["mean"]) < PRICE_TOL
+    assert rel_err(d.predictor_scale, model["scale"]) < PRICE_TOL
+    assert rel_err(d.condition_number, model["condition"]) < PRICE_TOL
+    assert rel_err(res.regression_coefficients[0], model["coef"]) < PRICE_TOL
+
+
+def _grouped_pair(hip, oracle, cols, *, base_intervals, paths, seed, **kw):
+    import finoptions_hip as fh
+
+    groups = hip_groups(cols)
+    mesh = fh.UniformExerciseMesh(base_intervals=base_intervals,
+                                  origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = hip_method(paths, paths, seed)
+    res = fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip, method=method,
+                                          **kw)
+    ref = oracle.price_grouped(oracle_grouped(cols, base_intervals=base_intervals,
+                                              training_paths=paths, evaluation_paths=paths,
+                                              seed=seed))
+    return res, ref, groups, mesh, method
+
+
+def test_grouped_strikes_plain_fp64_match_the_oracle(hip_f64, oracle):
+    cols = c2_columns(strikes=64)
+    res, ref, *_ = _grouped_pair(hip_f64, oracle, cols, base_intervals=50, paths=32_768, seed=11)
+    err = rel_err(res.prices, ref["prices"])
+    print(f"grouped 64 strikes x 50 intervals x 32768: max price rel err {err:.2e}")
+    assert err < PRICE_TOL
+    assert rel_err(res.standard_errors, ref["standard_errors"]) < PRICE_TOL
+    assert rel_err(res.sample_standard_deviations, ref["sample_standard_deviations"]) < PRICE_TOL
+    for name in ("regression_nodes", "no_training_support_nodes",
+                 "insufficient_exercise_support_nodes", "minimum_in_the_money_training_paths",
+                 "early_exercise_paths", "expiry_paths", "origin_decisions_fitted",
+                 "origin_exercised"):
+        assert np.array_equal(getattr(res, name), ref[name]), name
+    assert rel_err(res.maximum_regression_condition_numbers,
+                   ref["maximum_regression_condition_numbers"]) < PRICE_TOL
+
+
+def test_grouped_coefficients_match_the_oracle(hip_f64, oracle):
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=8)
+    groups = hip_groups(cols)
+    mesh = fh.UniformExerciseMesh(base_intervals=12, origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = hip_method(16_384, 16_384, 5)
+    coef = fh.grouped_fit_coefficients(groups=groups, mesh=mesh, executor=hip_f64, method=method,
+                                       group=0)
+    req = oracle_grouped(cols, base_intervals=12, training_paths=16_384, evaluation_paths=16_384,
+                         seed=5)
+    models = oracle.grouped_models(req, 0, coef.shape[1])
+    worst = 0.0
+    for s in range(coef.shape[0]):
+        for r in range(coef.shape[1]):
+            m = models[s][r]["model"]
+            if models[s][r]["kind"] != "regression":
+                continue
+            ref = [m["mean"], m["scale"], *m["alpha"], *m["beta"], *m["coef"], m["condition"]]
+            got = coef[s, r, :16]
+            scale = np.maximum(np.abs(ref), 1e-3)
+            worst = max(worst, float(np.max(np.abs(got - ref) / scale)))
+    print(f"coefficients: worst relative error {worst:.2e}")
+    assert worst < PRICE_TOL
+
+
+def test_grouped_cash_dividends_match_the_oracle(hip_f64, oracle):
+    cols = c2_columns(strikes=16)
+    del cols["yields"]
+    cols.update(cash_offsets=np.array([0, 3], dtype=np.int64),
+                cash_times=np.array([0.25, 0.5, 0.75]), cash_amounts=np.array([0.6, 0.8, 1.0]))
+    res, ref, *_ = _grouped_pair(hip_f64, oracle, cols, base_intervals=12, paths=16_384, seed=3)
+    assert rel_err(res.prices, ref["prices"]) < PRICE_TOL
+    assert np.array_equal(res.expiry_paths, ref["expiry_paths"])
+
+
+def test_results_are_deterministic_and_tile_invariant(hip_f64):
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=16)
+    groups = hip_groups(cols)
+    mesh = fh.UniformExerciseMesh(base_intervals=10, origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = hip_method(8192, 8192, 17)
+    outs = []
+    for tile in (0, 1, 4, 16, 0):
+        hip_f64.tune(eval_strike_tile=tile)
+        outs.append(fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip_f64,
+                                                    method=method))
+    hip_f64.tune(eval_strike_tile=0)
+    for o in outs[1:]:
+        assert np.array_equal(o.prices.view(np.uint64), outs[0].prices.view(np.uint64))
+        assert np.array_equal(o.standard_errors, outs[0].standard_errors)
+        assert np.array_equal(o.expiry_paths, outs[0].expiry_paths)
+
+
+def test_fp32_bulk_is_within_a_tenth_of_a_standard_error(hip_f32, oracle):
+    cols = c2_columns(strikes=64)
+    res, ref, *_ = _grouped_pair(hip_f32, oracle, cols, base_intervals=50, paths=32_768, seed=11)
+    bias = np.abs(res.prices - ref["prices"]) / ref["standard_errors"]
+    print(f"FP32 bulk vs FP64 oracle: max |bias|/SE {bias.max():.3f}, mean {bias.mean():.3f}")
+    assert bias.max() < 0.1
+
+
+def test_pathwise_greeks_match_the_oracle(hip_f64, oracle):
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=8)
+    greeks = (fh.Greek.DELTA, fh.Greek.VEGA, fh.Greek.RHO, fh.Greek.DIVIDEND_YIELD)
+    res, _ref, *_ = _grouped_pair(hip_f64, oracle, cols, base_intervals=8, paths=16_384, seed=9,
+                                  greeks=greeks)
+    req = oracle_grouped(cols, base_intervals=8, training_paths=16_384, evaluation_paths=16_384,
+                         seed=9)
+    ref = oracle.grouped_pathwise_greeks(req, 0)
+    for g in greeks:
+        err = rel_err(res.greeks[g], ref[g.value])
+        print(f"{g.value}: max rel err vs oracle pathwise {err:.2e}")
+        assert err < 1e-9
+
+
+def test_origin_exercise_case(hip_f64):
+    import finoptions_hip as fh
+
+    schedule = fh.EventSchedule(times=(0.0, 0.5, 1.0),
+                                dividends=(fh.CashDividend(node=1, amount=200.0),),
+                                exercise_nodes=(0,), expiry_node=2)
+    curve = fh.LogDiscountCurve(times=schedule.times, log_discounts=(0.0, 0.0, 0.0))
+    method = hip_method(1000, 1000, 7)
+    res = fh.price_lsm_call(initial_spot=200.0, strike=100.0, schedule=schedule, curve=curve,
+                            dividend_yield_curve=fh.CumulativeDividendYieldCurve.zero(
+                                times=schedule.times),
+                            volatility=fh.ConstantVolatility(0.30),
+                            dividend_policy=fh.DividendPolicy.LIQUIDATOR, executor=hip_f64,
+                            method=method)
+    assert res.price == 100.0
+    assert res.exercise_counts == (1000, 0)
+    assert isinstance(res.fit_diagnostics[0], fh.OriginDecisionDiagnostics)
+    assert res.fit_diagnostics[0].exercise
+
+
+def test_european_with_yield_covers_black_scholes(hip_f64):
+    import finoptions_hip as fh
+    from scipy import stats
+
+    spot, strike, rate, q, vol, t = 100.0, 105.0, 0.05, 0.02, 0.25, 1.0
+    schedule = fh.EventSchedule(times=(0.0, t), dividends=(), exercise_nodes=(), expiry_node=1)
+    res = fh.price_lsm_call(initial_spot=spot, strike=strike, schedule=schedule,
+                            curve=fh.LogDiscountCurve(times=schedule.times,
+                                                      log_discounts=(0.0, -rate * t)),
+                            dividend_yield_curve=fh.CumulativeDividendYieldCurve.constant(
+                                times=schedule.times, annualized_decimal=q),
+                            volatility=fh.ConstantVolatility(vol),
+                            dividend_policy=fh.DividendPolicy.LIQUIDATOR, executor=hip_f64,
+                            method=hip_method(131_072, 131_072, 31))
+    d1 = (math.log(spot / strike) + (rate - q + 0.5 * vol * vol) * t) / (vol * math.sqrt(t))
+    bs = (spot * math.exp(-q * t) * stats.norm.cdf(d1)
+          - strike * math.exp(-rate * t) * stats.norm.cdf(d1 - vol * math.sqrt(t)))
+    assert abs(res.price - bs) < 5 * res.standard_error

@@ -1,0 +1,365 @@
+"""Pins of the CPU oracle against the reference's golden vectors and against mathematics.
+
+Each check is chosen so that a plausible restatement mistake (a dropped term, wrong sign, wrong
+index, transposed operand, wrong event order) fails it. CPU only.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import math
+import re
+import struct
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from conftest import ROOT, c1_plan, oracle_grouped
+
+
+# ---------------------------------------------------------------------------- random stream
+
+def test_philox_known_answer_vectors(oracle, golden):
+    for kat in golden["philox_kats"]:
+        assert oracle.philox(kat["counter"], kat["key"]).tolist() == kat["expected"]
+
+
+def test_stream_words_follow_counter_layout(oracle):
+    # u64 draw 2b+h of path p is words (2h, 2h+1) of philox([b, 0, p, domain], [seed_lo, seed_hi])
+    seed, domain, path = 0x0123456789ABCDEF, 0x2A, 99
+    draws = oracle.stream_u64(seed, domain, path, 6)
+    for d in range(6):
+        words = oracle.philox([d // 2, 0, path, domain], [seed & 0xFFFFFFFF, seed >> 32])
+        lo, hi = (words[0], words[1]) if d % 2 == 0 else (words[2], words[3])
+        assert int(draws[d]) == int(lo) | (int(hi) << 32)
+
+
+def test_zignor_sequence_is_bit_exact(oracle, golden):
+    z = golden["zignor_sequence"]
+    values, draws = oracle.normals(z["seed"], z["domain"], z["path"], len(z["bits"]))
+    assert [f"0x{v:016x}" for v in values.view(np.uint64)] == z["bits"]
+    assert draws.tolist() == [1] * len(z["bits"])
+
+
+def _table_digest(x, f, r) -> str:
+    return hashlib.sha256(struct.pack("<257d", *x) + struct.pack("<257d", *f)
+                          + struct.pack("<d", r)).hexdigest()
+
+
+def test_ziggurat_tables_match_the_published_tables(oracle, golden):
+    x, f, r = oracle.zig_tables()
+    assert _table_digest(x, f, r) == golden["zig_table_sha256"]
+    # the product's generated table file encodes the same values
+    inc = (ROOT / "finoptions-engine_amd" / "csrc" / "zig_tables.inc").read_text()
+    body_x = inc.split("FOHIP_ZIG_X_INIT")[1].split("}")[0]
+    body_f = inc.split("FOHIP_ZIG_F_INIT")[1].split("}")[0]
+    px = [float.fromhex(v) for v in re.findall(r"(-?0x[0-9a-fp.+-]+)", body_x)]
+    pf = [float.fromhex(v) for v in re.findall(r"(-?0x[0-9a-fp.+-]+)", body_f)]
+    pr = float.fromhex(re.search(r"FOHIP_ZIG_R (\S+)", inc).group(1))
+    assert _table_digest(px, pf, pr) == golden["zig_table_sha256"]
+
+
+def test_ziggurat_layers_have_equal_area(oracle):
+    # Doornik's construction: every layer has area v = x_i (f(x_{i+1}) - f(x_i)) for i >= 1 and
+    # the base strip x_0 f(r) = v; F_i = exp(-x_i^2 / 2).
+    x, f, r = oracle.zig_tables()
+    v = 0.00492867323399
+    areas = [x[i] * (f[i + 1] - f[i]) for i in range(1, 255)]
+    assert max(abs(a - v) for a in areas) < 1e-15
+    assert abs(x[0] * math.exp(-r * r / 2) - v) < 1e-15
+    assert np.max(np.abs(f - np.exp(-0.5 * x * x))) < 1e-17 * 100
+
+
+def test_uniform_maps_are_exact(oracle):
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2**63, size=2000, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    bits = np.concatenate([bits, np.array([0, 2**64 - 1, 1 << 12, (1 << 64) - (1 << 12)],
+                                          dtype=np.uint64)])
+    u = oracle.uniforms(bits)
+    for b, s, o, t in zip(bits.tolist(), u["symmetric_unit"], u["open_unit"], u["standard_unit"]):
+        m = (b >> 12) & ((1 << 52) - 1)
+        assert Fraction(s) == Fraction(m, 1 << 51) - 1
+        assert Fraction(o) == Fraction(m, 1 << 52) + Fraction(1, 1 << 53)
+        assert Fraction(t) == Fraction(b >> 11, 1 << 53)
+        assert -1.0 <= s < 1.0 and 0.0 < o < 1.0 and 0.0 <= t < 1.0
+
+
+def test_normals_have_the_standard_distribution(oracle):
+    samples = []
+    for path in range(400):
+        values, _ = oracle.normals(2024, 3, path, 500)
+        samples.append(values)
+    z = np.concatenate(samples)
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+    assert abs(z.mean()) < 5 / math.sqrt(z.size)
+    assert abs(z.var() - 1.0) < 5 * math.sqrt(2.0 / z.size)
+    # tail (the |z| > R sampler) frequency: 2 (1 - Phi(R))
+    r = oracle.zig_tables()[2]
+    p = 2 * stats.norm.sf(r)
+    count = int(np.sum(np.abs(z) > r))
+    assert abs(count - p * z.size) < 5 * math.sqrt(p * z.size) + 1
+    # the wedge and tail samplers were exercised: some normals used more than one draw
+    _, draws = oracle.normals(2024, 3, 0, 20000)
+    assert draws.max() > 1 and draws.mean() < 1.05
+
+
+# ----------------------------------------------------------------------- moments / regression
+
+def test_moment_kernels_known_answers(oracle):
+    assert oracle.polynomial_moments([0.5, 1.5], [2.0, 3.0], [-1.0, 2.0]) == (5.0, 4.0, 6.5)
+    moments, nxt = oracle.recurrence_moments([0.5, 1.5], [2.0, 3.0], [0.0, 1.0], [1.0, 2.0], 0.5,
+                                             0.25)
+    assert moments == (3.0625, 5.25, 4.59375)
+    assert nxt.tolist() == [0.0, 1.75]
+
+
+def representative_population(n: int):
+    idx = np.arange(n, dtype=np.uint64)
+    unit = ((idx * np.uint64(104_729)) % np.uint64(n)).astype(np.float64) / n
+    x = 0.82 + 0.56 * unit + 0.015 * np.sin(17.0 * unit)
+    y = (2.0 - 0.7 * x + 0.35 * x**2 - 0.08 * x**3 + 0.015 * x**4 + 0.002 * np.cos(31.0 * x))
+    return x, y
+
+
+@pytest.mark.parametrize(("degree", "n"), [(0, 500), (1, 500), (2, 500), (3, 500), (4, 500),
+                                           (4, 65_536)])
+def test_regression_agrees_with_numpy_svd(oracle, degree, n):
+    x, y = representative_population(n)
+    info, model = oracle.fit_regression(x, y, degree)
+    z = np.zeros(n) if degree == 0 else (x - info["mean"]) / info["scale"]
+    design = np.vander(z, degree + 1, increasing=True)
+    coef, *_ = np.linalg.lstsq(design, y, rcond=1e-12)
+    expected = design @ coef
+    sv = np.linalg.svd(design, compute_uv=False)
+    pred = oracle.predict(model, x)
+    np.testing.assert_allclose(pred, expected, rtol=2e-12, atol=2e-11)
+    assert abs(info["condition"] - sv[0] / sv[-1]) <= 2e-11 + 2e-12 * sv[0] / sv[-1]
+    assert info["rank"] == degree + 1
+    if degree:
+        assert math.isclose(info["mean"], float(np.mean(x)), rel_tol=1e-12)
+        assert math.isclose(info["scale"], float(np.std(x)), rel_tol=1e-11)
+
+
+def test_regression_recovers_polynomial_after_affine_rescaling(oracle):
+    x = -1.0 + 2.0 * np.arange(101) / 100.0
+    y = 1.25 - 0.75 * x + 0.5 * x**2 + 0.2 * x**4
+    _, first = oracle.fit_regression(x, y, 4)
+    _, second = oracle.fit_regression(1_000_000.0 + 0.001 * x, y, 4)
+    np.testing.assert_allclose(oracle.predict(first, x), y, rtol=3e-12, atol=3e-12)
+    np.testing.assert_allclose(oracle.predict(second, 1_000_000.0 + 0.001 * x), y, rtol=2e-9,
+                               atol=2e-7)
+
+
+def test_degree_zero_is_the_mean(oracle):
+    info, model = oracle.fit_regression([2.0, 2.0, 2.0, 2.0], [1.0, 2.0, 3.0, 4.0], 0, 1e-12, 1.0)
+    assert oracle.predict(model, [100.0])[0] == 2.5
+    assert info["condition"] == 1.0
+
+
+def test_regression_failures_are_typed(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.fit_regression([1.0] * 4, [2.0] * 4, 4)
+    assert e.value.code == 300
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.fit_regression([1.0] * 6, [2.0] * 6, 4)
+    assert e.value.code == 301
+
+
+# ------------------------------------------------------------------------- simulation
+
+def test_gbm_step_is_the_exact_transition_with_yield(oracle):
+    # step.rs: var = sigma^2 dt; drift = -dlogD - dQ - var / 2; S' = S exp(drift + sqrt(var) Z)
+    plan = oracle.Plan(times=[0.0, 0.25, 1.0], log_discounts=[0.0, -0.01, -0.05],
+                       cum_yields=[0.0, 0.005, 0.02], volatility=0.3, exercise_nodes=[1])
+    z = np.array([[-1.3, 0.0, 0.7], [0.4, -2.0, 1.1]])
+    out = oracle.simulate_normals(plan, 100.0, z)
+    s1 = 100.0 * np.exp((0.01 - 0.005 - 0.5 * 0.09 * 0.25) + 0.3 * math.sqrt(0.25) * z[0])
+    s2 = s1 * np.exp((0.04 - 0.015 - 0.5 * 0.09 * 0.75) + 0.3 * math.sqrt(0.75) * z[1])
+    np.testing.assert_allclose(out[0], s1, rtol=1e-15)
+    np.testing.assert_allclose(out[1], s2, rtol=1e-15)
+
+
+def test_exercise_observation_precedes_cash_and_terminal_cash_precedes_expiry(oracle):
+    for policy in (0, 1):
+        plan = oracle.Plan(times=[0.0, 0.5, 1.0], log_discounts=[0.0, 0.0, 0.0],
+                           cum_yields=[0.0, 0.0, 0.0], volatility=1e-9, dividend_policy=policy,
+                           dividend_nodes=[1, 2], dividend_amounts=[30.0, 200.0],
+                           exercise_nodes=[1])
+        out = oracle.simulate_normals(plan, 100.0, np.zeros((2, 1)))
+        # node 1 observed before its 30.0 cash; expiry observed after the terminal 200.0 cash
+        assert abs(out[0, 0] - 100.0) < 1e-6
+        s2 = out[1, 0]
+        if policy == 0:
+            assert s2 == 0.0                            # liquidator: max(70 - 200, 0)
+        else:
+            assert abs(s2 - 70.0) < 1e-6                # survivor: unpaid when S <= D
+
+
+def test_seeded_paths_are_prefix_and_domain_consistent(oracle):
+    plan = c1_plan()
+    a = oracle.simulate_seeded(plan, 100.0, 500, 7, 1)
+    b = oracle.simulate_seeded(plan, 100.0, 300, 7, 1)
+    c = oracle.simulate_seeded(plan, 100.0, 300, 7, 2)
+    assert np.array_equal(a[:, :300], b)
+    assert not np.array_equal(b, c)
+
+
+def test_zero_is_absorbing_and_overflow_reports_the_lowest_path(oracle):
+    plan = oracle.Plan(times=[0.0, 1.0], log_discounts=[0.0, -1.0], cum_yields=[0.0, 0.0],
+                       volatility=0.2)
+    out = oracle.simulate_normals(plan, 0.0, np.array([[3.0, -3.0]]))
+    assert out.tolist() == [[0.0, 0.0]]
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.simulate_normals(plan, 1.0e308, np.array([[-40.0, 5.0, 6.0]]))
+    assert e.value.code == 200 and e.value.path == 1
+
+
+# --------------------------------------------------------------------------- fit / evaluate
+
+def _estimator_kwargs():
+    return {"degree": 4, "rank_tolerance": 1e-12, "condition_limit": 1e8}
+
+
+def test_european_slice_is_the_discounted_terminal_mean(oracle):
+    plan = oracle.Plan(times=[0.0, 1.0], log_discounts=[0.0, -0.05], cum_yields=[0.0, 0.0],
+                       volatility=0.30)
+    training = oracle.simulate_normals(plan, 100.0, np.array([[-1.5, -0.8, -0.2, 0.3, 0.9, 1.4]]))
+    evaluation = oracle.simulate_normals(plan, 100.0,
+                                         np.array([[-1.4, -0.7, -0.1, 0.4, 1.0, 1.5]]))
+    res = oracle.fit_evaluate_paths(plan, 100.0, 100.0, training, evaluation)
+    expected = np.mean(np.maximum(evaluation[0] - 100.0, 0.0) * math.exp(-0.05))
+    assert abs(res["price"] - expected) <= 32 * np.finfo(float).eps * expected
+    assert res["exercise_counts"] == [6]
+    sd = np.std(np.maximum(evaluation[0] - 100.0, 0.0) * math.exp(-0.05), ddof=1)
+    assert math.isclose(res["sd"], sd, rel_tol=1e-13)
+    assert math.isclose(res["se"], sd / math.sqrt(6), rel_tol=1e-13)
+
+
+def test_origin_exercise_is_one_common_decision(oracle):
+    plan = oracle.Plan(times=[0.0, 0.5, 1.0], log_discounts=[0.0, 0.0, 0.0],
+                       cum_yields=[0.0, 0.0, 0.0], volatility=0.30, dividend_nodes=[1],
+                       dividend_amounts=[200.0], exercise_nodes=[0])
+    res = oracle.price_lsm_call(plan, 200.0, 100.0, 1000, 1000, 7, 1, 7, 2)
+    assert res["price"] == 100.0
+    assert res["exercise_counts"] == [1000, 0]
+    assert res["fits"][0]["kind"] == "origin_decision" and res["fits"][0]["exercise"]
+
+
+def test_support_fixtures_with_explicit_normals(oracle):
+    plan = oracle.Plan(times=[0.0, 0.5, 1.0], log_discounts=[0.0, 0.0, 0.0],
+                       cum_yields=[0.0, 0.0, 0.0], volatility=0.30, exercise_nodes=[1])
+
+    def sim(first, second=None):
+        z = np.array([first, second if second is not None else [0.0] * len(first)])
+        return oracle.simulate_normals(plan, 100.0, z)
+
+    training = sim([-5.0] * 5)
+    evaluation = sim([5.0] * 5)
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.fit_evaluate_paths(plan, 100.0, 100.0, training, evaluation)
+    assert (e.value.code, e.value.node, e.value.index) == (402, 1, 5)
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.fit_evaluate_paths(plan, 100.0, 100.0, sim([2.0] * 5), evaluation)
+    assert (e.value.code, e.value.node) == (301, 1)
+    four = sim([2.0, 1.0, 0.5, 0.2, -5.0])
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.fit_evaluate_paths(plan, 100.0, 100.0, four, evaluation)
+    assert (e.value.code, e.value.node, e.value.index) == (300, 1, 4)
+    res = oracle.fit_evaluate_paths(plan, 100.0, 100.0, four, sim([2.2, 1.2, 0.7, 0.3, -4.0]),
+                                    insufficient_support=1)
+    assert res["fits"][0]["kind"] == "insufficient_exercise_support"
+    assert (res["fits"][0]["itm_paths"], res["fits"][0]["required_paths"]) == (4, 5)
+    assert res["exercise_counts"] == [0, 5]
+
+
+def test_european_with_yield_covers_black_scholes(oracle):
+    spot, strike, rate, q, vol, t = 100.0, 105.0, 0.05, 0.02, 0.25, 1.0
+    plan = oracle.Plan(times=[0.0, t], log_discounts=[0.0, -rate * t], cum_yields=[0.0, q * t],
+                       volatility=vol)
+    res = oracle.price_lsm_call(plan, spot, strike, 131_072, 131_072, 31, 1, 37, 2)
+    d1 = (math.log(spot / strike) + (rate - q + 0.5 * vol * vol) * t) / (vol * math.sqrt(t))
+    d2 = d1 - vol * math.sqrt(t)
+    bs = (spot * math.exp(-q * t) * stats.norm.cdf(d1)
+          - strike * math.exp(-rate * t) * stats.norm.cdf(d2))
+    assert abs(res["price"] - bs) < 5 * res["se"]
+
+
+def test_one_dividend_replications_cover_the_hhl_reference(oracle, golden):
+    steps = 32
+    times = [n / steps for n in range(steps + 1)]
+    logd = [0.0 if tm == 0.0 else -0.06 * tm for tm in times]
+    plan = oracle.Plan(times=times, log_discounts=logd, cum_yields=[0.0] * len(times),
+                       volatility=0.30, dividend_nodes=[steps // 2], dividend_amounts=[7.0],
+                       exercise_nodes=[steps // 2])
+    prices = [oracle.price_lsm_call(plan, 100.0, 100.0, 65_536, 65_536, 10_000 + k, 1, 20_000 + k,
+                                    2)["price"] for k in range(12)]
+    mean = float(np.mean(prices))
+    se = float(np.std(prices, ddof=1)) / math.sqrt(12)
+    half = golden["student_t_99_df_11"] * se
+    assert mean - half <= golden["hhl_one_dividend_reference"] <= mean + half
+
+
+# -------------------------------------------------------------------------------- grouped
+
+def _c1_like_cols():
+    return {
+        "group_offsets": np.array([0, 1], dtype=np.uint32), "initial_spots": np.array([100.0]),
+        "strikes": np.array([100.0]), "maturities": np.array([1.0]), "rates": np.array([0.03]),
+        "volatilities": np.array([0.25]), "yields": np.array([0.02]),
+    }
+
+
+def test_one_strike_group_equals_the_single_call(oracle):
+    cols = _c1_like_cols()
+    req = oracle_grouped(cols, base_intervals=8, training_paths=4096, evaluation_paths=4096,
+                         seed=11)
+    grouped = oracle.price_grouped(req)
+    grid = oracle.group_grid(req, 0)
+    times = grid["times"].tolist()
+    plan = oracle.Plan(times=times, log_discounts=[0.0] + [-0.03 * t for t in times[1:]],
+                       cum_yields=[0.0] + [0.02 * t for t in times[1:]], volatility=0.25,
+                       exercise_nodes=grid["exercise_nodes"].tolist())
+    single = oracle.price_lsm_call(plan, 100.0, 100.0, 4096, 4096, 11, 1, 11, 2)
+    assert grouped["prices"][0] == single["price"]
+    assert grouped["standard_errors"][0] == single["se"]
+    assert grouped["expiry_paths"][0] == single["exercise_counts"][-1]
+
+
+def test_duplicate_strikes_share_the_population(oracle):
+    cols = _c1_like_cols()
+    cols.update(group_offsets=np.array([0, 3], dtype=np.uint32),
+                strikes=np.array([95.0, 100.0, 95.0]))
+    out = oracle.price_grouped(oracle_grouped(cols, base_intervals=6, training_paths=2048,
+                                              evaluation_paths=2048, seed=3))
+    assert out["prices"][0] == out["prices"][2]
+    assert out["prices"][0] != out["prices"][1]
+
+
+def test_uniform_mesh_union_with_cash_times(oracle):
+    cols = _c1_like_cols()
+    cols.update(cash_offsets=np.array([0, 2], dtype=np.int64), cash_times=np.array([0.25, 0.3]),
+                cash_amounts=np.array([1.0, 0.5]))
+    req = oracle_grouped(cols, base_intervals=4, training_paths=64, evaluation_paths=64, seed=1)
+    grid = oracle.group_grid(req, 0)
+    assert grid["times"].tolist() == [0.0, 0.25, 0.3, 0.5, 0.75, 1.0]
+    assert grid["dividend_nodes"].tolist() == [1, 2]
+    assert grid["exercise_nodes"].tolist() == [1, 2, 3, 4]
+    base = oracle_grouped(_c1_like_cols(), base_intervals=3, training_paths=64,
+                          evaluation_paths=64, seed=1)
+    t = oracle.group_grid(base, 0)["times"]
+    assert t.tolist() == [0.0, 1.0 * (1 / 3), 1.0 * (2 / 3), 1.0]
+
+
+def test_pathwise_greeks_match_frozen_policy_finite_differences(oracle):
+    cols = _c1_like_cols()
+    req = oracle_grouped(cols, base_intervals=4, training_paths=8192, evaluation_paths=8192,
+                         seed=5)
+    greeks = oracle.grouped_pathwise_greeks(req, 0)
+    # delta and vega pathwise estimators are positive for a call and bounded (delta <= 1)
+    assert 0.0 < greeks["delta"][0] < 1.0
+    assert greeks["vega"][0] > 0.0
+    assert greeks["rho"][0] > 0.0
+    assert greeks["dividend_yield"][0] < 0.0
