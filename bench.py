@@ -49,7 +49,7 @@ def build_request(fh, paths: int, *, spot: float = 100.0, vol: float = 0.25):
     method = fh.Lsm(training_paths=paths, evaluation_paths=paths,
                     training_stream=fh.RandomStream(seed=11, domain=1),
                     evaluation_stream=fh.RandomStream(seed=11, domain=2),
-                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.FAIL)
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
     return groups, mesh, method
 
 
@@ -83,7 +83,7 @@ def cpu_baseline(sample_paths: int) -> dict:
         rates=np.array([0.03]), volatilities=np.array([0.25]), dividend_yields=np.array([0.02]),
         training_paths=sample_paths, evaluation_paths=sample_paths, training_seed=11,
         evaluation_seed=11, training_domain_start=1, evaluation_domain_start=2, mesh_kind=0,
-        base_intervals=INTERVALS, origin_exercise=0)
+        base_intervals=INTERVALS, origin_exercise=0, insufficient_support=1)
     start = time.perf_counter()
     pyoracle.price_grouped(req, threads=1)
     seconds = time.perf_counter() - start
@@ -232,6 +232,7 @@ def main() -> int:
                          f"S0 100, r 0.03, q 0.02, sigma 0.25, T 1, {INTERVALS} intervals, "
                          f"{paths}+{paths} paths, Philox seed 11 domains 1/2"),
             "method": "FP32 bulk paths/payoff/basis, FP64 Gram + LDL^T solve + price/SE",
+            "insufficient_support": "continue",
             "greeks": "pathwise delta+vega, frozen policy, paths regenerated from Philox",
             "path_steps_per_step": path_steps,
             "parallelism": "single GPU",

@@ -779,6 +779,14 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // ---- backward induction
         for (uint32_t j = 0; j < max_fit; ++j) {
             const uint32_t ni = step_items[j], nr = step_rows[j];
+            if (f32) {   // FP32 bulk: one fused HBM pass per backward step
+                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, t, fp, j, it, ni, max_strikes, ex->paths.ptr,
+                                                                    ex->values.ptr, models, ex->part_t.as<double>(),
+                                                                    errors));
+                PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
+                                                                        models, errors));
+                continue;
+            }
             PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, f32, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
                                        ex->part_t.as<double>(), errors));
             PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
@@ -1227,11 +1235,22 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         }
     }
     // method-level checks (lib.rs:1302-1400, input.rs:201-238)
-    if (rq->training_paths == 0 || rq->training_paths > 0xFFFFFFFFull)
-        return fail(err, FOHIP_E_INVALID_PATH_COUNT, 0, "training path count must be in [1, 4294967295]");
-    if (rq->evaluation_paths < 2 || rq->evaluation_paths > 0xFFFFFFFFull)
-        return fail(err, FOHIP_E_INSUFFICIENT_EVALUATION_PATHS, 0,
-                    "batched LSM requires at least two independent evaluation paths");
+    auto nt_of = [&](size_t g) -> uint64_t {
+        return rq->training_path_counts ? rq->training_path_counts[g] : rq->training_paths;
+    };
+    auto ne_of = [&](size_t g) -> uint64_t {
+        return rq->evaluation_path_counts ? rq->evaluation_path_counts[g] : rq->evaluation_paths;
+    };
+    for (size_t g = 0; g < groups; ++g) {
+        if (nt_of(g) == 0 || nt_of(g) > 0xFFFFFFFFull)
+            return fail(err, FOHIP_E_INVALID_PATH_COUNT, 0, "training path count of group %zu must be in [1, 4294967295]",
+                        g);
+        if (ne_of(g) < 2 || ne_of(g) > 0xFFFFFFFFull)
+            return fail(err, FOHIP_E_INSUFFICIENT_EVALUATION_PATHS, 0,
+                        "batched LSM requires at least two independent evaluation paths (group %zu)", g);
+        if (nt_of(g) >= (1ull << 36) / 16 || ne_of(g) >= (1ull << 36) / 16)
+            return fail(err, FOHIP_E_UNSUPPORTED, 0, "path count of group %zu exceeds this engine's per-group limit", g);
+    }
     {
         const uint64_t te = static_cast<uint64_t>(rq->training_domain_start) + groups;
         const uint64_t ee = static_cast<uint64_t>(rq->evaluation_domain_start) + groups;
@@ -1398,8 +1417,8 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         p.policy = rq->dividend_model == 1 ? rq->cash_policy : 0;
         p.strike_begin = rq->group_offsets[g];
         p.strike_end = rq->group_offsets[g + 1];
-        p.nt = rq->training_paths;
-        p.ne = rq->evaluation_paths;
+        p.nt = nt_of(g);
+        p.ne = ne_of(g);
         p.train_seed = rq->training_seed;
         p.eval_seed = rq->evaluation_seed;
         p.train_domain = rq->training_domain_start + static_cast<uint32_t>(g);
@@ -1428,14 +1447,17 @@ extern "C" int fohip_price_lsm_grouped(fohip_executor *ex, const fohip_grouped_r
     std::vector<RowOut> out;
     rc = run_groups(ex, plans, strikes, spec, out, nullptr, nullptr, err);
     if (rc) return rc;
+    std::vector<uint64_t> row_paths(out.size());
+    for (const auto &p : plans)
+        for (uint32_t r = p.strike_begin; r < p.strike_end; ++r) row_paths[r] = p.ne;
     for (size_t r = 0; r < out.size(); ++r) {
         const RowOut &o = out[r];
-        if (o.sum.early + static_cast<uint64_t>(o.sum.expiry) != rq->evaluation_paths) {
+        if (o.sum.early + static_cast<uint64_t>(o.sum.expiry) != row_paths[r]) {
             set_fields(err, static_cast<int64_t>(r), -1, -1, -1, 0, 0);
             return fail(err, FOHIP_E_STOPPING_COUNT_MISMATCH, FOHIP_PHASE_EVALUATION,
                         "row %zu stopping counts %llu do not partition %llu evaluation paths", r,
                         (unsigned long long)(o.sum.early + static_cast<uint64_t>(o.sum.expiry)),
-                        (unsigned long long)rq->evaluation_paths);
+                        (unsigned long long)row_paths[r]);
         }
     }
     for (size_t r = 0; r < out.size(); ++r) {

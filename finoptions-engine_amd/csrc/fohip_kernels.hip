@@ -628,6 +628,206 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_gram(DevTables t, FitPara
     }
 }
 
+// FP32-bulk backward step j over exercise row r = obs - 2 - j in ONE pass over HBM.
+// Cashflows are stored discounted to the origin, v_o = cashflow * exp(logD(stop)), so a path's
+// stored value changes only when it exercises; the response at row r is v_o / exp(logD(r)), the
+// reference's step-by-step discounted value (policy.rs:1049-1063) in exact arithmetic.
+// The pass applies row r + 1's rule (policy.rs:570-601) and accumulates, in FP64, the power sums
+// P_j = sum (x - c)^j, j <= 2d, and G_j = sum y (x - c)^j, j <= d, of the moneyness x = S/K of
+// in-the-money paths around a pivot c (row r + 1's fitted mean, else 1). The reduce kernel
+// recentres them on the exact mean, which merges the Welford pass (regression.rs:279-297) and the
+// Gram pass into one read of each value column.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
+                                                          const uint2 *__restrict__ items,
+                                                          const Real *__restrict__ paths,
+                                                          Real *__restrict__ values,
+                                                          const Model *__restrict__ models,
+                                                          double *__restrict__ partials,
+                                                          RowError *__restrict__ errors) {
+    __shared__ double scratch[14 * kWaves];
+    const uint2 item = items[blockIdx.x];
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t local = blockIdx.y;
+    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
+    const uint32_t srow = g.strike_begin + local;
+    if (row_dead(errors, srow)) return;
+    const uint32_t r = g.obs - 2 - step;
+    const Real K = static_cast<Real>(t.strikes[srow]);
+    const uint32_t n = g.nt;
+    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const Real *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
+    Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const Model &m_next = models[g.model_base + local * g.obs + r + 1];
+    const bool apply = step > 0 && m_next.kind == kRuleRegression;
+    const double pivot = apply ? m_next.mean : 1.0;
+    const Real d_next = static_cast<Real>(t.disc0[g.row_base + r + 1]);
+    const Real inv_d = static_cast<Real>(1.0 / t.disc0[g.row_base + r]);
+    double acc[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    for (int k = 0; k < kPathsPerThread; ++k) {
+        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
+        if (p >= n) break;
+        Real v;
+        const Real s1 = S_next[p];
+        if (step == 0) {
+            v = intrinsic(s1, K) * d_next;
+            V[p] = v;
+        } else {
+            v = V[p];
+            if (apply && s1 > K) {
+                const Real cont = predict(m_next, fp.degree, s1 / K);
+                if (!isfinite(cont)) {
+                    record_error(&errors[srow], error_key(step - 1, kStageApply, p));
+                } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
+                    v = (s1 - K) * d_next;
+                    V[p] = v;
+                }
+            }
+        }
+        if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
+        const Real s0 = S_r[p];
+        if (s0 > K) {
+            const double x = static_cast<double>(s0 / K);
+            const double y = static_cast<double>(v * inv_d);
+            const double u = x - pivot;
+            double pw = 1.0;
+#pragma unroll
+            for (int j = 0; j <= 8; ++j) {
+                acc[j] += pw;
+                if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
+                pw *= u;
+            }
+        }
+    }
+    block_sum<14>(acc, scratch);
+    if (threadIdx.x == 0) {
+        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+#pragma unroll
+        for (int i = 0; i < 14; ++i) out[i] = acc[i];
+        out[14] = pivot;
+    }
+}
+
+// Support decision, exact location/scale and the standardized moments from the pivoted power
+// sums (binomial recentring), then the same Stieltjes-from-moments solve as k_fit_reduce_gram.
+__global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitParams fp, uint32_t step,
+                                                            const uint32_t *__restrict__ rows,
+                                                            const double *__restrict__ partials,
+                                                            Model *__restrict__ models,
+                                                            RowError *__restrict__ errors) {
+    __shared__ double scratch[14 * kWaves];
+    const uint32_t srow = rows[blockIdx.x];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    if (step >= g.fit_rows || row_dead(errors, srow)) return;
+    const uint32_t local = srow - g.strike_begin;
+    const uint32_t r = g.obs - 2 - step;
+    double acc[14];
+    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    double pivot = 1.0;
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
+        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        for (int i = 0; i < 14; ++i) acc[i] += in[i];
+    }
+    pivot = partials[static_cast<uint64_t>(g.part_base_t + local * g.chunks_t) * kPartialStride + 14];
+    block_sum<14>(acc, scratch);
+    if (threadIdx.x != 0) return;
+    Model &m = models[g.model_base + local * g.obs + r];
+    const double nd = acc[0];
+    const uint32_t count = static_cast<uint32_t>(nd);
+    const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
+    m.itm = count;
+    m.rank = 0;
+    m.condition = 0.0;
+    if (count == 0) {
+        m.kind = fp.support == 1 ? kRuleContinue : kRuleNoSupport;
+        return;
+    }
+    if (count < required) {
+        if (fp.support == 1) {
+            m.kind = kRuleContinue;
+        } else {
+            m.kind = kRuleNone;
+            fail_row(&errors[srow], step, FOHIP_E_INSUFFICIENT_OBSERVATIONS, count, required, fp.degree);
+        }
+        return;
+    }
+    double mu[9], nu[5];
+    double mean = 0.0, scale = 1.0;
+    if (fp.degree == 0) {
+        mu[0] = nd;
+        nu[0] = acc[9];
+        for (int j = 1; j < 9; ++j) mu[j] = 0.0;
+        for (int j = 1; j < 5; ++j) nu[j] = 0.0;
+    } else {
+        const double d = acc[1] / nd;
+        mean = pivot + d;
+        const double m2 = acc[2] - acc[1] * d;
+        scale = sqrt(fmax(m2, 0.0) / nd);
+        if (!isfinite(mean) || !isfinite(scale) || scale <= 0.0) {
+            m.kind = kRuleNone;
+            fail_row(&errors[srow], step, FOHIP_E_DEGENERATE_PREDICTOR, fp.degree, scale, 0.0);
+            return;
+        }
+        // sum (x - mean)^j = sum_k C(j, k) (-d)^(j - k) P_k, then divide by scale^j
+        double binom[9][9];
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b) binom[a][b] = 0.0;
+        for (int a = 0; a < 9; ++a) {
+            binom[a][0] = 1.0;
+            for (int b = 1; b <= a; ++b) binom[a][b] = binom[a - 1][b - 1] + (b <= a - 1 ? binom[a - 1][b] : 0.0);
+        }
+        double negd[9];
+        negd[0] = 1.0;
+        for (int j = 1; j < 9; ++j) negd[j] = negd[j - 1] * (-d);
+        double inv_scale = 1.0 / scale, sp = 1.0;
+        for (int j = 0; j < 9; ++j) {
+            double q = 0.0, gq = 0.0;
+            for (int k = 0; k <= j; ++k) {
+                q += binom[j][k] * negd[j - k] * acc[k];
+                if (j < 5) gq += binom[j][k] * negd[j - k] * acc[9 + k];
+            }
+            mu[j] = q * sp;
+            if (j < 5) nu[j] = gq * sp;
+            sp *= inv_scale;
+        }
+        mu[0] = nd;
+        mu[1] = 0.0;   // exactly centred
+    }
+    m.mean = mean;
+    m.scale = scale;
+    m.prev_norm = 1.0;
+    for (int k = 0; k < kMaxDegree; ++k) m.alpha[k] = m.beta[k] = 0.0;
+    for (int k = 0; k <= kMaxDegree; ++k) m.coef[k] = m.norm[k] = 0.0;
+    m.kind = kRulePending;
+    double a_prev[5] = {0, 0, 0, 0, 0}, a_cur[5] = {1, 0, 0, 0, 0};
+    for (int k = 0; k <= fp.degree; ++k) {
+        double norm = 0.0, weighted = 0.0, projection = 0.0;
+        for (int i = 0; i <= k; ++i) {
+            projection += a_cur[i] * nu[i];
+            for (int j = 0; j <= k; ++j) {
+                norm += a_cur[i] * a_cur[j] * mu[i + j];
+                if (k < fp.degree) weighted += a_cur[i] * a_cur[j] * mu[i + j + 1];
+            }
+        }
+        stieltjes_step(m, fp, &errors[srow], step, k, norm, projection, weighted);
+        if (m.kind != kRulePending && m.kind != kRuleRegression) return;
+        if (k < fp.degree) {
+            double a_next[5] = {0, 0, 0, 0, 0};
+            for (int i = 0; i <= k; ++i) {
+                a_next[i + 1] += a_cur[i];
+                a_next[i] -= m.alpha[k] * a_cur[i];
+                a_next[i] -= m.beta[k] * a_prev[i];
+            }
+            for (int i = 0; i < 5; ++i) {
+                a_prev[i] = a_cur[i];
+                a_cur[i] = a_next[i];
+            }
+        }
+    }
+}
+
 // After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
 // cashflows for the valuation-origin decision (policy.rs:267-281).
 template <typename Real>
@@ -636,7 +836,8 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
                                                       const Real *__restrict__ values,
                                                       const Model *__restrict__ models,
                                                       double *__restrict__ partials,
-                                                      RowError *__restrict__ errors) {
+                                                      RowError *__restrict__ errors,
+                                                      int origin_discounted) {
     __shared__ double scratch[kWaves];
     const uint2 item = items[blockIdx.x];
     const DevGroup &g = t.groups[item.x];
@@ -652,7 +853,11 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
     const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     const Model &m = models[g.model_base + local * g.obs + last_row];
     const bool apply = g.fit_rows > 0 && m.kind == kRuleRegression;
-    const Real disc = static_cast<Real>(g.disc_final);
+    // Stored values are either discounted to the last fitted node (FP64 method, the reference's
+    // representation) or already discounted to the origin (FP32 bulk method).
+    const bool stored_at_origin = origin_discounted && g.fit_rows > 0;
+    const Real disc = stored_at_origin ? Real(1) : static_cast<Real>(g.disc_final);
+    const Real d_last = stored_at_origin ? static_cast<Real>(t.disc0[g.row_base + last_row]) : Real(1);
     double sum[1] = {0.0};
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
@@ -669,7 +874,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
                     if (!isfinite(cont)) {
                         record_error(&errors[srow], error_key(g.fit_rows - 1, kStageApply, p));
                     } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
-                        v = s1 - K;
+                        v = (s1 - K) * d_last;
                     }
                 }
             }
@@ -1174,6 +1379,24 @@ hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitP
     return hipGetLastError();
 }
 
+hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                                const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                                void *values, const Model *models, double *partials, RowError *errors) {
+    if (nitems == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_bulk_step<float>, grid2(nitems, max_strikes), dim3(kBlock), 0, st, t, fp, step, items,
+                       static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                                  const uint32_t *rows, uint32_t nrows, const double *partials, Model *models,
+                                  RowError *errors) {
+    if (nrows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fit_bulk_reduce, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
+                       errors);
+    return hipGetLastError();
+}
+
 hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const Model *models,
@@ -1183,11 +1406,11 @@ hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const 
     if (f32)
         hipLaunchKernelGGL(k_fit_final<float>, grid, dim3(kBlock), 0, st, t, fp, items,
                            static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors);
+                           errors, 1);
     else
         hipLaunchKernelGGL(k_fit_final<double>, grid, dim3(kBlock), 0, st, t, fp, items,
                            static_cast<const double *>(paths), static_cast<const double *>(values), models,
-                           partials, errors);
+                           partials, errors, 0);
     return hipGetLastError();
 }
 

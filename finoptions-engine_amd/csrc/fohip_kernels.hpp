@@ -46,6 +46,13 @@ hipError_t launch_fit_gram(hipStream_t st, bool f32, const DevTables &t, const F
 hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitParams &fp,
                                   uint32_t step, const uint32_t *rows, uint32_t nrows,
                                   const double *partials, Model *models, RowError *errors);
+// FP32 bulk: one fused HBM pass per backward step (apply, Welford + Gram via pivoted power sums)
+hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                                const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                                void *values, const Model *models, double *partials, RowError *errors);
+hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                                  const uint32_t *rows, uint32_t nrows, const double *partials, Model *models,
+                                  RowError *errors);
 hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const Model *models,

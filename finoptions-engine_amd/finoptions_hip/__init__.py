@@ -19,6 +19,7 @@ from .batch_lsm import (
     LsmStrikeGroups,
     OriginExercise,
     RaggedExerciseTimes,
+    RaggedPathCounts,
     UniformExerciseMesh,
     grouped_fit_coefficients,
     price_lsm_call_batch,
@@ -90,6 +91,7 @@ __all__ = [
     "OriginExercise",
     "RaggedCashDividends",
     "RaggedExerciseTimes",
+    "RaggedPathCounts",
     "RandomStream",
     "RegressionFitDiagnostics",
     "SimulationError",

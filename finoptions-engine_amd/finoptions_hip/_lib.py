@@ -130,6 +130,7 @@ class FohipGroupedRequest(C.Structure):
         ("mesh_kind", C.c_int32), ("base_intervals", C.c_uint32), ("origin_exercise", C.c_int32),
         ("exercise_offsets", _i64p), ("exercise_times", _dp), ("exercise_events", C.c_size_t),
         ("training_paths", C.c_uint64), ("evaluation_paths", C.c_uint64),
+        ("training_path_counts", _u32p), ("evaluation_path_counts", _u32p),
         ("training_seed", C.c_uint64), ("evaluation_seed", C.c_uint64),
         ("training_domain_start", C.c_uint32), ("evaluation_domain_start", C.c_uint32),
         ("polynomial_degree", C.c_uint32),

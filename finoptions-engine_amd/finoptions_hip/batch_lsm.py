@@ -144,6 +144,28 @@ class CashDividendBatch:
             raise InputError("maturity_boundary must be MaturityBoundary")
 
 
+@dataclass(frozen=True, slots=True)
+class RaggedPathCounts:
+    """Per-group (per-contract for a row batch) training and evaluation path counts.
+
+    An extension for ragged batches: the reference's ``Lsm`` carries one path count for every row.
+    When given, these counts replace ``method.training_paths`` / ``method.evaluation_paths``.
+    """
+
+    training: UInt32Vector
+    evaluation: UInt32Vector
+
+    def __post_init__(self) -> None:
+        validate_vector(self.training, field="training path counts", dtype="uint32")
+        validate_vector(self.evaluation, field="evaluation path counts", dtype="uint32")
+        if self.training.size != self.evaluation.size:
+            raise InputError("training and evaluation path counts must have one entry per group")
+        if np.any(self.training == 0):
+            raise InputError("training path counts must be positive")
+        if np.any(self.evaluation < _MIN_EVALUATION_PATHS):
+            raise InputError("ragged batches require at least two evaluation paths per group")
+
+
 BatchDividends = Union[ContinuousDividendYieldBatch, CashDividendBatch]
 ExerciseMesh = Union[UniformExerciseMesh, CashDividendExerciseMesh, ExplicitExerciseMesh]
 
@@ -234,7 +256,8 @@ def _mesh_fields(mesh: ExerciseMesh, groups: int) -> dict:
 
 def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates, volatilities,
            dividends: BatchDividends, mesh: ExerciseMesh, executor: HipExecutor, method: Lsm,
-           greeks: tuple[Greek, ...], *, what: str) -> LsmBatchResult:
+           greeks: tuple[Greek, ...], *, what: str,
+           path_counts: RaggedPathCounts | None = None) -> LsmBatchResult:
     if type(executor) is not HipExecutor:
         raise InputError("executor must be HipExecutor")
     if type(method) is not Lsm:
@@ -290,6 +313,13 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
         req.exercise_events = mf["exercise"].times.size
     req.training_paths = method.training_paths
     req.evaluation_paths = method.evaluation_paths
+    if path_counts is not None:
+        if type(path_counts) is not RaggedPathCounts:
+            raise InputError("path_counts must be RaggedPathCounts")
+        if path_counts.training.size != groups:
+            raise InputError(f"path_counts has {path_counts.training.size} groups; expected {groups}")
+        req.training_path_counts = ptr(path_counts.training, C.c_uint32)
+        req.evaluation_path_counts = ptr(path_counts.evaluation, C.c_uint32)
     req.training_seed = method.training_stream.seed
     req.evaluation_seed = method.evaluation_stream.seed
     req.training_domain_start = method.training_stream.domain
@@ -330,7 +360,12 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
             v.flags.writeable = False
     result = LsmBatchResult(**cols, greeks=greek_cols or None, greek_standard_errors=greek_se or None)
     stopping = result.early_exercise_paths.astype(np.uint64) + result.expiry_paths.astype(np.uint64)
-    if np.any(stopping != method.evaluation_paths):
+    if path_counts is None:
+        expected = np.full(rows, method.evaluation_paths, dtype=np.uint64)
+    else:
+        expected = np.repeat(path_counts.evaluation.astype(np.uint64),
+                             np.diff(group_offsets.astype(np.int64)))
+    if np.any(stopping != expected):
         raise LsmError(f"native {what} stopping counts do not partition evaluation paths")
     return result
 
@@ -342,6 +377,7 @@ def price_lsm_call_grouped_batch(
     executor: HipExecutor,
     method: Lsm,
     greeks: tuple[Greek, ...] = (),
+    path_counts: RaggedPathCounts | None = None,
 ) -> LsmBatchResult:
     """Price strike groups while simulating each declared market state exactly once.
 
@@ -364,7 +400,7 @@ def price_lsm_call_grouped_batch(
             "mesh must be UniformExerciseMesh, CashDividendExerciseMesh, or ExplicitExerciseMesh")
     return _price(groups.group_offsets, groups.initial_spots, groups.strikes, groups.maturities,
                   groups.rates, groups.volatilities, groups.dividends, mesh, executor, method,
-                  greeks, what="grouped batched LSM")
+                  greeks, what="grouped batched LSM", path_counts=path_counts)
 
 
 def price_lsm_call_batch(  # noqa: PLR0913 - each financial column is independently typed
@@ -379,8 +415,13 @@ def price_lsm_call_batch(  # noqa: PLR0913 - each financial column is independen
     executor: HipExecutor,
     method: Lsm,
     greeks: tuple[Greek, ...] = (),
+    path_counts: RaggedPathCounts | None = None,
 ) -> LsmBatchResult:
-    """Price constant-parameter Bermudan calls, one simulated state per contract row."""
+    """Price constant-parameter Bermudan calls, one simulated state per contract row.
+
+    With ``path_counts`` each contract has its own training/evaluation path count (a ragged
+    batch); all contracts are still priced in one request over shared device launches.
+    """
     for name, values in (("initial_spots", initial_spots), ("strikes", strikes),
                          ("maturities", maturities), ("rates", rates),
                          ("volatilities", volatilities)):
@@ -389,7 +430,7 @@ def price_lsm_call_batch(  # noqa: PLR0913 - each financial column is independen
         raise InputError(f"strikes has {strikes.size} rows; expected {initial_spots.size}")
     offsets = np.arange(initial_spots.size + 1, dtype=np.uint32)
     return _price(offsets, initial_spots, strikes, maturities, rates, volatilities, dividends, mesh,
-                  executor, method, greeks, what="batched LSM")
+                  executor, method, greeks, what="batched LSM", path_counts=path_counts)
 
 
 def grouped_fit_coefficients(*, groups: LsmStrikeGroups, mesh: ExerciseMesh,
@@ -476,6 +517,7 @@ __all__ = [
     "LsmStrikeGroups",
     "OriginExercise",
     "RaggedExerciseTimes",
+    "RaggedPathCounts",
     "UniformExerciseMesh",
     "grouped_fit_coefficients",
     "price_lsm_call_batch",

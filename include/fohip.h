@@ -14,7 +14,7 @@
  *                                  (crates/finoptions-python/src/lib.rs:1170-1248) ->
  *                                  LsmCallEstimator::price_seeded (crates/finoptions-lsm/src/policy.rs:306)
  *   fohip_price_lsm_grouped        _native.price_lsm_call_grouped_batch_{continuous_yield,
- *                                  cash_dividends}_f64 (crates/finoptions-python/src/lib.rs:1885-2060)
+ *                                  cash_dividends}_f64 (crates/finoptions-python/src/lib.rs:1885-2061)
  *                                  -> GroupedBatchLsmCallInputs::price (batch/pricing.rs:422)
  *   fohip_simulate_gbm_seeded      _native.simulate_gbm_seeded_f64 (lib.rs:1020-1060) ->
  *                                  GbmPathPlan::simulate_seeded_with (crates/finoptions-sim/src/executor.rs:34)
@@ -24,7 +24,7 @@
  *                                  parity probe
  *   fohip_validate_*               _native.validate_event_schedule / validate_log_discount_curve /
  *                                  validate_cumulative_dividend_yield_curve / validate_constant_volatility
- *                                  / validate_cash_dividend_amount (lib.rs:400-740)
+ *                                  / validate_cash_dividend_amount (lib.rs:407-470, 703-722)
  */
 #ifndef FOHIP_H
 #define FOHIP_H
@@ -250,6 +250,10 @@ typedef struct {
     const double *exercise_times;
     size_t exercise_events;
     uint64_t training_paths, evaluation_paths;
+    /* Optional per-group path counts (ragged batches; no counterpart in the reference, whose Lsm
+     * method carries one count for every row). Null pointers use training_paths/evaluation_paths. */
+    const uint32_t *training_path_counts;    /* [groups] or null */
+    const uint32_t *evaluation_path_counts;  /* [groups] or null */
     uint64_t training_seed, evaluation_seed;
     uint32_t training_domain_start, evaluation_domain_start;
     uint32_t polynomial_degree;

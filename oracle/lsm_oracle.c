@@ -1127,7 +1127,7 @@ int or_group_grid(const or_grouped_input *in, uint32_t g, double *times, uint32_
     const double *cash = NULL, *amounts = NULL;
     uint32_t ncash = 0;
     if (in->dividend_model == 1) {
-        int64_t a = in->cash_offsets[g], b = in->cash_offsets[g + 1];
+        int64_t a = in->cash_offsets[g] - in->cash_offsets[0], b = in->cash_offsets[g + 1] - in->cash_offsets[0];
         cash = in->cash_times + a;
         amounts = in->cash_amounts + a;
         ncash = (uint32_t)(b - a);
@@ -1137,7 +1137,8 @@ int or_group_grid(const or_grouped_input *in, uint32_t g, double *times, uint32_
     const double *ex = NULL;
     uint32_t nex = 0;
     if (in->mesh_kind == 2) {
-        int64_t a = in->exercise_offsets[g], b = in->exercise_offsets[g + 1];
+        int64_t a = in->exercise_offsets[g] - in->exercise_offsets[0];
+        int64_t b = in->exercise_offsets[g + 1] - in->exercise_offsets[0];
         ex = in->exercise_times + a;
         nex = (uint32_t)(b - a);
         if (nex + ncash + 2 > capacity) return -1;
@@ -1174,7 +1175,7 @@ typedef struct {
 
 static void group_plan(const or_grouped_input *in, uint32_t g, group_plan_t *gp) {
     uint32_t ncash = 0, nex = 0;
-    if (in->dividend_model == 1) ncash = (uint32_t)(in->cash_offsets[g + 1] - in->cash_offsets[g]);
+    if (in->dividend_model == 1) ncash = (uint32_t)(in->cash_offsets[g + 1] - in->cash_offsets[g]);  /* base cancels */
     if (in->mesh_kind == 2) nex = (uint32_t)(in->exercise_offsets[g + 1] - in->exercise_offsets[g]);
     uint32_t cap = (in->mesh_kind == 0 ? in->base_intervals : 1) + nex + ncash + 3;
     gp->times = malloc(sizeof(double) * cap);
@@ -1223,7 +1224,8 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
     group_plan(in, g, &gp);
     plan_t p;
     plan_build(&gp.in, &p);
-    uint32_t nt = in->training_paths, ne = in->evaluation_paths;
+    uint32_t nt = in->training_path_counts ? in->training_path_counts[g] : in->training_paths;
+    uint32_t ne = in->evaluation_path_counts ? in->evaluation_path_counts[g] : in->evaluation_paths;
     uint32_t first = in->group_offsets[g], last = in->group_offsets[g + 1];
     uint32_t strikes = last - first;
     int rc = 0;
@@ -1342,7 +1344,7 @@ int or_grouped_models(const or_grouped_input *in, uint32_t g, or_node_fit *fits_
     group_plan(in, g, &gp);
     plan_t p;
     plan_build(&gp.in, &p);
-    uint32_t nt = in->training_paths;
+    uint32_t nt = in->training_path_counts ? in->training_path_counts[g] : in->training_paths;
     uint32_t first = in->group_offsets[g], last = in->group_offsets[g + 1];
     double *train = malloc(sizeof(double) * p.obs * (size_t)nt);
     double *values = malloc(sizeof(double) * nt);
@@ -1386,7 +1388,8 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
     group_plan(in, g, &gp);
     plan_t p;
     plan_build(&gp.in, &p);
-    uint32_t nt = in->training_paths, ne = in->evaluation_paths;
+    uint32_t nt = in->training_path_counts ? in->training_path_counts[g] : in->training_paths;
+    uint32_t ne = in->evaluation_path_counts ? in->evaluation_path_counts[g] : in->evaluation_paths;
     uint32_t first = in->group_offsets[g], last = in->group_offsets[g + 1];
     uint32_t strikes = last - first;
     double *train = malloc(sizeof(double) * p.obs * (size_t)nt);

@@ -152,6 +152,8 @@ typedef struct {
     const int64_t *exercise_offsets;   /* [groups + 1] (explicit) */
     const double *exercise_times;
     uint32_t training_paths, evaluation_paths;
+    const uint32_t *training_path_counts;   /* [groups] or NULL (ragged extension) */
+    const uint32_t *evaluation_path_counts; /* [groups] or NULL */
     uint64_t training_seed, evaluation_seed;
     uint32_t training_domain_start, evaluation_domain_start;
     int32_t degree;

@@ -130,6 +130,8 @@ class OrGroupedInput(C.Structure):
         ("exercise_times", C.POINTER(C.c_double)),
         ("training_paths", C.c_uint32),
         ("evaluation_paths", C.c_uint32),
+        ("training_path_counts", C.POINTER(C.c_uint32)),
+        ("evaluation_path_counts", C.POINTER(C.c_uint32)),
         ("training_seed", C.c_uint64),
         ("evaluation_seed", C.c_uint64),
         ("training_domain_start", C.c_uint32),
@@ -483,6 +485,8 @@ class GroupedRequest:
     condition_limit: float = 1e8
     tie_rule: int = 0
     insufficient_support: int = 0
+    training_path_counts: np.ndarray | None = None
+    evaluation_path_counts: np.ndarray | None = None
 
     def struct(self) -> tuple[OrGroupedInput, list]:
         f64 = lambda a: np.ascontiguousarray(a if a is not None else [0.0], dtype=np.float64)
@@ -492,6 +496,11 @@ class GroupedRequest:
                 f64(self.volatilities), f64(self.strikes), f64(self.dividend_yields),
                 i64(self.cash_offsets), f64(self.cash_times), f64(self.cash_amounts),
                 i64(self.exercise_offsets), f64(self.exercise_times)]
+        counts = [None if c is None else np.ascontiguousarray(c, dtype=np.uint32)
+                  for c in (self.training_path_counts, self.evaluation_path_counts)]
+        keep += [c for c in counts if c is not None]
+        cptr = [C.cast(None, C.POINTER(C.c_uint32)) if c is None else _p(c, C.c_uint32)
+                for c in counts]
         groups = len(self.initial_spots)
         s = OrGroupedInput(
             groups, _p(keep[0], C.c_uint32), _p(keep[1], C.c_double), _p(keep[2], C.c_double),
@@ -501,7 +510,7 @@ class GroupedRequest:
             _p(keep[9], C.c_double), int(self.cash_policy), int(self.mesh_kind),
             int(self.base_intervals), int(self.origin_exercise), _p(keep[10], C.c_int64),
             _p(keep[11], C.c_double), int(self.training_paths), int(self.evaluation_paths),
-            int(self.training_seed), int(self.evaluation_seed), int(self.training_domain_start),
+            cptr[0], cptr[1], int(self.training_seed), int(self.evaluation_seed), int(self.training_domain_start),
             int(self.evaluation_domain_start), int(self.degree), float(self.rank_tolerance),
             float(self.condition_limit), int(self.tie_rule), int(self.insufficient_support))
         return s, keep

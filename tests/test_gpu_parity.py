@@ -121,18 +121,25 @@ def test_c1_plain_fp64_matches_the_oracle(hip_f64, oracle):
     assert rel_err(res.regression_coefficients[0], model["coef"]) < PRICE_TOL
 
 
-def _grouped_pair(hip, oracle, cols, *, base_intervals, paths, seed, **kw):
+def _grouped_pair(hip, oracle, cols, *, base_intervals, paths, seed, continue_support=True,
+                  **kw):
+    """Price one request on the GPU and on the oracle. Deep out-of-the-money strikes at the first
+    nodes have fewer in-the-money training paths than basis columns, so these requests declare
+    the CONTINUE support policy (the reference requires the policy to be explicit)."""
     import finoptions_hip as fh
 
     groups = hip_groups(cols)
     mesh = fh.UniformExerciseMesh(base_intervals=base_intervals,
                                   origin_exercise=fh.OriginExercise.EXCLUDED)
-    method = hip_method(paths, paths, seed)
+    support = (fh.InsufficientExerciseSupportPolicy.CONTINUE if continue_support
+               else fh.InsufficientExerciseSupportPolicy.FAIL)
+    method = hip_method(paths, paths, seed, support=support)
     res = fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip, method=method,
                                           **kw)
     ref = oracle.price_grouped(oracle_grouped(cols, base_intervals=base_intervals,
                                               training_paths=paths, evaluation_paths=paths,
-                                              seed=seed))
+                                              seed=seed,
+                                              insufficient_support=1 if continue_support else 0))
     return res, ref, groups, mesh, method
 
 
@@ -224,7 +231,7 @@ def test_pathwise_greeks_match_the_oracle(hip_f64, oracle):
     res, _ref, *_ = _grouped_pair(hip_f64, oracle, cols, base_intervals=8, paths=16_384, seed=9,
                                   greeks=greeks)
     req = oracle_grouped(cols, base_intervals=8, training_paths=16_384, evaluation_paths=16_384,
-                         seed=9)
+                         seed=9, insufficient_support=1)
     ref = oracle.grouped_pathwise_greeks(req, 0)
     for g in greeks:
         err = rel_err(res.greeks[g], ref[g.value])
@@ -270,3 +277,94 @@ def test_european_with_yield_covers_black_scholes(hip_f64):
     bs = (spot * math.exp(-q * t) * stats.norm.cdf(d1)
           - strike * math.exp(-rate * t) * stats.norm.cdf(d1 - vol * math.sqrt(t)))
     assert abs(res.price - bs) < 5 * res.standard_error
+
+
+def test_fail_policy_reports_insufficient_support_like_the_oracle(hip_f64, oracle):
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=64)
+    with pytest.raises(fh.LsmError) as gpu_err:
+        _grouped_pair(hip_f64, oracle, cols, base_intervals=50, paths=32_768, seed=11,
+                      continue_support=False)
+    with pytest.raises(oracle.OracleError) as cpu_err:
+        oracle.price_grouped(oracle_grouped(cols, base_intervals=50, training_paths=32_768,
+                                            evaluation_paths=32_768, seed=11))
+    assert gpu_err.value.code == cpu_err.value.code
+    assert gpu_err.value.row == cpu_err.value.row
+    assert gpu_err.value.node == cpu_err.value.node
+    assert gpu_err.value.index == cpu_err.value.index
+
+
+def _ragged_requests(fh, oracle, contracts, *, seed, path_scale, max_exercise):
+    from finoptions_hip.workloads import ragged_batch
+
+    w = ragged_batch(contracts, seed=seed, max_exercise=max_exercise, path_scale=path_scale)
+    rows = len(w["initial_spots"])
+    dividends = fh.CashDividendBatch(
+        schedules=fh.RaggedCashDividends(offsets=w["cash_offsets"], times=w["cash_times"],
+                                         amounts=w["cash_amounts"]),
+        policy=fh.DividendPolicy.LIQUIDATOR, maturity_boundary=fh.MaturityBoundary.STRICTLY_BEFORE)
+    mesh = fh.ExplicitExerciseMesh(
+        schedules=fh.RaggedExerciseTimes(offsets=w["exercise_offsets"], times=w["exercise_times"]),
+        origin_exercise=fh.OriginExercise.EXCLUDED)
+    counts = fh.RaggedPathCounts(training=w["paths"], evaluation=w["paths"])
+    method = hip_method(int(w["paths"].max()), int(w["paths"].max()), 2024,
+                        support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    req = oracle.GroupedRequest(
+        group_offsets=np.arange(rows + 1, dtype=np.uint32), initial_spots=w["initial_spots"],
+        strikes=w["strikes"], maturities=w["maturities"], rates=w["rates"],
+        volatilities=w["volatilities"], cash_offsets=w["cash_offsets"],
+        cash_times=w["cash_times"], cash_amounts=w["cash_amounts"], cash_policy=0,
+        training_paths=int(w["paths"].max()), evaluation_paths=int(w["paths"].max()),
+        training_path_counts=w["paths"], evaluation_path_counts=w["paths"],
+        training_seed=2024, evaluation_seed=2024, training_domain_start=1,
+        evaluation_domain_start=2 + rows, mesh_kind=2, exercise_offsets=w["exercise_offsets"],
+        exercise_times=w["exercise_times"], origin_exercise=0, insufficient_support=1)
+    method = fh.Lsm(training_paths=method.training_paths, evaluation_paths=method.evaluation_paths,
+                    training_stream=fh.RandomStream(seed=2024, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=2024, domain=2 + rows),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    return w, dividends, mesh, counts, method, req
+
+
+def test_ragged_batch_matches_the_oracle(hip_f64, hip_f32, oracle):
+    import finoptions_hip as fh
+
+    w, dividends, mesh, counts, method, req = _ragged_requests(fh, oracle, 40, seed=2024,
+                                                               path_scale=1 / 32,
+                                                               max_exercise=40)
+    kwargs = dict(initial_spots=w["initial_spots"], strikes=w["strikes"],
+                  maturities=w["maturities"], rates=w["rates"], volatilities=w["volatilities"],
+                  dividends=dividends, mesh=mesh, method=method, path_counts=counts)
+    res = fh.price_lsm_call_batch(executor=hip_f64, **kwargs)
+    ref = oracle.price_grouped(req)
+    err = rel_err(res.prices, ref["prices"])
+    print(f"ragged batch (40 contracts): FP64 max price rel err {err:.2e}")
+    assert err < PRICE_TOL
+    assert np.array_equal(res.expiry_paths, ref["expiry_paths"])
+    assert np.array_equal(res.early_exercise_paths + res.expiry_paths, w["paths"])
+    bulk = fh.price_lsm_call_batch(executor=hip_f32, **kwargs)
+    bias = np.abs(bulk.prices - ref["prices"]) / ref["standard_errors"]
+    print(f"ragged batch FP32 bulk: max |bias|/SE {bias.max():.3f}")
+    assert bias.max() < 0.1
+
+
+def test_large_grouped_frame_slice_matches_the_oracle(hip_f64, hip_f32, oracle):
+    import finoptions_hip as fh
+    from finoptions_hip.workloads import large_grouped_frame
+
+    w = large_grouped_frame(groups=6, strikes_per_group=24)
+    cols = {k: w[k] for k in ("group_offsets", "initial_spots", "strikes", "maturities", "rates",
+                              "volatilities", "cash_offsets", "cash_times", "cash_amounts")}
+    res_pair = _grouped_pair(hip_f64, oracle, cols, base_intervals=w["intervals"], paths=4096,
+                             seed=5)
+    res, ref = res_pair[0], res_pair[1]
+    err = rel_err(res.prices, ref["prices"])
+    print(f"grouped frame slice (6 x 24): FP64 max price rel err {err:.2e}")
+    assert err < PRICE_TOL
+    assert np.array_equal(res.expiry_paths, ref["expiry_paths"])
+    bulk = _grouped_pair(hip_f32, oracle, cols, base_intervals=w["intervals"], paths=4096,
+                         seed=5)[0]
+    bias = np.abs(bulk.prices - ref["prices"]) / ref["standard_errors"]
+    print(f"grouped frame slice FP32 bulk: max |bias|/SE {bias.max():.3f}")
+    assert bias.max() < 0.1
