@@ -64,9 +64,13 @@ def algorithmic_bytes(kernel: str, paths: int, real: int) -> int | None:
     obs = INTERVALS
     if kernel == "k_simulate":
         return obs * paths * real                            # write every observation once
+    if kernel == "k_fit_bulk_step":
+        # two spot rows + every strike's value column read once; value writes happen only for
+        # paths that exercise and are data-dependent, so they are left out (a lower bound)
+        return 2 * paths * real + STRIKES * paths * real
     if kernel == "k_fit_prepare":
         return 2 * paths * real + 2 * STRIKES * paths * real  # two spot rows, values read+write
-    if kernel in ("k_fit_gram", "k_fit_moments"):
+    if kernel == "k_fit_moments":
         return paths * real + STRIKES * paths * real         # one spot row, values read
     if kernel == "k_fit_final":
         return paths * real + STRIKES * paths * real

@@ -528,7 +528,7 @@ static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<d
         g.count_base = static_cast<uint32_t>(count_slots);
         count_slots += static_cast<uint64_t>(ns) * g.obs;
         g.chunks_t = static_cast<uint32_t>((p.nt + kChunk - 1) / kChunk);
-        g.chunks_e = static_cast<uint32_t>((p.ne + kChunk - 1) / kChunk);
+        g.chunks_e = static_cast<uint32_t>((p.ne + kEvalChunk - 1) / kEvalChunk);
         g.part_base_t = static_cast<uint32_t>(part_t_slots);
         part_t_slots += static_cast<uint64_t>(ns) * g.chunks_t;
         g.part_base_e = static_cast<uint32_t>(part_e_slots);
@@ -787,19 +787,13 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                                                                         models, errors));
                 continue;
             }
-            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, f32, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
+            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
                                        ex->part_t.as<double>(), errors));
             PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
-            if (f32) {
-                PROF_LAUNCH("k_fit_gram", launch_fit_gram(st, f32, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
-                                        ex->part_t.as<double>(), errors));
-                PROF_LAUNCH("k_fit_reduce_gram", launch_fit_reduce_gram(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
-            } else {
-                for (int k = 0; k <= fp.degree; ++k) {
-                    PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, f32, t, fp, j, k, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr,
-                                               models, ex->part_t.as<double>(), errors));
-                    PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
-                }
+            for (int k = 0; k <= fp.degree; ++k) {
+                PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, t, fp, j, k, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr,
+                                           models, ex->part_t.as<double>(), errors));
+                PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
             }
         }
         PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,

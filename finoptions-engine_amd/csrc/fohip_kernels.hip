@@ -1,12 +1,15 @@
 // CDNA4 kernels of the LSM hot path: seeded simulation, backward-induction fit passes,
 // deterministic chunk reductions, and the fused regenerate-and-evaluate pass (with pathwise Greeks).
 //
-// Work decomposition: a work item is (group, 1024-path chunk). Fit and evaluation passes launch
-// 256-thread blocks over (work item, strike or strike tile); every thread owns paths
-// chunk*1024 + threadIdx.x + 256*k, k = 0..3, so HBM rows are read fully coalesced. All reductions
-// go thread -> wave (down-shuffle tree) -> block (fixed order) -> chunk list (fixed order in a
-// one-block-per-strike reduce kernel), which makes results independent of timing and of launch
-// tuning.
+// Work decomposition. A training work item is (group, 1024-path chunk); fit passes launch one
+// 256-thread block per (work item, strike) and every thread owns paths
+// chunk*1024 + threadIdx.x + 256*k, k = 0..3, so HBM rows are read fully coalesced. The block
+// index is mapped XCD-aware: the MI355X dispatcher deals workgroups round-robin over its 8 XCDs,
+// so every strike of one chunk is given a block id with the same residue mod 8 and the chunk's
+// spot rows are fetched into one XCD's L2 once and reused by all of its strikes.
+// Evaluation work items are (group, 256-path chunk) with one path per thread.
+// Reductions go thread -> wave (down-shuffle tree) -> block (fixed order) -> chunk list (fixed
+// order in a one-block-per-strike reduce kernel): results are independent of timing and tuning.
 #include <hip/hip_runtime.h>
 
 #include "fohip_device.hpp"
@@ -29,14 +32,24 @@ __device__ __forceinline__ void load_zig(double *sx, double *sf) {
 
 // Path-level failures only lower the row's key; the failure kind is a function of the key's stage
 // and is decoded on the host, so racing writers cannot leave a key/code mismatch.
-__device__ __forceinline__ void record_error(RowError *e, unsigned long long key) {
-    atomicMin(&e->key, key);
+__device__ __forceinline__ void record_error(RowError *e, unsigned long long key) { atomicMin(&e->key, key); }
+
+// XCD-aware decomposition of a 1-D grid into (work item, column): block b runs on XCD b mod 8;
+// item = 8 * ((b / 8) / columns) + b mod 8 and column = (b / 8) mod columns, so all columns of an
+// item share an XCD and run back to back there.
+__device__ __forceinline__ bool xcd_item_column(uint32_t nitems, uint32_t columns, uint32_t &item,
+                                                uint32_t &column) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t k = b >> 3;
+    column = k % columns;
+    item = (k / columns) * 8u + (b & 7u);
+    return item < nitems;
 }
 
 // ------------------------------------------------------------------------ random-stream probes
 
-__global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
-                                uint32_t blocks, uint32_t *out) {
+__global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths, uint32_t blocks,
+                                uint32_t *out) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= static_cast<uint64_t>(paths) * blocks) return;
     const uint32_t p = static_cast<uint32_t>(i / blocks), b = static_cast<uint32_t>(i % blocks);
@@ -48,8 +61,8 @@ __global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, 
     out[i * 4 + 3] = w.w;
 }
 
-__global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
-                                 uint32_t count, double *out, uint32_t *draws, int *fail) {
+__global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths, uint32_t count,
+                                 double *out, uint32_t *draws, int *fail) {
     __shared__ double sx[257], sf[257];
     load_zig(sx, sf);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,22 +80,14 @@ __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0,
 
 // ------------------------------------------------------------------------------ simulation
 
-template <typename Real> struct StepCoef;
-template <> struct StepCoef<double> {
-    __device__ static double drift(const DevTables &t, uint32_t i) { return t.log_drift[i]; }
-    __device__ static double diff(const DevTables &t, uint32_t i) { return t.diffusion[i]; }
-};
-template <> struct StepCoef<float> {
-    __device__ static float drift(const DevTables &t, uint32_t i) { return t.log_drift32[i]; }
-    __device__ static float diff(const DevTables &t, uint32_t i) { return t.diffusion32[i]; }
-};
-
 // seeded.rs:399-459: one thread per path; for each interval draw a normal, advance, then apply a
 // terminal cash event before recording expiry, or record a pre-event exercise state before a
-// same-node cash event. Output is the time-major training matrix.
-template <typename Real>
+// same-node cash event. The state evolves in binary64 for both methods; the FP32-bulk method
+// stores the observations in binary32 (a random-walk of binary32 transitions accumulates ~7e-7
+// relative error over 50 steps, which moves LSM stopping decisions; storage rounding does not).
+template <typename Store>
 __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *__restrict__ items,
-                                                     Real *__restrict__ paths_out, int evaluation,
+                                                     Store *__restrict__ paths_out, int evaluation,
                                                      unsigned long long *__restrict__ fail) {
     __shared__ double sx[257], sf[257];
     load_zig(sx, sf);
@@ -93,11 +98,11 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     if (p >= n) return;
     PathStream s;
     s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p);
-    Real *out = paths_out + g.path_base + p;
-    Real spot = static_cast<Real>(g.S0);
+    Store *out = paths_out + g.path_base + p;
+    double spot = g.S0;
     uint32_t row = 0;
     if (g.origin_row) {
-        out[0] = spot;
+        out[0] = static_cast<Store>(spot);
         row = 1;
     }
     bool ok = true;
@@ -108,9 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
             ok = false;
             break;
         }
-        Real growth;
-        const Real next = gbm_advance(spot, StepCoef<Real>::diff(t, g.step_base + i),
-                                      StepCoef<Real>::drift(t, g.step_base + i), static_cast<Real>(z), growth);
+        double growth;
+        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth);
         if (!isfinite(next)) {
             ok = false;
             break;
@@ -120,21 +124,19 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
         const uint32_t flags = t.node_flags[node];
         const bool cash = flags & kNodeCash;
         const bool terminal = flags & kNodeTerminal;
-        if (cash && terminal) spot = apply_cash<Real>(g.policy, static_cast<Real>(t.cash[node]), spot);
+        if (cash && terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
         if (flags & kNodeObserved) {
-            out[static_cast<uint64_t>(row) * n] = spot;
+            out[static_cast<uint64_t>(row) * n] = static_cast<Store>(spot);
             row += 1;
         }
-        if (cash && !terminal) spot = apply_cash<Real>(g.policy, static_cast<Real>(t.cash[node]), spot);
+        if (cash && !terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
     }
     if (!ok) atomicMin(fail + item.x, static_cast<unsigned long long>(p));
 }
 
 // ------------------------------------------------------------------------------- fit passes
 
-__device__ __forceinline__ bool row_dead(const RowError *err, uint32_t s) {
-    return err[s].key != kNoError;
-}
+__device__ __forceinline__ bool row_dead(const RowError *err, uint32_t s) { return err[s].key != kNoError; }
 
 template <typename Real>
 __device__ __forceinline__ Real intrinsic(Real spot, Real strike) {
@@ -142,81 +144,79 @@ __device__ __forceinline__ Real intrinsic(Real spot, Real strike) {
     return v > Real(0) ? v : Real(0);
 }
 
-// Step j of backward induction over exercise row r = obs - 2 - j:
+__device__ __forceinline__ double *partial_slot(double *partials, const DevGroup &g, uint32_t local, uint32_t chunk) {
+    return partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + chunk) * kPartialStride;
+}
+
+// FP64 step j of backward induction over exercise row r = obs - 2 - j:
 //   j == 0: cashflow = terminal intrinsic; j > 0: apply the fitted rule of row r + 1
 //   (policy.rs:570-601), then discount from node(r + 1) to node(r) (policy.rs:1049-1063),
 //   store, and accumulate Welford moments of moneyness S/K over in-the-money paths
 //   (regression.rs:279-297).
-template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams fp, uint32_t step,
-                                                        const uint2 *__restrict__ items,
-                                                        const Real *__restrict__ paths,
-                                                        Real *__restrict__ values,
-                                                        const Model *__restrict__ models,
-                                                        double *__restrict__ partials,
-                                                        RowError *__restrict__ errors) {
+                                                        const uint2 *__restrict__ items, uint32_t nitems,
+                                                        uint32_t columns, const double *__restrict__ paths,
+                                                        double *__restrict__ values, const Model *__restrict__ models,
+                                                        double *__restrict__ partials, RowError *__restrict__ errors) {
     __shared__ double scratch[3 * kWaves];
-    const uint2 item = items[blockIdx.x];
+    uint32_t wi, local;
+    if (!xcd_item_column(nitems, columns, wi, local)) return;
+    const uint2 item = items[wi];
     const DevGroup &g = t.groups[item.x];
-    const uint32_t local = blockIdx.y;
     if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
     const uint32_t srow = g.strike_begin + local;
     if (row_dead(errors, srow)) return;
     const uint32_t r = g.obs - 2 - step;
-    const double Kd = t.strikes[srow];
-    const Real K = static_cast<Real>(Kd);
+    const double K = t.strikes[srow];
     const uint32_t n = g.nt;
-    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
-    const Real *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
-    Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const double *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const double *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
+    double *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     const Model &m_next = models[g.model_base + local * g.obs + r + 1];
     const bool apply = step > 0 && m_next.kind == kRuleRegression;
-    const Real disc = static_cast<Real>(t.disc_step[g.row_base + r]);
+    const double disc = t.disc_step[g.row_base + r];
     Moments acc = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
         if (p >= n) break;
-        Real v;
+        double v;
         if (step == 0) {
             v = intrinsic(S_next[p], K);
         } else {
             v = V[p];
             if (apply) {
-                const Real s1 = S_next[p];
+                const double s1 = S_next[p];
                 if (s1 > K) {
-                    const Real x = s1 / K;
-                    const Real cont = predict(m_next, fp.degree, x);
+                    const double cont = predict(m_next, fp.degree, s1 / K);
                     if (!isfinite(cont)) {
                         record_error(&errors[srow], error_key(step - 1, kStageApply, p));
-                    } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
+                    } else if (should_exercise(fp.tie, s1 - K, cont)) {
                         v = s1 - K;
                     }
                 }
             }
         }
         v *= disc;
-        if (!isfinite(v))
-            record_error(&errors[srow], error_key(step, kStageDiscount, p));
+        if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
         V[p] = v;
-        const Real s0 = S_r[p];
+        const double s0 = S_r[p];
         if (s0 > K) {
-            const Real x = s0 / K;
-            if (!isfinite(x))
-                record_error(&errors[srow], error_key(step, kStagePredictor, p));
-            welford_push(acc, static_cast<double>(x));
+            const double x = s0 / K;
+            if (!isfinite(x)) record_error(&errors[srow], error_key(step, kStagePredictor, p));
+            welford_push(acc, x);
         }
     }
     acc = block_chan(acc, scratch);
     if (threadIdx.x == 0) {
-        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        double *out = partial_slot(partials, g, local, item.y);
         out[0] = acc.n;
         out[1] = acc.mean;
         out[2] = acc.m2;
     }
 }
 
-__device__ void fail_row(RowError *e, uint32_t step, int32_t code, int64_t index, double observed,
-                         double limit) {
+__device__ void fail_row(RowError *e, uint32_t step, int32_t code, int64_t index, double observed, double limit) {
     const unsigned long long key = error_key(step, kStageRegression, 0);
     if (key < e->key) {
         e->key = key;
@@ -227,11 +227,44 @@ __device__ void fail_row(RowError *e, uint32_t step, int32_t code, int64_t index
     }
 }
 
-// Support decision and predictor location/scale of one (strike, row) (policy.rs:480-529,
-// regression.rs:315-325).
+// Support decision of one (strike, row) from the in-the-money count (policy.rs:480-529). Returns
+// true when a regression is to be fitted.
+__device__ bool decide_support(Model &m, const FitParams &fp, RowError *e, uint32_t step, uint32_t count) {
+    const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
+    m.itm = count;
+    m.rank = 0;
+    m.condition = 0.0;
+    if (count == 0) {
+        m.kind = fp.support == 1 ? kRuleContinue : kRuleNoSupport;
+        return false;
+    }
+    if (count < required) {
+        if (fp.support == 1) {
+            m.kind = kRuleContinue;
+        } else {
+            m.kind = kRuleNone;
+            fail_row(e, step, FOHIP_E_INSUFFICIENT_OBSERVATIONS, count, required, fp.degree);
+        }
+        return false;
+    }
+    return true;
+}
+
+__device__ void begin_fit(Model &m, double mean, double scale) {
+    m.mean = mean;
+    m.scale = scale;
+    m.prev_norm = 1.0;
+#pragma unroll
+    for (int k = 0; k < kMaxDegree; ++k) m.alpha[k] = m.beta[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k <= kMaxDegree; ++k) m.coef[k] = m.norm[k] = 0.0;
+    m.kind = kRulePending;
+}
+
+// Predictor location/scale of one (strike, row) (regression.rs:315-325).
 __global__ __launch_bounds__(kBlock) void k_fit_reduce_welford(DevTables t, FitParams fp, uint32_t step,
                                                                const uint32_t *__restrict__ rows,
-                                                               const double *__restrict__ partials,
+                                                               double *__restrict__ partials,
                                                                Model *__restrict__ models,
                                                                RowError *__restrict__ errors) {
     __shared__ double scratch[3 * kWaves];
@@ -242,30 +275,13 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_welford(DevTables t, FitP
     const uint32_t r = g.obs - 2 - step;
     Moments acc = {0.0, 0.0, 0.0};
     for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
-        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        const double *in = partial_slot(partials, g, local, c);
         acc = chan_merge(acc, Moments{in[0], in[1], in[2]});
     }
     acc = block_chan(acc, scratch);
     if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs + r];
-    const uint32_t count = static_cast<uint32_t>(acc.n);
-    const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
-    m.itm = count;
-    m.rank = 0;
-    m.condition = 0.0;
-    if (count == 0) {
-        m.kind = fp.support == 1 ? kRuleContinue : kRuleNoSupport;
-        return;
-    }
-    if (count < required) {
-        if (fp.support == 1) {
-            m.kind = kRuleContinue;
-        } else {
-            m.kind = kRuleNone;
-            fail_row(&errors[srow], step, FOHIP_E_INSUFFICIENT_OBSERVATIONS, count, required, fp.degree);
-        }
-        return;
-    }
+    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(acc.n))) return;
     double mean = 0.0, scale = 1.0;
     if (fp.degree != 0) {
         scale = sqrt(acc.m2 / acc.n);
@@ -276,29 +292,23 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_welford(DevTables t, FitP
         }
         mean = acc.mean;
     }
-    m.mean = mean;
-    m.scale = scale;
-    m.prev_norm = 1.0;
-    for (int k = 0; k < kMaxDegree; ++k) m.alpha[k] = m.beta[k] = 0.0;
-    for (int k = 0; k <= kMaxDegree; ++k) m.coef[k] = m.norm[k] = 0.0;
-    m.kind = kRulePending;
+    begin_fit(m, mean, scale);
 }
 
 // Moments of the basis vector p_k (runtime/src/lib.rs:84-95, 141-158): p_k is rebuilt from the
 // standardized predictor with the recurrence coefficients fitted so far, so no basis vector is
 // stored between passes.
-template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_fit_moments(DevTables t, FitParams fp, uint32_t step, int k,
-                                                        const uint2 *__restrict__ items,
-                                                        const Real *__restrict__ paths,
-                                                        const Real *__restrict__ values,
-                                                        const Model *__restrict__ models,
-                                                        double *__restrict__ partials,
+                                                        const uint2 *__restrict__ items, uint32_t nitems,
+                                                        uint32_t columns, const double *__restrict__ paths,
+                                                        const double *__restrict__ values,
+                                                        const Model *__restrict__ models, double *__restrict__ partials,
                                                         const RowError *__restrict__ errors) {
     __shared__ double scratch[3 * kWaves];
-    const uint2 item = items[blockIdx.x];
+    uint32_t wi, local;
+    if (!xcd_item_column(nitems, columns, wi, local)) return;
+    const uint2 item = items[wi];
     const DevGroup &g = t.groups[item.x];
-    const uint32_t local = blockIdx.y;
     if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
     const uint32_t srow = g.strike_begin + local;
     const uint32_t r = g.obs - 2 - step;
@@ -306,17 +316,18 @@ __global__ __launch_bounds__(kBlock) void k_fit_moments(DevTables t, FitParams f
     if (row_dead(errors, srow) || m.kind != kRulePending) return;
     const double K = t.strikes[srow];
     const uint32_t n = g.nt;
-    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
-    const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const double *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const double *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     double sums[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int q = 0; q < kPathsPerThread; ++q) {
         const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
         if (p >= n) break;
-        const double s0 = static_cast<double>(S_r[p]);
+        const double s0 = S_r[p];
         if (!(s0 > K)) continue;
         const double x = s0 / K;
         const double z = fp.degree == 0 ? 0.0 : (x - m.mean) / m.scale;
-        const double y = static_cast<double>(V[p]);
+        const double y = V[p];
         double previous = 0.0, current = 1.0;
         for (int j = 0; j < k; ++j) {
             const double next = fma(z - m.alpha[j], current, -m.beta[j] * previous);
@@ -330,31 +341,40 @@ __global__ __launch_bounds__(kBlock) void k_fit_moments(DevTables t, FitParams f
     }
     block_sum<3>(sums, scratch);
     if (threadIdx.x == 0) {
-        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        double *out = partial_slot(partials, g, local, item.y);
         out[0] = sums[0];
         out[1] = sums[1];
         out[2] = sums[2];
     }
 }
 
-// ---- 5x5 conditioning diagnostic (regression.rs:540-577, 711-850), one thread per fit
+// ---- 5x5 conditioning diagnostic (regression.rs:540-577, 711-850), one thread per fit.
+// Loops run over the fixed maximum of five columns with the degree as a guard, so every array
+// stays in registers.
 
-__device__ int orthonormal_transform(int degree, const Model &m, double *tr) {
+__device__ int orthonormal_transform(int degree, const Model &m, double (&tr)[25]) {
     double previous[5] = {0, 0, 0, 0, 0}, current[5] = {1, 0, 0, 0, 0};
+#pragma unroll
     for (int i = 0; i < 25; ++i) tr[i] = 0.0;
-    for (int k = 0; k <= degree; ++k) {
+#pragma unroll
+    for (int k = 0; k <= kMaxDegree; ++k) {
+        if (k > degree) break;
         const double inverse_norm = 1.0 / sqrt(m.norm[k]);
         if (!isfinite(inverse_norm)) return k + 1;
+#pragma unroll
         for (int j = 0; j <= k; ++j) tr[k * 5 + j] = current[j] * inverse_norm;
         if (k == degree) break;
         double next[5] = {0, 0, 0, 0, 0};
+#pragma unroll
         for (int j = 0; j <= k; ++j) {
             next[j + 1] += current[j];
             next[j] -= m.alpha[k] * current[j];
             next[j] -= m.beta[k] * previous[j];
         }
+#pragma unroll
         for (int j = 0; j < 5; ++j)
             if (!isfinite(next[j])) return k + 2;
+#pragma unroll
         for (int j = 0; j < 5; ++j) {
             previous[j] = current[j];
             current[j] = next[j];
@@ -363,15 +383,19 @@ __device__ int orthonormal_transform(int degree, const Model &m, double *tr) {
     return 0;
 }
 
-__device__ int jacobi_singular_values(double *mat, int columns, double *sv) {
+template <int COLUMNS>
+__device__ int jacobi_singular_values(double (&mat)[25], double (&sv)[5]) {
     const double tol = 16.0 * kEpsilon;
     bool converged = false;
     for (int sweep = 0; sweep < 32; ++sweep) {
         converged = true;
-        for (int left = 0; left < columns - 1; ++left) {
-            for (int right = left + 1; right < columns; ++right) {
+#pragma unroll
+        for (int left = 0; left < COLUMNS - 1; ++left) {
+#pragma unroll
+            for (int right = left + 1; right < COLUMNS; ++right) {
                 double ln = 0.0, rn = 0.0, cross = 0.0;
-                for (int row = 0; row < columns; ++row) {
+#pragma unroll
+                for (int row = 0; row < COLUMNS; ++row) {
                     const double lv = mat[left * 5 + row], rv = mat[right * 5 + row];
                     ln = fma(lv, lv, ln);
                     rn = fma(rv, rv, rn);
@@ -381,15 +405,11 @@ __device__ int jacobi_singular_values(double *mat, int columns, double *sv) {
                 if (fabs(cross) <= tol * sqrt(ln * rn)) continue;
                 converged = false;
                 const double zeta = (rn - ln) / (2.0 * cross);
-                double tangent;
-                if (zeta == 0.0) {
-                    tangent = 1.0;
-                } else {
-                    tangent = copysign(1.0, zeta) / (fabs(zeta) + hypot(zeta, 1.0));
-                }
+                const double tangent = zeta == 0.0 ? 1.0 : copysign(1.0, zeta) / (fabs(zeta) + hypot(zeta, 1.0));
                 const double cosine = 1.0 / sqrt(fma(tangent, tangent, 1.0));
                 const double sine = cosine * tangent;
-                for (int row = 0; row < columns; ++row) {
+#pragma unroll
+                for (int row = 0; row < COLUMNS; ++row) {
                     const double lv = mat[left * 5 + row], rv = mat[right * 5 + row];
                     mat[left * 5 + row] = fma(cosine, lv, -sine * rv);
                     mat[right * 5 + row] = fma(sine, lv, cosine * rv);
@@ -399,24 +419,27 @@ __device__ int jacobi_singular_values(double *mat, int columns, double *sv) {
         if (converged) break;
     }
     if (!converged) return -1;
+#pragma unroll
     for (int c = 0; c < 5; ++c) sv[c] = 0.0;
-    for (int c = 0; c < columns; ++c) {
+#pragma unroll
+    for (int c = 0; c < COLUMNS; ++c) {
         double sq = 0.0;
-        for (int row = 0; row < columns; ++row) {
+#pragma unroll
+        for (int row = 0; row < COLUMNS; ++row) {
             const double v = mat[c * 5 + row];
             sq += v * v;
         }
         sv[c] = sqrt(sq);
     }
-    for (int i = 1; i < columns; ++i) {   // descending insertion sort
-        const double v = sv[i];
-        int j = i - 1;
-        while (j >= 0 && sv[j] < v) {
-            sv[j + 1] = sv[j];
-            --j;
+    // descending order (sorting network over the fixed width)
+#pragma unroll
+    for (int i = 0; i < COLUMNS; ++i)
+#pragma unroll
+        for (int j = 0; j + 1 < COLUMNS - i; ++j) {
+            const double a = sv[j], b = sv[j + 1];
+            sv[j] = a < b ? b : a;
+            sv[j + 1] = a < b ? a : b;
         }
-        sv[j + 1] = v;
-    }
     return 0;
 }
 
@@ -430,7 +453,14 @@ __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, 
         fail_row(e, step, FOHIP_E_NON_FINITE_FIT, bad - 1, 0.0, 0.0);
         return;
     }
-    const int jr = jacobi_singular_values(tr, columns, sv);
+    int jr = 0;
+    switch (columns) {
+        case 1: jr = jacobi_singular_values<1>(tr, sv); break;
+        case 2: jr = jacobi_singular_values<2>(tr, sv); break;
+        case 3: jr = jacobi_singular_values<3>(tr, sv); break;
+        case 4: jr = jacobi_singular_values<4>(tr, sv); break;
+        default: jr = jacobi_singular_values<5>(tr, sv); break;
+    }
     if (jr < 0) {
         m.kind = kRuleNone;
         fail_row(e, step, FOHIP_E_JACOBI_NOT_CONVERGED, columns, 0.0, 0.0);
@@ -443,13 +473,18 @@ __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, 
     }
     const double largest = sv[0], threshold = fp.rank_tol * largest;
     int rank = 0;
-    for (int c = 0; c < columns; ++c) rank += sv[c] > threshold ? 1 : 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) rank += (c < columns && sv[c] > threshold) ? 1 : 0;
     if (rank < columns) {
         m.kind = kRuleNone;
         fail_row(e, step, FOHIP_E_RANK_DEFICIENT, rank, columns, fp.rank_tol);
         return;
     }
-    const double condition = largest / sv[columns - 1];
+    double smallest = sv[0];
+#pragma unroll
+    for (int c = 1; c < 5; ++c)
+        if (c < columns) smallest = sv[c];
+    const double condition = largest / smallest;
     if (!isfinite(condition)) {
         m.kind = kRuleNone;
         fail_row(e, step, FOHIP_E_NON_FINITE_FIT, fp.degree, 0.0, 0.0);
@@ -466,8 +501,8 @@ __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, 
 }
 
 // One Stieltjes step (regression.rs:448-514) from the reduced moments of p_k.
-__device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint32_t step, int k,
-                               double norm, double projection, double weighted) {
+__device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint32_t step, int k, double norm,
+                               double projection, double weighted) {
     if (!isfinite(norm) || !isfinite(projection) || !isfinite(weighted)) {
         m.kind = kRuleNone;
         fail_row(e, step, FOHIP_E_NON_FINITE_FIT, k, 0.0, 0.0);
@@ -501,7 +536,7 @@ __device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint3
 
 __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitParams fp, uint32_t step, int k,
                                                                const uint32_t *__restrict__ rows,
-                                                               const double *__restrict__ partials,
+                                                               double *__restrict__ partials,
                                                                Model *__restrict__ models,
                                                                RowError *__restrict__ errors) {
     __shared__ double scratch[3 * kWaves];
@@ -514,118 +549,13 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
     if (row_dead(errors, srow) || m.kind != kRulePending) return;
     double sums[3] = {0.0, 0.0, 0.0};
     for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
-        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        const double *in = partial_slot(partials, g, local, c);
         sums[0] += in[0];
         sums[1] += in[1];
         sums[2] += in[2];
     }
     block_sum<3>(sums, scratch);
     if (threadIdx.x == 0) stieltjes_step(m, fp, &errors[srow], step, k, sums[0], sums[1], sums[2]);
-}
-
-// FP32-bulk fit: one pass accumulating, in FP64, the power sums sum z^j (j <= 2d) and
-// sum y z^j (j <= d) of the standardized FP32 predictor (the Hankel Gram matrix and right-hand
-// side of the normal equations).
-template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_fit_gram(DevTables t, FitParams fp, uint32_t step,
-                                                     const uint2 *__restrict__ items,
-                                                     const Real *__restrict__ paths,
-                                                     const Real *__restrict__ values,
-                                                     const Model *__restrict__ models,
-                                                     double *__restrict__ partials,
-                                                     const RowError *__restrict__ errors) {
-    __shared__ double scratch[14 * kWaves];
-    const uint2 item = items[blockIdx.x];
-    const DevGroup &g = t.groups[item.x];
-    const uint32_t local = blockIdx.y;
-    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
-    const uint32_t srow = g.strike_begin + local;
-    const uint32_t r = g.obs - 2 - step;
-    const Model &m = models[g.model_base + local * g.obs + r];
-    if (row_dead(errors, srow) || m.kind != kRulePending) return;
-    const Real K = static_cast<Real>(t.strikes[srow]);
-    const Real mean = static_cast<Real>(m.mean), inv_scale = Real(1) / static_cast<Real>(m.scale);
-    const uint32_t n = g.nt;
-    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
-    const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
-    double acc[14];
-#pragma unroll
-    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    for (int q = 0; q < kPathsPerThread; ++q) {
-        const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
-        if (p >= n) break;
-        const Real s0 = S_r[p];
-        if (!(s0 > K)) continue;
-        const Real z = fp.degree == 0 ? Real(0) : (s0 / K - mean) * inv_scale;
-        const Real y = V[p];
-        Real pw = Real(1);
-#pragma unroll
-        for (int j = 0; j <= 8; ++j) {
-            acc[j] += static_cast<double>(pw);
-            if (j <= 4) acc[9 + j] += static_cast<double>(y * pw);
-            pw *= z;
-        }
-    }
-    block_sum<14>(acc, scratch);
-    if (threadIdx.x == 0) {
-        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
-#pragma unroll
-        for (int i = 0; i < 14; ++i) out[i] = acc[i];
-    }
-}
-
-// Stieltjes recurrence from the power sums: the monic orthogonal basis p_k is carried as monomial
-// coefficients, norms and alpha use the Hankel moments, projections the right-hand side. This is
-// the LDL^T factorization of the Gram matrix in orthogonal-polynomial form, so the fitted rule has
-// exactly the representation (alpha, beta, coef) of the FP64 method.
-__global__ __launch_bounds__(kBlock) void k_fit_reduce_gram(DevTables t, FitParams fp, uint32_t step,
-                                                            const uint32_t *__restrict__ rows,
-                                                            const double *__restrict__ partials,
-                                                            Model *__restrict__ models,
-                                                            RowError *__restrict__ errors) {
-    __shared__ double scratch[14 * kWaves];
-    const uint32_t srow = rows[blockIdx.x];
-    const DevGroup &g = t.groups[t.strike_group[srow]];
-    if (step >= g.fit_rows) return;
-    const uint32_t local = srow - g.strike_begin;
-    const uint32_t r = g.obs - 2 - step;
-    Model &m = models[g.model_base + local * g.obs + r];
-    if (row_dead(errors, srow) || m.kind != kRulePending) return;
-    double acc[14];
-    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
-        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
-        for (int i = 0; i < 14; ++i) acc[i] += in[i];
-    }
-    block_sum<14>(acc, scratch);
-    if (threadIdx.x != 0) return;
-    const double *mu = acc;        // sum z^j, j = 0..8
-    const double *nu = acc + 9;    // sum y z^j, j = 0..4
-    double a_prev[5] = {0, 0, 0, 0, 0}, a_cur[5] = {1, 0, 0, 0, 0};
-    for (int k = 0; k <= fp.degree; ++k) {
-        double norm = 0.0, weighted = 0.0, projection = 0.0;
-        for (int i = 0; i <= k; ++i) {
-            projection += a_cur[i] * nu[i];
-            for (int j = 0; j <= k; ++j) {
-                norm += a_cur[i] * a_cur[j] * mu[i + j];
-                if (k < fp.degree) weighted += a_cur[i] * a_cur[j] * mu[i + j + 1];
-            }
-        }
-        stieltjes_step(m, fp, &errors[srow], step, k, norm, projection, weighted);
-        if (m.kind != kRulePending && m.kind != kRuleRegression) return;
-        if (k < fp.degree) {
-            double a_next[5] = {0, 0, 0, 0, 0};
-            for (int i = 0; i <= k; ++i) {
-                a_next[i + 1] += a_cur[i];
-                a_next[i] -= m.alpha[k] * a_cur[i];
-                a_next[i] -= m.beta[k] * a_prev[i];
-            }
-            for (int i = 0; i < 5; ++i) {
-                a_prev[i] = a_cur[i];
-                a_cur[i] = a_next[i];
-            }
-        }
-    }
 }
 
 // FP32-bulk backward step j over exercise row r = obs - 2 - j in ONE pass over HBM.
@@ -637,47 +567,56 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_gram(DevTables t, FitPara
 // in-the-money paths around a pivot c (row r + 1's fitted mean, else 1). The reduce kernel
 // recentres them on the exact mean, which merges the Welford pass (regression.rs:279-297) and the
 // Gram pass into one read of each value column.
-template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
-                                                          const uint2 *__restrict__ items,
-                                                          const Real *__restrict__ paths,
-                                                          Real *__restrict__ values,
-                                                          const Model *__restrict__ models,
+                                                          const uint2 *__restrict__ items, uint32_t nitems,
+                                                          uint32_t columns, const float *__restrict__ paths,
+                                                          float *__restrict__ values, const Model *__restrict__ models,
                                                           double *__restrict__ partials,
                                                           RowError *__restrict__ errors) {
     __shared__ double scratch[14 * kWaves];
-    const uint2 item = items[blockIdx.x];
+    uint32_t wi, local;
+    if (!xcd_item_column(nitems, columns, wi, local)) return;
+    const uint2 item = items[wi];
     const DevGroup &g = t.groups[item.x];
-    const uint32_t local = blockIdx.y;
     if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
     const uint32_t srow = g.strike_begin + local;
     if (row_dead(errors, srow)) return;
     const uint32_t r = g.obs - 2 - step;
-    const Real K = static_cast<Real>(t.strikes[srow]);
+    const float K = static_cast<float>(t.strikes[srow]);
     const uint32_t n = g.nt;
-    const Real *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
-    const Real *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
-    Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
+    const float *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
+    const float *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
+    float *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     const Model &m_next = models[g.model_base + local * g.obs + r + 1];
     const bool apply = step > 0 && m_next.kind == kRuleRegression;
     const double pivot = apply ? m_next.mean : 1.0;
-    const Real d_next = static_cast<Real>(t.disc0[g.row_base + r + 1]);
-    const Real inv_d = static_cast<Real>(1.0 / t.disc0[g.row_base + r]);
+    const float d_next = static_cast<float>(t.disc0[g.row_base + r + 1]);
+    const float inv_d = static_cast<float>(1.0 / t.disc0[g.row_base + r]);
     double acc[14];
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+    float s1v[kPathsPerThread], s0v[kPathsPerThread], vv[kPathsPerThread];
+#pragma unroll
+    for (int k = 0; k < kPathsPerThread; ++k) {   // issue every load of the thread first
+        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
+        const bool in = p < n;
+        s1v[k] = in ? S_next[p] : 0.0f;
+        s0v[k] = in ? S_r[p] : 0.0f;
+        vv[k] = (in && step > 0) ? V[p] : 0.0f;
+    }
+#pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
         if (p >= n) break;
-        Real v;
-        const Real s1 = S_next[p];
+        float v;
+        const float s1 = s1v[k];
         if (step == 0) {
             v = intrinsic(s1, K) * d_next;
             V[p] = v;
         } else {
-            v = V[p];
+            v = vv[k];
             if (apply && s1 > K) {
-                const Real cont = predict(m_next, fp.degree, s1 / K);
+                const float cont = predict(m_next, fp.degree, s1 / K);
                 if (!isfinite(cont)) {
                     record_error(&errors[srow], error_key(step - 1, kStageApply, p));
                 } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
@@ -687,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
             }
         }
         if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
-        const Real s0 = S_r[p];
+        const float s0 = s0v[k];
         if (s0 > K) {
             const double x = static_cast<double>(s0 / K);
             const double y = static_cast<double>(v * inv_d);
@@ -703,18 +642,68 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
     }
     block_sum<14>(acc, scratch);
     if (threadIdx.x == 0) {
-        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
+        double *out = partial_slot(partials, g, local, item.y);
 #pragma unroll
         for (int i = 0; i < 14; ++i) out[i] = acc[i];
         out[14] = pivot;
     }
 }
 
-// Support decision, exact location/scale and the standardized moments from the pivoted power
-// sums (binomial recentring), then the same Stieltjes-from-moments solve as k_fit_reduce_gram.
+// Stieltjes recurrence from standardized power moments mu_j = sum z^j (j <= 2d) and projections
+// nu_j = sum y z^j (j <= d): the monic orthogonal basis p_k is carried as monomial coefficients,
+// norms and alpha use the Hankel moments, projections the right-hand side. This is the LDL^T
+// factorization of the Gram matrix in orthogonal-polynomial form, so the fitted rule has exactly
+// the representation (alpha, beta, coef) of the FP64 method.
+__device__ void stieltjes_from_moments(Model &m, const FitParams &fp, RowError *e, uint32_t step,
+                                       const double (&mu)[9], const double (&nu)[5]) {
+    double a_prev[5] = {0, 0, 0, 0, 0}, a_cur[5] = {1, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k <= kMaxDegree; ++k) {
+        if (k > fp.degree) break;
+        double norm = 0.0, weighted = 0.0, projection = 0.0;
+#pragma unroll
+        for (int i = 0; i <= k; ++i) {
+            projection += a_cur[i] * nu[i];
+#pragma unroll
+            for (int j = 0; j <= k; ++j) {
+                norm += a_cur[i] * a_cur[j] * mu[i + j];
+                weighted += a_cur[i] * a_cur[j] * mu[i + j + 1 < 9 ? i + j + 1 : 8];
+            }
+        }
+        stieltjes_step(m, fp, e, step, k, norm, projection, weighted);
+        if (m.kind != kRulePending) return;
+        double a_next[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < kMaxDegree; ++i) {
+            if (i <= k) {
+                a_next[i + 1] += a_cur[i];
+                a_next[i] -= m.alpha[k] * a_cur[i];
+                a_next[i] -= m.beta[k] * a_prev[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            a_prev[i] = a_cur[i];
+            a_cur[i] = a_next[i];
+        }
+    }
+}
+
+// Binomial coefficients C(j, k), j, k <= 8.
+__device__ __forceinline__ double binom9(int j, int k) {
+    constexpr double c[9][9] = {{1, 0, 0, 0, 0, 0, 0, 0, 0},       {1, 1, 0, 0, 0, 0, 0, 0, 0},
+                                {1, 2, 1, 0, 0, 0, 0, 0, 0},       {1, 3, 3, 1, 0, 0, 0, 0, 0},
+                                {1, 4, 6, 4, 1, 0, 0, 0, 0},       {1, 5, 10, 10, 5, 1, 0, 0, 0},
+                                {1, 6, 15, 20, 15, 6, 1, 0, 0},    {1, 7, 21, 35, 35, 21, 7, 1, 0},
+                                {1, 8, 28, 56, 70, 56, 28, 8, 1}};
+    return c[j][k];
+}
+
+// Support decision, exact location/scale and standardized moments from the pivoted power sums
+// (binomial recentring), then the Stieltjes solve above.
 __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitParams fp, uint32_t step,
                                                             const uint32_t *__restrict__ rows,
-                                                            const double *__restrict__ partials,
+                                                            double *__restrict__ partials,
                                                             Model *__restrict__ models,
                                                             RowError *__restrict__ errors) {
     __shared__ double scratch[14 * kWaves];
@@ -724,42 +713,26 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
     const uint32_t local = srow - g.strike_begin;
     const uint32_t r = g.obs - 2 - step;
     double acc[14];
+#pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    double pivot = 1.0;
     for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
-        const double *in = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride;
+        const double *in = partial_slot(partials, g, local, c);
+#pragma unroll
         for (int i = 0; i < 14; ++i) acc[i] += in[i];
     }
-    pivot = partials[static_cast<uint64_t>(g.part_base_t + local * g.chunks_t) * kPartialStride + 14];
+    const double pivot = partial_slot(partials, g, local, 0)[14];
     block_sum<14>(acc, scratch);
     if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs + r];
     const double nd = acc[0];
-    const uint32_t count = static_cast<uint32_t>(nd);
-    const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
-    m.itm = count;
-    m.rank = 0;
-    m.condition = 0.0;
-    if (count == 0) {
-        m.kind = fp.support == 1 ? kRuleContinue : kRuleNoSupport;
-        return;
-    }
-    if (count < required) {
-        if (fp.support == 1) {
-            m.kind = kRuleContinue;
-        } else {
-            m.kind = kRuleNone;
-            fail_row(&errors[srow], step, FOHIP_E_INSUFFICIENT_OBSERVATIONS, count, required, fp.degree);
-        }
-        return;
-    }
+    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(nd))) return;
     double mu[9], nu[5];
     double mean = 0.0, scale = 1.0;
     if (fp.degree == 0) {
-        mu[0] = nd;
-        nu[0] = acc[9];
-        for (int j = 1; j < 9; ++j) mu[j] = 0.0;
-        for (int j = 1; j < 5; ++j) nu[j] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) mu[j] = j == 0 ? nd : 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) nu[j] = j == 0 ? acc[9] : 0.0;
     } else {
         const double d = acc[1] / nd;
         mean = pivot + d;
@@ -770,23 +743,20 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
             fail_row(&errors[srow], step, FOHIP_E_DEGENERATE_PREDICTOR, fp.degree, scale, 0.0);
             return;
         }
-        // sum (x - mean)^j = sum_k C(j, k) (-d)^(j - k) P_k, then divide by scale^j
-        double binom[9][9];
-        for (int a = 0; a < 9; ++a)
-            for (int b = 0; b < 9; ++b) binom[a][b] = 0.0;
-        for (int a = 0; a < 9; ++a) {
-            binom[a][0] = 1.0;
-            for (int b = 1; b <= a; ++b) binom[a][b] = binom[a - 1][b - 1] + (b <= a - 1 ? binom[a - 1][b] : 0.0);
-        }
+        // sum (x - mean)^j = sum_k C(j, k) (-d)^(j - k) P_k, divided by scale^j
         double negd[9];
         negd[0] = 1.0;
+#pragma unroll
         for (int j = 1; j < 9; ++j) negd[j] = negd[j - 1] * (-d);
-        double inv_scale = 1.0 / scale, sp = 1.0;
+        const double inv_scale = 1.0 / scale;
+        double sp = 1.0;
+#pragma unroll
         for (int j = 0; j < 9; ++j) {
             double q = 0.0, gq = 0.0;
+#pragma unroll
             for (int k = 0; k <= j; ++k) {
-                q += binom[j][k] * negd[j - k] * acc[k];
-                if (j < 5) gq += binom[j][k] * negd[j - k] * acc[9 + k];
+                q += binom9(j, k) * negd[j - k] * acc[k];
+                if (j < 5) gq += binom9(j, k) * negd[j - k] * acc[9 + k];
             }
             mu[j] = q * sp;
             if (j < 5) nu[j] = gq * sp;
@@ -795,70 +765,40 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
         mu[0] = nd;
         mu[1] = 0.0;   // exactly centred
     }
-    m.mean = mean;
-    m.scale = scale;
-    m.prev_norm = 1.0;
-    for (int k = 0; k < kMaxDegree; ++k) m.alpha[k] = m.beta[k] = 0.0;
-    for (int k = 0; k <= kMaxDegree; ++k) m.coef[k] = m.norm[k] = 0.0;
-    m.kind = kRulePending;
-    double a_prev[5] = {0, 0, 0, 0, 0}, a_cur[5] = {1, 0, 0, 0, 0};
-    for (int k = 0; k <= fp.degree; ++k) {
-        double norm = 0.0, weighted = 0.0, projection = 0.0;
-        for (int i = 0; i <= k; ++i) {
-            projection += a_cur[i] * nu[i];
-            for (int j = 0; j <= k; ++j) {
-                norm += a_cur[i] * a_cur[j] * mu[i + j];
-                if (k < fp.degree) weighted += a_cur[i] * a_cur[j] * mu[i + j + 1];
-            }
-        }
-        stieltjes_step(m, fp, &errors[srow], step, k, norm, projection, weighted);
-        if (m.kind != kRulePending && m.kind != kRuleRegression) return;
-        if (k < fp.degree) {
-            double a_next[5] = {0, 0, 0, 0, 0};
-            for (int i = 0; i <= k; ++i) {
-                a_next[i + 1] += a_cur[i];
-                a_next[i] -= m.alpha[k] * a_cur[i];
-                a_next[i] -= m.beta[k] * a_prev[i];
-            }
-            for (int i = 0; i < 5; ++i) {
-                a_prev[i] = a_cur[i];
-                a_cur[i] = a_next[i];
-            }
-        }
-    }
+    begin_fit(m, mean, scale);
+    stieltjes_from_moments(m, fp, &errors[srow], step, mu, nu);
 }
 
 // After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
-// cashflows for the valuation-origin decision (policy.rs:267-281).
+// cashflows for the valuation-origin decision (policy.rs:267-281). Stored values are either
+// discounted to the last fitted node (FP64 method, the reference's representation) or already
+// discounted to the origin (FP32 bulk method).
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp, const uint2 *__restrict__ items,
-                                                      const Real *__restrict__ paths,
-                                                      const Real *__restrict__ values,
-                                                      const Model *__restrict__ models,
-                                                      double *__restrict__ partials,
-                                                      RowError *__restrict__ errors,
+                                                      uint32_t nitems, uint32_t columns, const Real *__restrict__ paths,
+                                                      const Real *__restrict__ values, const Model *__restrict__ models,
+                                                      double *__restrict__ partials, RowError *__restrict__ errors,
                                                       int origin_discounted) {
     __shared__ double scratch[kWaves];
-    const uint2 item = items[blockIdx.x];
+    uint32_t wi, local;
+    if (!xcd_item_column(nitems, columns, wi, local)) return;
+    const uint2 item = items[wi];
     const DevGroup &g = t.groups[item.x];
-    const uint32_t local = blockIdx.y;
     if (g.strike_begin + local >= g.strike_end) return;
     const uint32_t srow = g.strike_begin + local;
     if (row_dead(errors, srow)) return;
-    const double Kd = t.strikes[srow];
-    const Real K = static_cast<Real>(Kd);
+    const Real K = static_cast<Real>(t.strikes[srow]);
     const uint32_t n = g.nt;
     const uint32_t last_row = g.origin_row;   // smallest positive-time exercise row
     const Real *S_last = paths + g.path_base + static_cast<uint64_t>(g.fit_rows ? last_row : g.obs - 1) * n;
     const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     const Model &m = models[g.model_base + local * g.obs + last_row];
     const bool apply = g.fit_rows > 0 && m.kind == kRuleRegression;
-    // Stored values are either discounted to the last fitted node (FP64 method, the reference's
-    // representation) or already discounted to the origin (FP32 bulk method).
     const bool stored_at_origin = origin_discounted && g.fit_rows > 0;
     const Real disc = stored_at_origin ? Real(1) : static_cast<Real>(g.disc_final);
     const Real d_last = stored_at_origin ? static_cast<Real>(t.disc0[g.row_base + last_row]) : Real(1);
     double sum[1] = {0.0};
+#pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
         if (p >= n) break;
@@ -880,20 +820,16 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
             }
         }
         v *= disc;
-        if (!isfinite(v))
-            record_error(&errors[srow], error_key(g.fit_rows, kStageDiscount, p));
+        if (!isfinite(v)) record_error(&errors[srow], error_key(g.fit_rows, kStageDiscount, p));
         sum[0] += static_cast<double>(v);
     }
     block_sum<1>(sum, scratch);
-    if (threadIdx.x == 0 && g.origin_row) {
-        double *out = partials + static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + item.y) * kPartialStride;
-        out[0] = sum[0];
-    }
+    if (threadIdx.x == 0 && g.origin_row) partial_slot(partials, g, local, item.y)[0] = sum[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitParams fp,
                                                               const uint32_t *__restrict__ rows,
-                                                              const double *__restrict__ partials,
+                                                              double *__restrict__ partials,
                                                               Model *__restrict__ models,
                                                               RowError *__restrict__ errors) {
     __shared__ double scratch[kWaves];
@@ -902,8 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
     if (!g.origin_row || row_dead(errors, srow)) return;
     const uint32_t local = srow - g.strike_begin;
     double sum[1] = {0.0};
-    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock)
-        sum[0] += partials[static_cast<uint64_t>(g.part_base_t + local * g.chunks_t + c) * kPartialStride];
+    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) sum[0] += partial_slot(partials, g, local, c)[0];
     block_sum<1>(sum, scratch);
     if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs];
@@ -917,78 +852,70 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
 
 // ------------------------------------------------------------------------------- evaluation
 
-// Fused regenerate-and-evaluate (policy.rs:642-736): each thread regenerates its evaluation
-// paths from the Philox counters, applies every strike's frozen rule in row order, and reduces
-// discounted payoffs per strike. With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along
-// the same transitions and the stopping row of each strike is held fixed (pathwise estimator).
+// Fused regenerate-and-evaluate (policy.rs:642-736). One evaluation path per thread, 256-path
+// chunks, a tile of TS strikes per block: the thread regenerates its path from the Philox
+// counters (binary64 state), applies every strike's frozen rule in row order (Real arithmetic for
+// moneyness, basis and payoff), and the block reduces each strike's discounted payoff moments.
+// With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along the transitions and each
+// strike's stopping row is held fixed (pathwise estimator under the frozen policy).
 template <typename Real, int TS, bool GREEKS>
 __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
-                                                     const Model *__restrict__ models,
-                                                     double *__restrict__ partials,
-                                                     uint32_t *__restrict__ counts,
-                                                     uint32_t *__restrict__ nosupport,
-                                                     RowError *__restrict__ errors,
-                                                     unsigned long long *__restrict__ sim_fail,
-                                                     int use_lds_hist) {
+                                                     uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
+                                                     double *__restrict__ partials, uint32_t *__restrict__ counts,
+                                                     uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
+                                                     unsigned long long *__restrict__ sim_fail, int use_lds_hist) {
     extern __shared__ uint32_t hist[];   // [TS][obs] when use_lds_hist
     __shared__ double sx[257], sf[257];
     constexpr int NACC = GREEKS ? 10 : 2;
-    __shared__ double scratch[NACC * TS * kWaves];
-    const uint2 item = items[blockIdx.x];
+    __shared__ double scratch[NACC * kWaves];
+    uint32_t wi, tile;
+    if (!xcd_item_column(nitems, tiles, wi, tile)) return;
+    const uint2 item = items[wi];
     const DevGroup &g = t.groups[item.x];
-    const uint32_t s_first = g.strike_begin + blockIdx.y * TS;
+    const uint32_t s_first = g.strike_begin + tile * TS;
     if (s_first >= g.strike_end) return;
     const int ns = static_cast<int>(min(static_cast<uint32_t>(TS), g.strike_end - s_first));
     const uint32_t obs = g.obs;
+    const uint32_t local0 = s_first - g.strike_begin;
     load_zig(sx, sf);
     if (use_lds_hist) {
         for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
         __syncthreads();
     }
     Real K[TS];
-    bool live[TS];
+    uint32_t live = 0;
 #pragma unroll
     for (int i = 0; i < TS; ++i) {
-        const uint32_t srow = s_first + i;
-        live[i] = i < ns;
         K[i] = Real(1);
-        if (live[i]) {
+        if (i < ns) {
+            const uint32_t srow = s_first + i;
             K[i] = static_cast<Real>(t.strikes[srow]);
-            const Model &m0 = models[g.model_base + (srow - g.strike_begin) * obs];
-            if (g.origin_row && m0.kind == kRuleOrigin && m0.exercise) live[i] = false;
-            if (errors[srow].key != kNoError) live[i] = false;
+            const Model &m0 = models[g.model_base + (local0 + i) * obs];
+            const bool origin_exercise = g.origin_row && m0.kind == kRuleOrigin && m0.exercise;
+            if (!origin_exercise && errors[srow].key == kNoError) live |= 1u << i;
         }
     }
-    double acc[TS][NACC];
-#pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int a = 0; a < NACC; ++a) acc[i][a] = 0.0;
     const uint32_t n = g.ne;
     const uint32_t last = obs - 1;
-    for (int q = 0; q < kPathsPerThread; ++q) {
-        const uint32_t p = item.y * kChunk + q * kBlock + threadIdx.x;
-        const bool valid = p < n;
+    const uint32_t p = item.y * kEvalChunk + threadIdx.x;
+    const bool valid = p < n;
+    uint32_t active = valid ? live : 0u;
+    Real pay[TS];
+    Real dpay[TS][GREEKS ? 4 : 1];
+#pragma unroll
+    for (int i = 0; i < TS; ++i) {
+        pay[i] = Real(0);
+#pragma unroll
+        for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Real(0);
+    }
+    if (valid) {
         PathStream s;
         s.init(g.eval_seed, g.eval_domain, p);
-        Real spot = static_cast<Real>(g.S0);
-        Real tan[4] = {Real(1), Real(0), Real(0), Real(0)};
-        uint32_t active = 0;
-#pragma unroll
-        for (int i = 0; i < TS; ++i) active |= (live[i] && valid) ? (1u << i) : 0u;
-        Real pay[TS];
-        Real dpay[TS][4];
-        uint32_t stop[TS];
-#pragma unroll
-        for (int i = 0; i < TS; ++i) {
-            pay[i] = Real(0);
-            stop[i] = last;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) dpay[i][a] = Real(0);
-        }
-        uint32_t row = g.origin_row;   // origin row holds no regression rule
+        double spot = g.S0;
+        double tan[4] = {1.0, 0.0, 0.0, 0.0};
+        uint32_t row = g.origin_row;   // the origin row holds no regression rule
         bool ok = true;
-        for (uint32_t iv = 0; iv < g.intervals && valid; ++iv) {
+        for (uint32_t iv = 0; iv < g.intervals; ++iv) {
             double z;
             uint32_t draws = 0;
             if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
@@ -996,25 +923,23 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
                 break;
             }
             const uint32_t step_idx = g.step_base + iv;
-            Real growth;
-            const Real before = spot;
-            const Real next = gbm_advance(spot, StepCoef<Real>::diff(t, step_idx), StepCoef<Real>::drift(t, step_idx),
-                                          static_cast<Real>(z), growth);
+            double growth;
+            const double before = spot;
+            const double next = gbm_advance(spot, t.diffusion[step_idx], t.log_drift[step_idx], z, growth);
             if (!isfinite(next)) {
                 ok = false;
                 break;
             }
             spot = next;
             if (GREEKS) {
-                if (before == Real(0)) {
-                    tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
+                if (before == 0.0) {
+                    tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
                 } else {
-                    const Real dt = static_cast<Real>(t.dt[step_idx]);
-                    const Real sdt = static_cast<Real>(t.sqrt_dt[step_idx]);
+                    const double dt = t.dt[step_idx];
                     tan[0] = tan[0] * growth;
-                    tan[1] = tan[1] * growth + next * (sdt * static_cast<Real>(z) - static_cast<Real>(g.sigma) * dt);
-                    tan[2] = tan[2] * growth + next * dt;
-                    tan[3] = tan[3] * growth - next * dt;
+                    tan[1] = fma(tan[1], growth, next * fma(t.sqrt_dt[step_idx], z, -g.sigma * dt));
+                    tan[2] = fma(tan[2], growth, next * dt);
+                    tan[3] = fma(tan[3], growth, -next * dt);
                 }
             }
             const uint32_t node = g.node_base + iv + 1;
@@ -1022,113 +947,110 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
             const bool cash = flags & kNodeCash;
             const bool terminal = flags & kNodeTerminal;
             if (cash && terminal) {
-                const Real amount = static_cast<Real>(t.cash[node]);
-                if (GREEKS && g.policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
-                spot = apply_cash<Real>(g.policy, amount, spot);
+                if (GREEKS && g.policy == 0 && !(spot > t.cash[node])) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
+                spot = apply_cash<double>(g.policy, t.cash[node], spot);
             }
-            if (flags & kNodeObserved) {
-                if (row < last) {
-                    const double disc = t.disc0[g.row_base + row];
+            if ((flags & kNodeObserved) && row < last) {
+                const Real S = static_cast<Real>(spot);
+                const Real disc = static_cast<Real>(t.disc0[g.row_base + row]);
 #pragma unroll
-                    for (int i = 0; i < TS; ++i) {
-                        if (!(active & (1u << i)) || !(spot > K[i])) continue;
-                        const Model &m = models[g.model_base + (s_first + i - g.strike_begin) * obs + row];
-                        if (m.kind == kRuleRegression) {
-                            const Real intr = spot - K[i];
-                            const Real cont = predict(m, fp.degree, spot / K[i]);
-                            if (!isfinite(cont)) {
-                                record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
-                                active &= ~(1u << i);
-                                continue;
-                            }
-                            if (should_exercise(fp.tie, static_cast<double>(intr), static_cast<double>(cont))) {
-                                pay[i] = intr * static_cast<Real>(disc);
-                                stop[i] = row;
-                                active &= ~(1u << i);
-                                if (GREEKS) {
-                                    const Real dd = static_cast<Real>(disc);
-                                    dpay[i][0] = tan[0] * dd;
-                                    dpay[i][1] = tan[1] * dd;
-                                    dpay[i][2] = tan[2] * dd - static_cast<Real>(t.row_time[g.row_base + row]) * pay[i];
-                                    dpay[i][3] = tan[3] * dd;
-                                }
-                            }
-                        } else if (m.kind == kRuleNoSupport) {
-                            atomicAdd(&nosupport[g.count_base + (s_first + i - g.strike_begin) * obs + row], 1u);
+                for (int i = 0; i < TS; ++i) {
+                    if (!(active & (1u << i)) || !(S > K[i])) continue;
+                    const Model &m = models[g.model_base + (local0 + i) * obs + row];
+                    if (m.kind == kRuleRegression) {
+                        const Real intr = S - K[i];
+                        const Real cont = predict(m, fp.degree, S / K[i]);
+                        if (!isfinite(cont)) {
+                            record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
+                            active &= ~(1u << i);
+                            continue;
                         }
+                        if (should_exercise(fp.tie, static_cast<double>(intr), static_cast<double>(cont))) {
+                            pay[i] = intr * disc;
+                            active &= ~(1u << i);
+                            if (!isfinite(pay[i]))
+                                record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPayoff, p));
+                            if (use_lds_hist)
+                                atomicAdd(&hist[i * obs + row], 1u);
+                            else
+                                atomicAdd(&counts[g.count_base + (local0 + i) * obs + row], 1u);
+                            if (GREEKS) {
+                                const double dd = t.disc0[g.row_base + row];
+                                dpay[i][0] = static_cast<Real>(tan[0] * dd);
+                                dpay[i][1] = static_cast<Real>(tan[1] * dd);
+                                dpay[i][2] = static_cast<Real>(tan[2] * dd - t.row_time[g.row_base + row] *
+                                                                                 static_cast<double>(pay[i]));
+                                dpay[i][3] = static_cast<Real>(tan[3] * dd);
+                            }
+                        }
+                    } else if (m.kind == kRuleNoSupport) {
+                        atomicAdd(&nosupport[g.count_base + (local0 + i) * obs + row], 1u);
                     }
                 }
-                row += 1;
             }
+            if (flags & kNodeObserved) row += 1;
             if (cash && !terminal) {
-                const Real amount = static_cast<Real>(t.cash[node]);
-                if (GREEKS && g.policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = Real(0);
-                spot = apply_cash<Real>(g.policy, amount, spot);
+                if (GREEKS && g.policy == 0 && !(spot > t.cash[node])) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
+                spot = apply_cash<double>(g.policy, t.cash[node], spot);
             }
         }
         if (!ok) {
             atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
             active = 0;
-        }
-        if (valid && ok) {
-            const double disc = t.disc0[g.row_base + last];
+        } else {
+            const Real S = static_cast<Real>(spot);
+            const Real disc = static_cast<Real>(t.disc0[g.row_base + last]);
 #pragma unroll
             for (int i = 0; i < TS; ++i) {
                 if (!(active & (1u << i))) continue;
-                const Real intr = intrinsic(spot, K[i]);
-                pay[i] = intr * static_cast<Real>(disc);
-                stop[i] = last;
-                if (GREEKS && spot > K[i]) {
-                    const Real dd = static_cast<Real>(disc);
-                    dpay[i][0] = tan[0] * dd;
-                    dpay[i][1] = tan[1] * dd;
-                    dpay[i][2] = tan[2] * dd - static_cast<Real>(t.row_time[g.row_base + last]) * pay[i];
-                    dpay[i][3] = tan[3] * dd;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < TS; ++i) {
-                if (!live[i]) continue;
-                const double v = static_cast<double>(pay[i]);
-                if (!isfinite(v))
-                    record_error(&errors[s_first + i], error_key(kEvalStepBase + stop[i], kStageEvalPayoff, p));
-                acc[i][0] += v;
-                acc[i][1] += v * v;
-                if (GREEKS) {
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        const double d = static_cast<double>(dpay[i][a]);
-                        acc[i][2 + a] += d;
-                        acc[i][6 + a] += d * d;
-                    }
-                }
+                pay[i] = intrinsic(S, K[i]) * disc;
+                if (!isfinite(pay[i]))
+                    record_error(&errors[s_first + i], error_key(kEvalStepBase + last, kStageEvalPayoff, p));
                 if (use_lds_hist)
-                    atomicAdd(&hist[i * obs + stop[i]], 1u);
+                    atomicAdd(&hist[i * obs + last], 1u);
                 else
-                    atomicAdd(&counts[g.count_base + (s_first + i - g.strike_begin) * obs + stop[i]], 1u);
+                    atomicAdd(&counts[g.count_base + (local0 + i) * obs + last], 1u);
+                if (GREEKS && S > K[i]) {
+                    const double dd = t.disc0[g.row_base + last];
+                    dpay[i][0] = static_cast<Real>(tan[0] * dd);
+                    dpay[i][1] = static_cast<Real>(tan[1] * dd);
+                    dpay[i][2] = static_cast<Real>(tan[2] * dd - t.row_time[g.row_base + last] *
+                                                                     static_cast<double>(pay[i]));
+                    dpay[i][3] = static_cast<Real>(tan[3] * dd);
+                }
             }
         }
     }
-    double flat[TS * NACC];
+    // per-strike block reduction of (sum v, sum v^2[, sum dv_a, sum dv_a^2])
 #pragma unroll
-    for (int i = 0; i < TS; ++i)
+    for (int i = 0; i < TS; ++i) {
+        if (i >= ns) break;
+        const bool contributes = valid && (live & (1u << i));
+        double vals[NACC];
+        const double v = contributes ? static_cast<double>(pay[i]) : 0.0;
+        vals[0] = v;
+        vals[1] = v * v;
+        if (GREEKS) {
 #pragma unroll
-        for (int a = 0; a < NACC; ++a) flat[i * NACC + a] = acc[i][a];
-    block_sum<TS * NACC>(flat, scratch);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < ns; ++i) {
-            double *out = partials + static_cast<uint64_t>(g.part_base_e + (s_first + i - g.strike_begin) * g.chunks_e + item.y) * kPartialStride;
-            for (int a = 0; a < NACC; ++a) out[a] = flat[i * NACC + a];
+            for (int a = 0; a < 4; ++a) {
+                const double d = contributes ? static_cast<double>(dpay[i][a]) : 0.0;
+                vals[2 + a] = d;
+                vals[6 + a] = d * d;
+            }
+        }
+        block_sum<NACC>(vals, scratch);
+        if (threadIdx.x == 0) {
+            double *out = partials + static_cast<uint64_t>(g.part_base_e + (local0 + i) * g.chunks_e + item.y) *
+                                         kPartialStride;
+#pragma unroll
+            for (int a = 0; a < NACC; ++a) out[a] = vals[a];
         }
     }
     if (use_lds_hist) {
         __syncthreads();
         for (uint32_t idx = threadIdx.x; idx < static_cast<uint32_t>(ns) * obs; idx += kBlock) {
             const uint32_t c = hist[idx];
-            if (c) {
-                const uint32_t i = idx / obs, rr = idx % obs;
-                atomicAdd(&counts[g.count_base + (s_first + i - g.strike_begin) * obs + rr], c);
-            }
+            if (c) atomicAdd(&counts[g.count_base + (local0 + idx / obs) * obs + idx % obs], c);
         }
     }
 }
@@ -1147,11 +1069,12 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
     double gs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t c = threadIdx.x; c < g.chunks_e; c += kBlock) {
         const double *in = partials + static_cast<uint64_t>(g.part_base_e + local * g.chunks_e + c) * kPartialStride;
-        const double nc = static_cast<double>(min(static_cast<uint32_t>(kChunk), g.ne - c * kChunk));
+        const double nc = static_cast<double>(min(static_cast<uint32_t>(kEvalChunk), g.ne - c * kEvalChunk));
         const double mean = in[0] / nc;
         const double m2 = fmax(in[1] - in[0] * mean, 0.0);
         acc = chan_merge(acc, Moments{nc, mean, m2});
         if (greeks)
+#pragma unroll
             for (int a = 0; a < 8; ++a) gs[a] += in[2 + a];
     }
     acc = block_chan(acc, mscratch);
@@ -1171,14 +1094,13 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
     }
 }
 
-
 // Per strike row: the 12-column summary of try_from_parts (batch/pricing.rs:711-812) computed
 // from the fitted rules and stopping counts, plus the first evaluation row whose declared
 // missing-support rule met an active in-the-money path (policy.rs:684-703).
 __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model *__restrict__ models,
                                                         const uint32_t *__restrict__ counts,
-                                                        const uint32_t *__restrict__ nosupport,
-                                                        uint32_t rows, RowSummary *__restrict__ out) {
+                                                        const uint32_t *__restrict__ nosupport, uint32_t rows,
+                                                        RowSummary *__restrict__ out) {
     const uint32_t srow = blockIdx.x * blockDim.x + threadIdx.x;
     if (srow >= rows) return;
     const DevGroup &g = t.groups[t.strike_group[srow]];
@@ -1268,11 +1190,13 @@ __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint3
 // ------------------------------------------------------------------------------ launchers
 
 namespace {
-inline dim3 grid2(uint32_t x, uint32_t y) { return dim3(x, y, 1); }
+inline uint32_t xcd_blocks(uint32_t nitems, uint32_t columns) {
+    return ((nitems + 7u) / 8u) * 8u * columns;
 }
+}  // namespace
 
-hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
-                                uint32_t paths, uint32_t blocks, uint32_t *out) {
+hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                uint32_t blocks, uint32_t *out) {
     const uint64_t total = static_cast<uint64_t>(paths) * blocks;
     if (total == 0) return hipSuccess;
     const uint32_t nb = static_cast<uint32_t>((total + kBlock - 1) / kBlock);
@@ -1280,19 +1204,18 @@ hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, 
     return hipGetLastError();
 }
 
-hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
-                                 uint32_t paths, uint32_t count, double *out, uint32_t *draws,
-                                 int *fail) {
+hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
+                                 uint32_t count, double *out, uint32_t *draws, int *fail) {
     if (paths == 0) return hipSuccess;
     hipLaunchKernelGGL(k_stream_normals, dim3((paths + kBlock - 1) / kBlock), dim3(kBlock), 0, st, seed, domain,
                        path0, paths, count, out, draws, fail);
     return hipGetLastError();
 }
 
-hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const uint2 *items,
-                           uint32_t nitems, void *paths, int evaluation, unsigned long long *fail) {
+hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const uint2 *items, uint32_t nitems,
+                           void *paths, int evaluation, unsigned long long *fail) {
     if (nitems == 0) return hipSuccess;
-    const dim3 grid = grid2(nitems, kChunk / kBlock);
+    const dim3 grid(nitems, kChunk / kBlock);
     if (f32)
         hipLaunchKernelGGL(k_simulate<float>, grid, dim3(kBlock), 0, st, t, items, static_cast<float *>(paths),
                            evaluation, fail);
@@ -1302,79 +1225,40 @@ hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const u
     return hipGetLastError();
 }
 
-hipError_t launch_fit_prepare(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                              uint32_t step, const uint2 *items, uint32_t nitems,
-                              uint32_t max_strikes, const void *paths, void *values,
-                              const Model *models, double *partials, RowError *errors) {
+hipError_t launch_fit_prepare(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                              const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                              void *values, const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
-    const dim3 grid = grid2(nitems, max_strikes);
-    if (f32)
-        hipLaunchKernelGGL(k_fit_prepare<float>, grid, dim3(kBlock), 0, st, t, fp, step, items,
-                           static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
-    else
-        hipLaunchKernelGGL(k_fit_prepare<double>, grid, dim3(kBlock), 0, st, t, fp, step, items,
-                           static_cast<const double *>(paths), static_cast<double *>(values), models, partials,
-                           errors);
+    hipLaunchKernelGGL(k_fit_prepare, dim3(xcd_blocks(nitems, max_strikes)), dim3(kBlock), 0, st, t, fp, step, items,
+                       nitems, max_strikes, static_cast<const double *>(paths), static_cast<double *>(values), models,
+                       partials, errors);
     return hipGetLastError();
 }
 
-hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                     uint32_t step, const uint32_t *rows, uint32_t nrows,
-                                     const double *partials, Model *models, RowError *errors) {
+hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                                     const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
+                                     RowError *errors) {
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_reduce_welford, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
                        errors);
     return hipGetLastError();
 }
 
-hipError_t launch_fit_moments(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                              uint32_t step, int k, const uint2 *items, uint32_t nitems,
-                              uint32_t max_strikes, const void *paths, const void *values,
-                              const Model *models, double *partials, const RowError *errors) {
+hipError_t launch_fit_moments(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step, int k,
+                              const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                              const void *values, const Model *models, double *partials, const RowError *errors) {
     if (nitems == 0) return hipSuccess;
-    const dim3 grid = grid2(nitems, max_strikes);
-    if (f32)
-        hipLaunchKernelGGL(k_fit_moments<float>, grid, dim3(kBlock), 0, st, t, fp, step, k, items,
-                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors);
-    else
-        hipLaunchKernelGGL(k_fit_moments<double>, grid, dim3(kBlock), 0, st, t, fp, step, k, items,
-                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
-                           partials, errors);
+    hipLaunchKernelGGL(k_fit_moments, dim3(xcd_blocks(nitems, max_strikes)), dim3(kBlock), 0, st, t, fp, step, k,
+                       items, nitems, max_strikes, static_cast<const double *>(paths),
+                       static_cast<const double *>(values), models, partials, errors);
     return hipGetLastError();
 }
 
-hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                     uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
-                                     const double *partials, Model *models, RowError *errors) {
+hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step, int k,
+                                     const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
+                                     RowError *errors) {
     if (nrows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fit_reduce_moments, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, k, rows, partials,
-                       models, errors);
-    return hipGetLastError();
-}
-
-hipError_t launch_fit_gram(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                           uint32_t step, const uint2 *items, uint32_t nitems,
-                           uint32_t max_strikes, const void *paths, const void *values,
-                           const Model *models, double *partials, const RowError *errors) {
-    if (nitems == 0) return hipSuccess;
-    const dim3 grid = grid2(nitems, max_strikes);
-    if (f32)
-        hipLaunchKernelGGL(k_fit_gram<float>, grid, dim3(kBlock), 0, st, t, fp, step, items,
-                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors);
-    else
-        hipLaunchKernelGGL(k_fit_gram<double>, grid, dim3(kBlock), 0, st, t, fp, step, items,
-                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
-                           partials, errors);
-    return hipGetLastError();
-}
-
-hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                  uint32_t step, const uint32_t *rows, uint32_t nrows,
-                                  const double *partials, Model *models, RowError *errors) {
-    if (nrows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fit_reduce_gram, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
+    hipLaunchKernelGGL(k_fit_reduce_moments, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, k, rows, partials, models,
                        errors);
     return hipGetLastError();
 }
@@ -1383,13 +1267,14 @@ hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitPar
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
                                 void *values, const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fit_bulk_step<float>, grid2(nitems, max_strikes), dim3(kBlock), 0, st, t, fp, step, items,
-                       static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
+    hipLaunchKernelGGL(k_fit_bulk_step, dim3(xcd_blocks(nitems, max_strikes)), dim3(kBlock), 0, st, t, fp, step,
+                       items, nitems, max_strikes, static_cast<const float *>(paths), static_cast<float *>(values),
+                       models, partials, errors);
     return hipGetLastError();
 }
 
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
-                                  const uint32_t *rows, uint32_t nrows, const double *partials, Model *models,
+                                  const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors) {
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_bulk_reduce, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
@@ -1397,26 +1282,24 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
     return hipGetLastError();
 }
 
-hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                            const uint2 *items, uint32_t nitems, uint32_t max_strikes,
-                            const void *paths, const void *values, const Model *models,
-                            double *partials, RowError *errors) {
+hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, const uint2 *items,
+                            uint32_t nitems, uint32_t max_strikes, const void *paths, const void *values,
+                            const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
-    const dim3 grid = grid2(nitems, max_strikes);
+    const dim3 grid(xcd_blocks(nitems, max_strikes));
     if (f32)
-        hipLaunchKernelGGL(k_fit_final<float>, grid, dim3(kBlock), 0, st, t, fp, items,
+        hipLaunchKernelGGL(k_fit_final<float>, grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
                            errors, 1);
     else
-        hipLaunchKernelGGL(k_fit_final<double>, grid, dim3(kBlock), 0, st, t, fp, items,
-                           static_cast<const double *>(paths), static_cast<const double *>(values), models,
-                           partials, errors, 0);
+        hipLaunchKernelGGL(k_fit_final<double>, grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
+                           static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
+                           errors, 0);
     return hipGetLastError();
 }
 
-hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                    const uint32_t *rows, uint32_t nrows, const double *partials,
-                                    Model *models, RowError *errors) {
+hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows,
+                                    uint32_t nrows, double *partials, Model *models, RowError *errors) {
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_reduce_origin, dim3(nrows), dim3(kBlock), 0, st, t, fp, rows, partials, models, errors);
     return hipGetLastError();
@@ -1426,7 +1309,7 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     const uint32_t options_plain[] = {1, 4, 16};
     const uint32_t options_greeks[] = {1, 4, 8};
     const uint32_t *opts = greeks ? options_greeks : options_plain;
-    uint32_t want = requested ? requested : (max_strikes <= 1 ? 1 : (greeks ? 8 : 16));
+    const uint32_t want = requested ? requested : (max_strikes <= 1 ? 1 : (greeks ? 8 : 16));
     uint32_t tile = opts[0];
     for (int i = 0; i < 3; ++i)
         if (opts[i] <= want) tile = opts[i];
@@ -1441,20 +1324,21 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    hipLaunchKernelGGL((k_evaluate<Real, TS, G>), dim3(nitems, tiles), dim3(kBlock), use_lds ? hist_bytes : 0, st,
-                       t, fp, items, models, partials, counts, nosupport, errors, sim_fail, use_lds);
+    hipLaunchKernelGGL((k_evaluate<Real, TS, G>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+                       use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
+                       errors, sim_fail, use_lds);
     return hipGetLastError();
 }
 
 template <typename Real>
-static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, const DevTables &t,
-                                const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
-                                uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
-                                uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
-#define FOHIP_EVAL_CASE(TSV, GV)                                                                              \
-    if (tile == TSV && greeks == GV)                                                                          \
-        return launch_eval_t<Real, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, \
-                                            counts, nosupport, errors, sim_fail);
+static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, const DevTables &t, const FitParams &fp,
+                                const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
+                                const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
+                                RowError *errors, unsigned long long *sim_fail) {
+#define FOHIP_EVAL_CASE(TSV, GV)                                                                                   \
+    if (tile == TSV && greeks == GV)                                                                               \
+        return launch_eval_t<Real, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, \
+                                            nosupport, errors, sim_fail);
     FOHIP_EVAL_CASE(1, false)
     FOHIP_EVAL_CASE(4, false)
     FOHIP_EVAL_CASE(16, false)
@@ -1465,10 +1349,9 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, cons
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
-                           const DevTables &t, const FitParams &fp, const uint2 *items,
-                           uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
-                           const Model *models, double *partials, uint32_t *counts,
+hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile, const DevTables &t,
+                           const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
+                           uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
     if (nitems == 0) return hipSuccess;
     if (f32)
@@ -1478,23 +1361,22 @@ hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
                                  counts, nosupport, errors, sim_fail);
 }
 
-hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models,
-                              const uint32_t *counts, const uint32_t *nosupport, uint32_t rows,
-                              RowSummary *out) {
+hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models, const uint32_t *counts,
+                              const uint32_t *nosupport, uint32_t rows, RowSummary *out) {
     if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_row_summary, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, models, counts,
                        nosupport, rows, out);
     return hipGetLastError();
 }
 
-hipError_t launch_replay_path(hipStream_t st, const DevTables &t, uint32_t group, int evaluation,
-                              uint32_t path, double *out) {
+hipError_t launch_replay_path(hipStream_t st, const DevTables &t, uint32_t group, int evaluation, uint32_t path,
+                              double *out) {
     hipLaunchKernelGGL(k_replay_path, dim3(1), dim3(64), 0, st, t, group, evaluation, path, out);
     return hipGetLastError();
 }
 
-hipError_t launch_eval_reduce(hipStream_t st, const DevTables &t, const uint32_t *rows,
-                              uint32_t nrows, const double *partials, int greeks, double *out) {
+hipError_t launch_eval_reduce(hipStream_t st, const DevTables &t, const uint32_t *rows, uint32_t nrows,
+                              const double *partials, int greeks, double *out) {
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_eval_reduce, dim3(nrows), dim3(kBlock), 0, st, t, rows, partials, greeks, out);
     return hipGetLastError();

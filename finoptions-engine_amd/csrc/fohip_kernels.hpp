@@ -25,40 +25,32 @@ hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain,
                                  int *fail);
 hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const uint2 *items,
                            uint32_t nitems, void *paths, int evaluation, unsigned long long *fail);
-hipError_t launch_fit_prepare(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                              uint32_t step, const uint2 *items, uint32_t nitems,
-                              uint32_t max_strikes, const void *paths, void *values,
-                              const Model *models, double *partials, RowError *errors);
+// FP64 method: prepare (apply + discount + Welford), then per degree k the moments of p_k
+hipError_t launch_fit_prepare(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+                              const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                              void *values, const Model *models, double *partials, RowError *errors);
 hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp,
                                      uint32_t step, const uint32_t *rows, uint32_t nrows,
-                                     const double *partials, Model *models, RowError *errors);
-hipError_t launch_fit_moments(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                              uint32_t step, int k, const uint2 *items, uint32_t nitems,
-                              uint32_t max_strikes, const void *paths, const void *values,
-                              const Model *models, double *partials, const RowError *errors);
+                                     double *partials, Model *models, RowError *errors);
+hipError_t launch_fit_moments(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step, int k,
+                              const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                              const void *values, const Model *models, double *partials, const RowError *errors);
 hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
                                      uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
-                                     const double *partials, Model *models, RowError *errors);
-hipError_t launch_fit_gram(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
-                           uint32_t step, const uint2 *items, uint32_t nitems,
-                           uint32_t max_strikes, const void *paths, const void *values,
-                           const Model *models, double *partials, const RowError *errors);
-hipError_t launch_fit_reduce_gram(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                  uint32_t step, const uint32_t *rows, uint32_t nrows,
-                                  const double *partials, Model *models, RowError *errors);
+                                     double *partials, Model *models, RowError *errors);
 // FP32 bulk: one fused HBM pass per backward step (apply, Welford + Gram via pivoted power sums)
 hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
                                 void *values, const Model *models, double *partials, RowError *errors);
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
-                                  const uint32_t *rows, uint32_t nrows, const double *partials, Model *models,
+                                  const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
 hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const Model *models,
                             double *partials, RowError *errors);
 hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
-                                    const uint32_t *rows, uint32_t nrows, const double *partials,
+                                    const uint32_t *rows, uint32_t nrows, double *partials,
                                     Model *models, RowError *errors);
 hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
                            const DevTables &t, const FitParams &fp, const uint2 *items,

@@ -14,6 +14,7 @@ namespace fohip {
 
 constexpr int kChunk = 1024;            // paths per reduction chunk (fixes every reduction tree)
 constexpr int kPathsPerThread = 4;      // kChunk / threads per block
+constexpr int kEvalChunk = 256;         // evaluation paths per chunk: one path per thread
 constexpr int kPartialStride = 16;      // doubles per chunk partial
 constexpr int kMaxDegree = 4;
 

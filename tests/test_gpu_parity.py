@@ -122,7 +122,7 @@ def test_c1_plain_fp64_matches_the_oracle(hip_f64, oracle):
 
 
 def _grouped_pair(hip, oracle, cols, *, base_intervals, paths, seed, continue_support=True,
-                  **kw):
+                  eval_domain=2, **kw):
     """Price one request on the GPU and on the oracle. Deep out-of-the-money strikes at the first
     nodes have fewer in-the-money training paths than basis columns, so these requests declare
     the CONTINUE support policy (the reference requires the policy to be explicit)."""
@@ -133,12 +133,12 @@ def _grouped_pair(hip, oracle, cols, *, base_intervals, paths, seed, continue_su
                                   origin_exercise=fh.OriginExercise.EXCLUDED)
     support = (fh.InsufficientExerciseSupportPolicy.CONTINUE if continue_support
                else fh.InsufficientExerciseSupportPolicy.FAIL)
-    method = hip_method(paths, paths, seed, support=support)
+    method = hip_method(paths, paths, seed, support=support, eval_domain=eval_domain)
     res = fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip, method=method,
                                           **kw)
     ref = oracle.price_grouped(oracle_grouped(cols, base_intervals=base_intervals,
                                               training_paths=paths, evaluation_paths=paths,
-                                              seed=seed,
+                                              seed=seed, eval_domain=eval_domain,
                                               insufficient_support=1 if continue_support else 0))
     return res, ref, groups, mesh, method
 
@@ -356,15 +356,16 @@ def test_large_grouped_frame_slice_matches_the_oracle(hip_f64, hip_f32, oracle):
     w = large_grouped_frame(groups=6, strikes_per_group=24)
     cols = {k: w[k] for k in ("group_offsets", "initial_spots", "strikes", "maturities", "rates",
                               "volatilities", "cash_offsets", "cash_times", "cash_amounts")}
+    # groups take consecutive stream domains: training 1..6, evaluation 7..12
     res_pair = _grouped_pair(hip_f64, oracle, cols, base_intervals=w["intervals"], paths=4096,
-                             seed=5)
+                             seed=5, eval_domain=7)
     res, ref = res_pair[0], res_pair[1]
     err = rel_err(res.prices, ref["prices"])
     print(f"grouped frame slice (6 x 24): FP64 max price rel err {err:.2e}")
     assert err < PRICE_TOL
     assert np.array_equal(res.expiry_paths, ref["expiry_paths"])
     bulk = _grouped_pair(hip_f32, oracle, cols, base_intervals=w["intervals"], paths=4096,
-                         seed=5)[0]
+                         seed=5, eval_domain=7)[0]
     bias = np.abs(bulk.prices - ref["prices"]) / ref["standard_errors"]
     print(f"grouped frame slice FP32 bulk: max |bias|/SE {bias.max():.3f}")
     assert bias.max() < 0.1
