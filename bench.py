@@ -3,8 +3,9 @@
 
 Workload (one step = one complete pricing request): one simulated state, 64 strikes
 linspace(80, 120), S0 100, r 0.03, q 0.02, sigma 0.25, T 1, 50 equal intervals (49 exercise
-nodes + expiry), 2^20 training + 2^20 evaluation paths, streams (11, 1) / (11, 2); FP32 bulk paths,
-payoff and basis with FP64 Gram/solve and price/SE reductions, pathwise delta + vega.
+nodes + expiry), 2^20 training + 2^20 evaluation paths, streams (11, 1) / (11, 2); the fused
+method (one HBM pass per exercise node, binary64 storage and decisions), pathwise delta + vega.
+The binary32-storage variant (FP32 bulk) is timed beside it with its accuracy.
 Path-steps = intervals x (training + evaluation paths), the reference's transition unit.
 
 Also reported: the plain FP64 method's price time and its price + bump-and-reprice delta/vega
@@ -128,23 +129,27 @@ def main() -> int:
     import finoptions_hip as fh
 
     paths = args.paths
-    fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
+    fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
+    bulk32 = fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
     plain = fh.HipExecutor(device=0, precision=fh.HipPrecision.F64)
     greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
     log(f"device: {fast.device_name}; paths {paths}+{paths}, {STRIKES} strikes, "
         f"{INTERVALS} intervals")
 
-    # ---- fast path: FP32 bulk + pathwise delta/vega
-    for i in range(args.warmup):
-        price(fh, fast, paths, greeks)
-        log(f"warmup {i + 1}/{args.warmup} done")
-    times = []
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        fast_res = price(fh, fast, paths, greeks)   # synchronous: returns after the device drains
-        times.append(time.perf_counter() - t0)
-        log(f"fast step {i + 1}/{args.steps}: {times[-1] * 1e3:.2f} ms")
-    ms_fast = 1e3 * float(np.mean(times))
+    def timed(executor, label, steps, warmup):
+        for i in range(warmup):
+            price(fh, executor, paths, greeks)
+            log(f"{label} warmup {i + 1}/{warmup} done")
+        times, res = [], None
+        for i in range(steps):
+            t0 = time.perf_counter()
+            res = price(fh, executor, paths, greeks)   # synchronous: returns after the device drains
+            times.append(time.perf_counter() - t0)
+            log(f"{label} step {i + 1}/{steps}: {times[-1] * 1e3:.2f} ms")
+        return 1e3 * float(np.mean(times)), res
+
+    # ---- fast path: fused FP64 method + pathwise delta/vega
+    ms_fast, fast_res = timed(fast, "fast", args.steps, args.warmup)
     path_steps = INTERVALS * 2 * paths
     value = path_steps / (ms_fast / 1e3)
 
@@ -159,10 +164,10 @@ def main() -> int:
     kernel_table = {k: {"launches": n, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / n, 2)}
                     for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1])}
     log("kernel times (one fast step): " + json.dumps(kernel_table))
-    hbm_kernels = [(k, v) for k, v in stats.items() if algorithmic_bytes(k, paths, 4)]
+    hbm_kernels = [(k, v) for k, v in stats.items() if algorithmic_bytes(k, paths, 8)]
     hbm_name, (hbm_n, hbm_ms) = max(hbm_kernels, key=lambda kv: kv[1][1])
     hbm_avg_ms = hbm_ms / hbm_n
-    alg = algorithmic_bytes(hbm_name, paths, 4)
+    alg = algorithmic_bytes(hbm_name, paths, 8)
     achieved = alg / (hbm_avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": hbm_name, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -170,6 +175,9 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": alg,
                 "dominant_kernel": name, "dominant_avg_launch_us": round(avg_ms * 1e3, 2),
                 "dominant_share": round(total_ms / sum(v[1] for v in stats.values()), 3)}
+
+    # ---- FP32-bulk storage variant (binary32 spots and cashflows), for comparison
+    ms_bulk32, bulk32_res = timed(bulk32, "f32-bulk", max(1, min(args.steps, 3)), 1)
 
     # ---- plain FP64: price and price + bump-and-reprice delta/vega (1 + 4 pricings)
     plain_info = None
@@ -186,26 +194,34 @@ def main() -> int:
         dn_v = price(fh, plain, paths, vol=0.25 - h_v)
         t_bumps = time.perf_counter() - t0
         ms_plain = 1e3 * (t_price + t_bumps)
-        fd_delta = (up_s.prices - dn_s.prices) / (2 * h_s)
-        fd_vega = (up_v.prices - dn_v.prices) / (2 * h_v)
+        # the bumped pricings are the baseline's cost; with the policy re-fitted at every bump
+        # their differences are not compared with the frozen-policy pathwise estimator
+        del up_s, dn_s, up_v, dn_v
         log(f"plain FP64: price {t_price * 1e3:.1f} ms, price+4 bumps {ms_plain:.1f} ms")
         ref = price(fh, plain, paths, greeks)   # FP64 pathwise Greeks: accuracy reference
-        bias_price = np.abs(fast_res.prices - base.prices) / base.standard_errors
-        bias_delta = (np.abs(fast_res.greeks[fh.Greek.DELTA] - ref.greeks[fh.Greek.DELTA])
-                      / ref.greek_standard_errors[fh.Greek.DELTA])
-        bias_vega = (np.abs(fast_res.greeks[fh.Greek.VEGA] - ref.greeks[fh.Greek.VEGA])
-                     / ref.greek_standard_errors[fh.Greek.VEGA])
+
+        def biases(res):
+            bp = np.abs(res.prices - base.prices) / base.standard_errors
+            bd = (np.abs(res.greeks[fh.Greek.DELTA] - ref.greeks[fh.Greek.DELTA])
+                  / ref.greek_standard_errors[fh.Greek.DELTA])
+            bv = (np.abs(res.greeks[fh.Greek.VEGA] - ref.greeks[fh.Greek.VEGA])
+                  / ref.greek_standard_errors[fh.Greek.VEGA])
+            return bp, bd, bv
+
+        bias_price, bias_delta, bias_vega = biases(fast_res)
+        b32 = biases(bulk32_res)
         accuracy = {
             "max_price_bias_over_se": float(bias_price.max()),
             "max_delta_bias_over_se": float(bias_delta.max()),
             "max_vega_bias_over_se": float(bias_vega.max()),
             "all_within_0.1_se": bool(max(bias_price.max(), bias_delta.max(),
                                           bias_vega.max()) < 0.1),
-            "reference": "plain FP64 GPU method (matches the CPU checker to 1e-10 relative)",
-            "fd_vs_pathwise_delta_max_abs": float(np.max(np.abs(fd_delta
-                                                                - ref.greeks[fh.Greek.DELTA]))),
-            "fd_vs_pathwise_vega_max_abs": float(np.max(np.abs(fd_vega
-                                                               - ref.greeks[fh.Greek.VEGA]))),
+            "f32_bulk_max_bias_over_se": {"price": float(b32[0].max()),
+                                          "delta": float(b32[1].max()),
+                                          "vega": float(b32[2].max())},
+            "f32_bulk_strikes_over_0.1_se": int(np.sum(np.maximum.reduce(b32) > 0.1)),
+            "reference": ("plain FP64 GPU method: prices match the CPU checker to 1e-10 "
+                          "relative, pathwise Greeks to 1e-9"),
         }
         plain_info = {"ms_price": round(t_price * 1e3, 2),
                       "ms_price_bump_greeks": round(ms_plain, 2),
@@ -229,13 +245,15 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": (f"grouped-strike LSM call: 1 state x {STRIKES} strikes linspace(80,120), "
                          f"S0 100, r 0.03, q 0.02, sigma 0.25, T 1, {INTERVALS} intervals, "
                          f"{paths}+{paths} paths, Philox seed 11 domains 1/2"),
-            "method": "FP32 bulk paths/payoff/basis, FP64 Gram + LDL^T solve + price/SE",
+            "method": ("fused: one HBM pass per exercise node (binary64 spots and cashflows, "
+                       "pivoted power sums, moment-form Stieltjes solve), binary64 exercise "
+                       "decisions, binary32 Greek tangents"),
             "insufficient_support": "continue",
             "greeks": "pathwise delta+vega, frozen policy, paths regenerated from Philox",
             "path_steps_per_step": path_steps,
@@ -244,6 +262,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_fp64": plain_info,
+        "f32_bulk": {"ms_per_step": round(ms_bulk32, 3),
+                     "path_steps_per_s": path_steps / (ms_bulk32 / 1e3)},
         "accuracy": accuracy,
         "kernels": kernel_table,
     }

@@ -97,7 +97,7 @@ __device__ __forceinline__ double standard_unit(uint64_t bits) {
 
 // Rejected draws: wedge test, redraw, or tail (normal.rs:50-91). Returns false at the bounded
 // rejection limit. `draws` counts consumed u64 draws (parity probe only).
-__device__ __noinline__ bool zignor_rejected(PathStream &s, int index, double unit, double value,
+__device__ __forceinline__ bool zignor_rejected(PathStream &s, int index, double unit, double value,
                                              const double *__restrict__ X,
                                              const double *__restrict__ F, double zr, double &out,
                                              uint32_t &draws) {
@@ -172,6 +172,17 @@ __device__ __forceinline__ Real apply_cash(int policy, Real amount, Real spot) {
     return spot > amount ? spot - amount : spot;
 }
 
+// ----------------------------------------------------------------- wave-uniform read-only data
+
+// Tables and fitted rules that every lane of a wave addresses identically are read through the
+// constant address space, so the compiler emits scalar loads (SGPR results, no per-lane VMEM
+// latency chain). Only data that no kernel writes while the reading kernel runs is read this way.
+#define FOHIP_UNIFORM __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const FOHIP_UNIFORM T *uniform(const T *p) {
+    return (const FOHIP_UNIFORM T *)p;
+}
+
 // ------------------------------------------------------------------ regression prediction
 
 // regression.rs:197-224: z = (x - mean) / scale; p_{k+1} = fma(z - alpha_k, p_k, -beta_k p_{k-1});
@@ -200,6 +211,32 @@ __device__ __forceinline__ float predict(const Model &m, int degree, float x) {
     return prediction;
 }
 
+// The same recurrence read through any pointer kind (generic or uniform).
+template <typename MP>
+__device__ __forceinline__ double predict_ref(MP m, int degree, double x) {
+    const double z = degree == 0 ? 0.0 : (x - m->mean) / m->scale;
+    double prediction = m->coef[0], previous = 0.0, current = 1.0;
+    for (int k = 0; k < degree; ++k) {
+        const double next = fma(z - m->alpha[k], current, -m->beta[k] * previous);
+        prediction = fma(m->coef[k + 1], next, prediction);
+        previous = current;
+        current = next;
+    }
+    return prediction;
+}
+
+// Evaluation form of an accepted rule at spot S: z = fma(S, 1/(scale K), -mean/scale), Horner in
+// z. Unused degrees carry zero coefficients, so the fixed four FMAs serve every degree.
+template <typename MP>
+__device__ __forceinline__ double continuation_fast(MP m, double spot) {
+    const double z = fma(spot, m->a, m->b);
+    double c = m->hz[4];
+    c = fma(c, z, m->hz[3]);
+    c = fma(c, z, m->hz[2]);
+    c = fma(c, z, m->hz[1]);
+    return fma(c, z, m->hz[0]);
+}
+
 __device__ __forceinline__ bool should_exercise(int tie, double intrinsic, double continuation) {
     return tie == 0 ? intrinsic > continuation : intrinsic >= continuation;
 }
@@ -209,21 +246,37 @@ __device__ __forceinline__ bool should_exercise(int tie, double intrinsic, doubl
 constexpr int kBlock = 256;      // threads per block
 constexpr int kWaves = kBlock / 64;
 
-// Down-sweep wave tree; the total is valid in lane 0. The combination order depends only on lane
-// indices, so results are reproducible bit for bit.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
-    return v;
+// Wave total through DPP lane exchanges: pairs, quads, half-rows and rows of 16 are combined
+// with commutative adds (both partners hold identical sums), then the four row totals are
+// combined as (r0 + r1) + (r2 + r3). The order is fixed by lane indices, so the result is
+// reproducible; every lane receives the total.
+template <int CTRL>
+__device__ __forceinline__ double dpp_partner(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_partner<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_partner<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_partner<0x141>(v);   // row_half_mirror
+    v += dpp_partner<0x140>(v);   // row_mirror
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 // Block sum of N values per thread; the result is returned in thread 0 (fixed order
-// ((w0 + w1) + (w2 + w3))). `scratch` holds kWaves * N doubles.
+// ((w0 + w1) + (w2 + w3)) over the wave totals). `scratch` holds kWaves * N doubles. Every thread
+// of the block must call it.
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N], double *scratch) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+    for (int i = 0; i < N; ++i) v[i] = wave_total(v[i]);
     if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < N; ++i) scratch[wave * N + i] = v[i];

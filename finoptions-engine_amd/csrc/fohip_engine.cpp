@@ -260,7 +260,7 @@ struct fohip_executor {
     size_t batch_budget = 0;   // bytes for paths + values per batch
     std::mutex mu;
     DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, items_e,
-        rows, eval_out, summary, replay, probe_a, probe_b, probe_c;
+        items_b, rows, eval_out, summary, replay, probe_a, probe_b, probe_c;
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
     std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
@@ -353,9 +353,10 @@ extern "C" int fohip_executor_create(int32_t device, int32_t precision, fohip_ex
     clear_error(err);
     if (!out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "executor output pointer is null");
     *out = nullptr;
-    if (precision != FOHIP_PRECISION_F64 && precision != FOHIP_PRECISION_F32_BULK)
-        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "precision must be FP64 (0) or FP32 bulk (1); observed %d",
-                    precision);
+    if (precision != FOHIP_PRECISION_F64 && precision != FOHIP_PRECISION_F32_BULK &&
+        precision != FOHIP_PRECISION_FUSED_F64)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0,
+                    "precision must be FP64 (0), FP32 bulk (1) or fused FP64 (2); observed %d", precision);
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n <= 0)
@@ -386,7 +387,7 @@ extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     (void)hipSetDevice(ex->device);
     (void)hipStreamSynchronize(ex->stream);
     for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
-                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t, &ex->items_e, &ex->rows,
+                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t, &ex->items_e, &ex->items_b, &ex->rows,
                       &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c})
         b->release();
     for (auto &p : ex->pending) {
@@ -457,8 +458,14 @@ struct GroupPlan {
 struct RunSpec {
     FitParams fp;
     uint32_t greeks = 0;
-    bool f32 = false;
+    bool store32 = false;   // binary32 spots and cashflows
+    bool fused = false;     // one fused fit pass per node and the evaluation form of the rules
 };
+
+static void set_method(RunSpec &spec, int32_t precision) {
+    spec.store32 = precision == FOHIP_PRECISION_F32_BULK;
+    spec.fused = precision != FOHIP_PRECISION_F64;
+}
 
 struct RowOut {
     double price = 0, sd = 0, se = 0;
@@ -529,6 +536,7 @@ static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<d
         count_slots += static_cast<uint64_t>(ns) * g.obs;
         g.chunks_t = static_cast<uint32_t>((p.nt + kChunk - 1) / kChunk);
         g.chunks_e = static_cast<uint32_t>((p.ne + kEvalChunk - 1) / kEvalChunk);
+        g.chunks_b = static_cast<uint32_t>((p.nt + kBulkChunk - 1) / kBulkChunk);
         g.part_base_t = static_cast<uint32_t>(part_t_slots);
         part_t_slots += static_cast<uint64_t>(ns) * g.chunks_t;
         g.part_base_e = static_cast<uint32_t>(part_e_slots);
@@ -636,7 +644,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                       std::vector<uint32_t> *counts_out, fohip_error *err) {
     HIP_TRY(hipSetDevice(ex->device));
     hipStream_t st = ex->stream;
-    const bool f32 = spec.f32;
+    const bool f32 = spec.store32;
     const size_t real = f32 ? sizeof(float) : sizeof(double);
     out.assign(strikes.size(), RowOut());
     // memory-bounded batches of consecutive groups
@@ -688,24 +696,27 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         for (uint32_t g = 0; g < ngroups; ++g) order[g] = g;
         std::stable_sort(order.begin(), order.end(),
                          [&](uint32_t a, uint32_t b) { return gt.groups[a].fit_rows > gt.groups[b].fit_rows; });
-        std::vector<uint2> items_t, items_e;
+        std::vector<uint2> items_t, items_e, items_b;
         std::vector<uint32_t> rows_sorted;
-        std::vector<uint32_t> step_items(max_fit + 1, 0), step_rows(max_fit + 1, 0);
+        std::vector<uint32_t> step_items(max_fit + 1, 0), step_rows(max_fit + 1, 0), step_items_b(max_fit + 1, 0);
         for (uint32_t g : order) {
             for (uint32_t c = 0; c < gt.groups[g].chunks_t; ++c) items_t.push_back(make_uint2(g, c));
+            for (uint32_t c = 0; c < gt.groups[g].chunks_b; ++c) items_b.push_back(make_uint2(g, c));
             for (uint32_t s = plans[g].strike_begin; s < plans[g].strike_end; ++s) rows_sorted.push_back(s);
         }
         for (uint32_t g = 0; g < ngroups; ++g)
             for (uint32_t c = 0; c < gt.groups[g].chunks_e; ++c) items_e.push_back(make_uint2(g, c));
         for (uint32_t j = 0; j <= max_fit; ++j) {
-            uint32_t ni = 0, nr = 0;
+            uint32_t ni = 0, nr = 0, nb = 0;
             for (uint32_t g : order) {
                 if (gt.groups[g].fit_rows > j || j == max_fit) {
                     ni += gt.groups[g].chunks_t;
+                    nb += gt.groups[g].chunks_b;
                     nr += plans[g].strike_end - plans[g].strike_begin;
                 }
             }
             step_items[j] = ni;
+            step_items_b[j] = nb;
             step_rows[j] = nr;
         }
         // ---- device buffers
@@ -733,6 +744,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
         HIP_TRY(ex->items_t.ensure(std::max<size_t>(items_t.size(), 1) * sizeof(uint2)));
         HIP_TRY(ex->items_e.ensure(std::max<size_t>(items_e.size(), 1) * sizeof(uint2)));
+        HIP_TRY(ex->items_b.ensure(std::max<size_t>(items_b.size(), 1) * sizeof(uint2)));
         HIP_TRY(ex->rows.ensure(std::max<size_t>(rows_sorted.size(), 1) * sizeof(uint32_t)));
         HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
         HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
@@ -761,6 +773,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         t.strike_group = reinterpret_cast<const uint32_t *>(tb + offs[14]);
         HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items_t.data(), items_t.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ex->items_e.ptr, items_e.data(), items_e.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ex->items_b.ptr, items_b.data(), items_b.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ex->rows.ptr, rows_sorted.data(), rows_sorted.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(ex->models.ptr, 0, model_count * sizeof(Model), st));
         HIP_TRY(hipMemsetAsync(ex->counts.ptr, 0, count_slots * sizeof(uint32_t), st));
@@ -779,8 +792,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // ---- backward induction
         for (uint32_t j = 0; j < max_fit; ++j) {
             const uint32_t ni = step_items[j], nr = step_rows[j];
-            if (f32) {   // FP32 bulk: one fused HBM pass per backward step
-                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, t, fp, j, it, ni, max_strikes, ex->paths.ptr,
+            if (spec.fused) {   // one fused HBM pass per backward step
+                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, ex->items_b.as<uint2>(), step_items_b[j], max_strikes, ex->paths.ptr,
                                                                     ex->values.ptr, models, ex->part_t.as<double>(),
                                                                     errors));
                 PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
@@ -796,13 +809,13 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                 PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
             }
         }
-        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
+        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
                                  ex->values.ptr, models, ex->part_t.as<double>(), errors));
         PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));
         // ---- evaluation
         const bool greeks = spec.greeks != 0;
         const uint32_t tile = eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
-        PROF_LAUNCH("k_evaluate", launch_evaluate(st, f32, greeks, tile, t, fp, ex->items_e.as<uint2>(),
+        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, tile, t, fp, ex->items_e.as<uint2>(),
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
                                 ex->fail_e.as<unsigned long long>()));
@@ -1078,7 +1091,7 @@ extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request
     RunSpec spec;
     spec.fp = fp;
     spec.greeks = rq->greeks;
-    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    set_method(spec, ex->precision);
     std::vector<RowOut> out;
     std::vector<Model> models;
     std::vector<uint32_t> counts;
@@ -1437,7 +1450,7 @@ extern "C" int fohip_price_lsm_grouped(fohip_executor *ex, const fohip_grouped_r
     RunSpec spec;
     spec.fp = fp;
     spec.greeks = rq->greeks;
-    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    set_method(spec, ex->precision);
     std::vector<RowOut> out;
     rc = run_groups(ex, plans, strikes, spec, out, nullptr, nullptr, err);
     if (rc) return rc;
@@ -1511,7 +1524,7 @@ extern "C" int fohip_grouped_diagnostics(fohip_executor *ex, const fohip_grouped
     RunSpec spec;
     spec.fp = fp;
     spec.greeks = 0;
-    spec.f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
+    set_method(spec, ex->precision);
     std::vector<RowOut> o;
     std::vector<Model> models;
     std::vector<GroupPlan> one{p};

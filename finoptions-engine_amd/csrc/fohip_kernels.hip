@@ -12,6 +12,8 @@
 // order in a one-block-per-strike reduce kernel): results are independent of timing and tuning.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fohip_device.hpp"
 #include "fohip_kernels.hpp"
 
@@ -44,6 +46,12 @@ __device__ __forceinline__ bool xcd_item_column(uint32_t nitems, uint32_t column
     column = k % columns;
     item = (k / columns) * 8u + (b & 7u);
     return item < nitems;
+}
+
+// Work item wi as wave-uniform scalars.
+__device__ __forceinline__ uint2 load_item(const uint2 *items, uint32_t wi) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(items + wi);
+    return make_uint2(*uniform(w), *uniform(w + 1));
 }
 
 // ------------------------------------------------------------------------ random-stream probes
@@ -229,8 +237,10 @@ __device__ void fail_row(RowError *e, uint32_t step, int32_t code, int64_t index
 
 // Support decision of one (strike, row) from the in-the-money count (policy.rs:480-529). Returns
 // true when a regression is to be fitted.
-__device__ bool decide_support(Model &m, const FitParams &fp, RowError *e, uint32_t step, uint32_t count) {
+__device__ bool decide_support(Model &m, const FitParams &fp, RowError *e, uint32_t step, uint32_t count,
+                               double strike) {
     const uint32_t required = static_cast<uint32_t>(fp.degree) + 1;
+    m.K = strike;
     m.itm = count;
     m.rank = 0;
     m.condition = 0.0;
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_welford(DevTables t, FitP
     acc = block_chan(acc, scratch);
     if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs + r];
-    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(acc.n))) return;
+    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(acc.n), t.strikes[srow])) return;
     double mean = 0.0, scale = 1.0;
     if (fp.degree != 0) {
         scale = sqrt(acc.m2 / acc.n);
@@ -443,6 +453,34 @@ __device__ int jacobi_singular_values(double (&mat)[25], double (&sv)[5]) {
     return 0;
 }
 
+// Expands sum_k coef_k p_k(z) into monomials of z (p_0 = 1, p_1 = z - alpha_0,
+// p_{k+1} = (z - alpha_k) p_k - beta_k p_{k-1}) and stores z's affine map.
+__device__ void set_evaluation_form(Model &m, int degree) {
+    double previous[5] = {0, 0, 0, 0, 0}, current[5] = {1, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) m.hz[j] = 0.0;
+    m.hz[0] = m.coef[0];
+#pragma unroll
+    for (int k = 0; k < kMaxDegree; ++k) {
+        if (k >= degree) break;
+        double next[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) next[j + 1] += current[j];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) next[j] = fma(-m.alpha[k], current[j], next[j]) - m.beta[k] * previous[j];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            m.hz[j] = fma(m.coef[k + 1], next[j], m.hz[j]);
+            previous[j] = current[j];
+            current[j] = next[j];
+        }
+    }
+    m.degree = degree;
+    const double inv_scale = degree == 0 ? 0.0 : 1.0 / m.scale;
+    m.a = inv_scale / m.K;
+    m.b = degree == 0 ? 0.0 : -m.mean * inv_scale;
+}
+
 // Rank gate, condition number and acceptance of a completed fit.
 __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, uint32_t step) {
     double tr[25], sv[5];
@@ -497,6 +535,7 @@ __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, 
     }
     m.rank = rank;
     m.condition = condition;
+    set_evaluation_form(m, fp.degree);
     m.kind = kRuleRegression;
 }
 
@@ -558,94 +597,141 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
     if (threadIdx.x == 0) stieltjes_step(m, fp, &errors[srow], step, k, sums[0], sums[1], sums[2]);
 }
 
-// FP32-bulk backward step j over exercise row r = obs - 2 - j in ONE pass over HBM.
-// Cashflows are stored discounted to the origin, v_o = cashflow * exp(logD(stop)), so a path's
-// stored value changes only when it exercises; the response at row r is v_o / exp(logD(r)), the
-// reference's step-by-step discounted value (policy.rs:1049-1063) in exact arithmetic.
-// The pass applies row r + 1's rule (policy.rs:570-601) and accumulates, in FP64, the power sums
+// Fused backward step j over exercise row r = obs - 2 - j in ONE pass over HBM.
+// Cashflows are stored discounted to the origin, v_o = cashflow * exp(logD(stop)), so a
+// path's stored value changes only when it exercises; the response at row r is v_o / exp(logD(r)),
+// the reference's step-by-step discounted value (policy.rs:1049-1063) in exact arithmetic.
+// The pass applies row r + 1's rule (policy.rs:570-601) and accumulates the power sums
 // P_j = sum (x - c)^j, j <= 2d, and G_j = sum y (x - c)^j, j <= d, of the moneyness x = S/K of
 // in-the-money paths around a pivot c (row r + 1's fitted mean, else 1). The reduce kernel
 // recentres them on the exact mean, which merges the Welford pass (regression.rs:279-297) and the
 // Gram pass into one read of each value column.
+// Storage (Store) is binary64 or binary32; moneyness, continuation, exercise decision, responses
+// and power sums are binary64. Decisions made in binary32 arithmetic flip often enough to move
+// prices by several tenths of a standard error; binary32 storage alone flips rare near-tie
+// decisions, which backward induction then amplifies.
+// Block = (2048-path chunk, set of up to 8 strikes): every thread keeps its 8 paths' spot values
+// of rows r and r + 1 in registers for all strikes of the set; each wave reduces its sums by DPP
+// exchanges into LDS, and one barrier at the end combines the waves in fixed order.
+template <typename Store>
 __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
                                                           const uint2 *__restrict__ items, uint32_t nitems,
-                                                          uint32_t columns, const float *__restrict__ paths,
-                                                          float *__restrict__ values, const Model *__restrict__ models,
+                                                          uint32_t sets, const Store *__restrict__ paths,
+                                                          Store *__restrict__ values, const Model *__restrict__ models,
                                                           double *__restrict__ partials,
                                                           RowError *__restrict__ errors) {
-    __shared__ double scratch[14 * kWaves];
-    uint32_t wi, local;
-    if (!xcd_item_column(nitems, columns, wi, local)) return;
-    const uint2 item = items[wi];
-    const DevGroup &g = t.groups[item.x];
-    if (g.strike_begin + local >= g.strike_end || step >= g.fit_rows) return;
-    const uint32_t srow = g.strike_begin + local;
-    if (row_dead(errors, srow)) return;
-    const uint32_t r = g.obs - 2 - step;
-    const float K = static_cast<float>(t.strikes[srow]);
-    const uint32_t n = g.nt;
-    const float *S_r = paths + g.path_base + static_cast<uint64_t>(r) * n;
-    const float *S_next = paths + g.path_base + static_cast<uint64_t>(r + 1) * n;
-    float *V = values + g.value_base + static_cast<uint64_t>(local) * n;
-    const Model &m_next = models[g.model_base + local * g.obs + r + 1];
-    const bool apply = step > 0 && m_next.kind == kRuleRegression;
-    const double pivot = apply ? m_next.mean : 1.0;
-    const float d_next = static_cast<float>(t.disc0[g.row_base + r + 1]);
-    const float inv_d = static_cast<float>(1.0 / t.disc0[g.row_base + r]);
-    double acc[14];
+    __shared__ double red[kWaves][kBulkStrikes][16];
+    uint32_t wi, set;
+    if (!xcd_item_column(nitems, sets, wi, set)) return;
+    const uint2 item = load_item(items, wi);
+    const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
+    const uint32_t strike_begin = g->strike_begin, strike_end = g->strike_end;
+    const uint32_t s_first = strike_begin + set * kBulkStrikes;
+    if (s_first >= strike_end || step >= g->fit_rows) return;
+    const int ns = static_cast<int>(min(static_cast<uint32_t>(kBulkStrikes), strike_end - s_first));
+    const uint32_t obs = g->obs, r = obs - 2 - step, n = g->nt;
+    const Store *S_r = paths + g->path_base + static_cast<uint64_t>(r) * n;
+    const Store *S_next = S_r + n;
+    const uint32_t base = item.y * kBulkChunk + threadIdx.x;
+    Store s0v[kBulkPathsPerThread], s1v[kBulkPathsPerThread];
 #pragma unroll
-    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    float s1v[kPathsPerThread], s0v[kPathsPerThread], vv[kPathsPerThread];
-#pragma unroll
-    for (int k = 0; k < kPathsPerThread; ++k) {   // issue every load of the thread first
-        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
-        const bool in = p < n;
-        s1v[k] = in ? S_next[p] : 0.0f;
-        s0v[k] = in ? S_r[p] : 0.0f;
-        vv[k] = (in && step > 0) ? V[p] : 0.0f;
+    for (int k = 0; k < kBulkPathsPerThread; ++k) {
+        const uint32_t p = base + k * kBlock;
+        s0v[k] = p < n ? S_r[p] : Store(0);
+        s1v[k] = p < n ? S_next[p] : Store(0);
     }
+    const uint32_t row_base = g->row_base;
+    const double d_next = *uniform(t.disc0 + row_base + r + 1);
+    const double inv_d = 1.0 / *uniform(t.disc0 + row_base + r);
+    const uint32_t model_row = g->model_base + r + 1;
+    const uint64_t value_base = g->value_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // rows that already failed are skipped; the key is written by other blocks, so it is read
+    // through the vector path once for the whole set
+    uint32_t alive = 0;
+    for (int i = 0; i < ns; ++i)
+        alive |= (errors[s_first + i].key == kNoError ? 1u : 0u) << i;
+    alive = __builtin_amdgcn_readfirstlane(alive);
+    // cashflows of the next strike are loaded while the current one is processed
+    Store vnext[kBulkPathsPerThread];
+    auto load_values = [&](int i, Store (&dst)[kBulkPathsPerThread]) {
+        const Store *V = values + value_base + static_cast<uint64_t>(s_first + i - strike_begin) * n;
 #pragma unroll
-    for (int k = 0; k < kPathsPerThread; ++k) {
-        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
-        if (p >= n) break;
-        float v;
-        const float s1 = s1v[k];
-        if (step == 0) {
-            v = intrinsic(s1, K) * d_next;
-            V[p] = v;
-        } else {
-            v = vv[k];
-            if (apply && s1 > K) {
-                const float cont = predict(m_next, fp.degree, s1 / K);
-                if (!isfinite(cont)) {
-                    record_error(&errors[srow], error_key(step - 1, kStageApply, p));
-                } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
-                    v = (s1 - K) * d_next;
-                    V[p] = v;
+        for (int k = 0; k < kBulkPathsPerThread; ++k) {
+            const uint32_t p = base + k * kBlock;
+            dst[k] = (step > 0 && p < n) ? V[p] : Store(0);
+        }
+    };
+    load_values(0, vnext);
+    for (int i = 0; i < kBulkStrikes; ++i) {
+        if (i >= ns) break;
+        Store vv[kBulkPathsPerThread];
+#pragma unroll
+        for (int k = 0; k < kBulkPathsPerThread; ++k) vv[k] = vnext[k];
+        if (i + 1 < ns) load_values(i + 1, vnext);
+        const uint32_t srow = s_first + i, local = srow - strike_begin;
+        double acc[14];
+#pragma unroll
+        for (int q = 0; q < 14; ++q) acc[q] = 0.0;
+        const FOHIP_UNIFORM Model *mn = uniform(models + model_row + local * obs);
+        const bool apply = step > 0 && mn->kind == kRuleRegression;
+        const double pivot = apply ? mn->mean : 1.0;
+        const double K = *uniform(t.strikes + srow);
+        const double invK = 1.0 / K;
+        if (alive & (1u << i)) {
+            Store *V = values + value_base + static_cast<uint64_t>(local) * n;
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                const uint32_t p = base + k * kBlock;
+                if (p >= n) break;
+                const double s1 = s1v[k];
+                double v;
+                if (step == 0) {
+                    const Store vf = static_cast<Store>((s1 > K ? s1 - K : 0.0) * d_next);
+                    V[p] = vf;
+                    v = vf;
+                } else {
+                    v = vv[k];
+                    if (apply && s1 > K) {
+                        const double cont = continuation_fast(mn, s1);
+                        if (!isfinite(cont)) {
+                            record_error(&errors[srow], error_key(step - 1, kStageApply, p));
+                        } else if (should_exercise(fp.tie, s1 - K, cont)) {
+                            const Store vf = static_cast<Store>((s1 - K) * d_next);
+                            V[p] = vf;
+                            v = vf;
+                        }
+                    }
+                }
+                if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
+                const double s0 = s0v[k];
+                if (s0 > K) {
+                    const double u = fma(s0, invK, -pivot);
+                    const double y = v * inv_d;
+                    double pw = 1.0;
+#pragma unroll
+                    for (int j = 0; j <= 8; ++j) {
+                        acc[j] += pw;
+                        if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
+                        pw *= u;
+                    }
                 }
             }
         }
-        if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
-        const float s0 = s0v[k];
-        if (s0 > K) {
-            const double x = static_cast<double>(s0 / K);
-            const double y = static_cast<double>(v * inv_d);
-            const double u = x - pivot;
-            double pw = 1.0;
 #pragma unroll
-            for (int j = 0; j <= 8; ++j) {
-                acc[j] += pw;
-                if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
-                pw *= u;
-            }
+        for (int q = 0; q < 14; ++q) acc[q] = wave_total(acc[q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 14; ++q) red[wave][i][q] = acc[q];
+            red[wave][i][14] = pivot;
         }
     }
-    block_sum<14>(acc, scratch);
-    if (threadIdx.x == 0) {
-        double *out = partial_slot(partials, g, local, item.y);
-#pragma unroll
-        for (int i = 0; i < 14; ++i) out[i] = acc[i];
-        out[14] = pivot;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < ns * 15; idx += kBlock) {
+        const int i = idx / 15, q = idx % 15;
+        const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
+        const uint32_t local = s_first + i - strike_begin;
+        partials[static_cast<uint64_t>(g->part_base_t + local * g->chunks_t + item.y) * kPartialStride + q] = v;
     }
 }
 
@@ -715,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
     double acc[14];
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    for (uint32_t c = threadIdx.x; c < g.chunks_t; c += kBlock) {
+    for (uint32_t c = threadIdx.x; c < g.chunks_b; c += kBlock) {
         const double *in = partial_slot(partials, g, local, c);
 #pragma unroll
         for (int i = 0; i < 14; ++i) acc[i] += in[i];
@@ -725,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
     if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs + r];
     const double nd = acc[0];
-    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(nd))) return;
+    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(nd), t.strikes[srow])) return;
     double mu[9], nu[5];
     double mean = 0.0, scale = 1.0;
     if (fp.degree == 0) {
@@ -771,14 +857,14 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitPara
 
 // After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
 // cashflows for the valuation-origin decision (policy.rs:267-281). Stored values are either
-// discounted to the last fitted node (FP64 method, the reference's representation) or already
-// discounted to the origin (FP32 bulk method).
-template <typename Real>
+// discounted to the last fitted node (FP64 method, the reference's representation and arithmetic)
+// or values already discounted to the origin (fused methods, evaluation-form decisions).
+template <typename Real, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                       uint32_t nitems, uint32_t columns, const Real *__restrict__ paths,
                                                       const Real *__restrict__ values, const Model *__restrict__ models,
-                                                      double *__restrict__ partials, RowError *__restrict__ errors,
-                                                      int origin_discounted) {
+                                                      double *__restrict__ partials, RowError *__restrict__ errors) {
+    constexpr bool kBulk = FUSED;
     __shared__ double scratch[kWaves];
     uint32_t wi, local;
     if (!xcd_item_column(nitems, columns, wi, local)) return;
@@ -787,41 +873,40 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
     if (g.strike_begin + local >= g.strike_end) return;
     const uint32_t srow = g.strike_begin + local;
     if (row_dead(errors, srow)) return;
-    const Real K = static_cast<Real>(t.strikes[srow]);
+    const double K = t.strikes[srow];
     const uint32_t n = g.nt;
     const uint32_t last_row = g.origin_row;   // smallest positive-time exercise row
     const Real *S_last = paths + g.path_base + static_cast<uint64_t>(g.fit_rows ? last_row : g.obs - 1) * n;
     const Real *V = values + g.value_base + static_cast<uint64_t>(local) * n;
     const Model &m = models[g.model_base + local * g.obs + last_row];
     const bool apply = g.fit_rows > 0 && m.kind == kRuleRegression;
-    const bool stored_at_origin = origin_discounted && g.fit_rows > 0;
-    const Real disc = stored_at_origin ? Real(1) : static_cast<Real>(g.disc_final);
-    const Real d_last = stored_at_origin ? static_cast<Real>(t.disc0[g.row_base + last_row]) : Real(1);
+    const bool stored_at_origin = kBulk && g.fit_rows > 0;
+    const double disc = stored_at_origin ? 1.0 : g.disc_final;
+    const double d_last = stored_at_origin ? t.disc0[g.row_base + last_row] : 1.0;
+    const double invK = 1.0 / K;
     double sum[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
         if (p >= n) break;
-        Real v;
+        double v;
+        const double s1 = S_last[p];
         if (g.fit_rows == 0) {
-            v = intrinsic(S_last[p], K);
+            v = s1 > K ? s1 - K : 0.0;
         } else {
             v = V[p];
-            if (apply) {
-                const Real s1 = S_last[p];
-                if (s1 > K) {
-                    const Real cont = predict(m, fp.degree, s1 / K);
-                    if (!isfinite(cont)) {
-                        record_error(&errors[srow], error_key(g.fit_rows - 1, kStageApply, p));
-                    } else if (should_exercise(fp.tie, static_cast<double>(s1 - K), static_cast<double>(cont))) {
-                        v = (s1 - K) * d_last;
-                    }
+            if (apply && s1 > K) {
+                const double cont = kBulk ? continuation_fast(&m, s1) : predict_ref(&m, fp.degree, s1 / K);
+                if (!isfinite(cont)) {
+                    record_error(&errors[srow], error_key(g.fit_rows - 1, kStageApply, p));
+                } else if (should_exercise(fp.tie, s1 - K, cont)) {
+                    v = kBulk ? static_cast<double>(static_cast<Real>((s1 - K) * d_last)) : s1 - K;
                 }
             }
         }
         v *= disc;
         if (!isfinite(v)) record_error(&errors[srow], error_key(g.fit_rows, kStageDiscount, p));
-        sum[0] += static_cast<double>(v);
+        sum[0] += v;
     }
     block_sum<1>(sum, scratch);
     if (threadIdx.x == 0 && g.origin_row) partial_slot(partials, g, local, item.y)[0] = sum[0];
@@ -854,180 +939,197 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
 
 // Fused regenerate-and-evaluate (policy.rs:642-736). One evaluation path per thread, 256-path
 // chunks, a tile of TS strikes per block: the thread regenerates its path from the Philox
-// counters (binary64 state), applies every strike's frozen rule in row order (Real arithmetic for
-// moneyness, basis and payoff), and the block reduces each strike's discounted payoff moments.
+// counters in binary64, applies every strike's frozen rule in row order and the block reduces
+// each strike's discounted payoff moments. FAST selects the evaluation form of the rules (FP32
+// bulk method); otherwise the reference's recurrence and divisions are used (FP64 method).
+// Control flow over intervals, rows and strikes is wave-uniform (a failed lane keeps stepping
+// with its result discarded), so rules and tables are read with scalar loads.
 // With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along the transitions and each
 // strike's stopping row is held fixed (pathwise estimator under the frozen policy).
-template <typename Real, int TS, bool GREEKS>
+template <bool FAST, int TS, bool GREEKS>
 __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
                                                      uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
                                                      unsigned long long *__restrict__ sim_fail, int use_lds_hist) {
+    using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ uint32_t hist[];   // [TS][obs] when use_lds_hist
     __shared__ double sx[257], sf[257];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
     uint32_t wi, tile;
     if (!xcd_item_column(nitems, tiles, wi, tile)) return;
-    const uint2 item = items[wi];
-    const DevGroup &g = t.groups[item.x];
-    const uint32_t s_first = g.strike_begin + tile * TS;
-    if (s_first >= g.strike_end) return;
-    const int ns = static_cast<int>(min(static_cast<uint32_t>(TS), g.strike_end - s_first));
-    const uint32_t obs = g.obs;
-    const uint32_t local0 = s_first - g.strike_begin;
+    const uint2 item = load_item(items, wi);
+    const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
+    const uint32_t strike_begin = g->strike_begin, strike_end = g->strike_end;
+    const uint32_t s_first = strike_begin + tile * TS;
+    if (s_first >= strike_end) return;
+    const int ns = static_cast<int>(min(static_cast<uint32_t>(TS), strike_end - s_first));
+    const uint32_t obs = g->obs, last = obs - 1, intervals = g->intervals, n = g->ne;
+    const uint32_t step_base = g->step_base, node_base = g->node_base, row_base = g->row_base;
+    const uint32_t origin_row = g->origin_row, count_base = g->count_base;
+    const int policy = g->policy;
+    const double sigma = g->sigma;
+    const uint32_t local0 = s_first - strike_begin;
+    const Model *tile_models = models + g->model_base + local0 * obs;   // [strike][row]
+    const int degree = fp.degree, tie = fp.tie;
     load_zig(sx, sf);
     if (use_lds_hist) {
         for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
         __syncthreads();
     }
-    Real K[TS];
     uint32_t live = 0;
 #pragma unroll
     for (int i = 0; i < TS; ++i) {
-        K[i] = Real(1);
         if (i < ns) {
-            const uint32_t srow = s_first + i;
-            K[i] = static_cast<Real>(t.strikes[srow]);
-            const Model &m0 = models[g.model_base + (local0 + i) * obs];
-            const bool origin_exercise = g.origin_row && m0.kind == kRuleOrigin && m0.exercise;
-            if (!origin_exercise && errors[srow].key == kNoError) live |= 1u << i;
+            const FOHIP_UNIFORM Model *m0 = uniform(tile_models + i * obs);
+            const bool origin_exercise = origin_row && m0->kind == kRuleOrigin && m0->exercise;
+            const int healthy = __builtin_amdgcn_readfirstlane(errors[s_first + i].key == kNoError ? 1 : 0);
+            if (!origin_exercise && healthy) live |= 1u << i;
         }
     }
-    const uint32_t n = g.ne;
-    const uint32_t last = obs - 1;
     const uint32_t p = item.y * kEvalChunk + threadIdx.x;
-    const bool valid = p < n;
-    uint32_t active = valid ? live : 0u;
-    Real pay[TS];
-    Real dpay[TS][GREEKS ? 4 : 1];
+    const int lane = threadIdx.x & 63;
+    bool ok = p < n;
+    uint32_t active = ok ? live : 0u;
+    double pay[TS];
+    Tan dpay[TS][GREEKS ? 4 : 1];
 #pragma unroll
     for (int i = 0; i < TS; ++i) {
-        pay[i] = Real(0);
+        pay[i] = 0.0;
 #pragma unroll
-        for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Real(0);
+        for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Tan(0);
     }
-    if (valid) {
-        PathStream s;
-        s.init(g.eval_seed, g.eval_domain, p);
-        double spot = g.S0;
-        double tan[4] = {1.0, 0.0, 0.0, 0.0};
-        uint32_t row = g.origin_row;   // the origin row holds no regression rule
-        bool ok = true;
-        for (uint32_t iv = 0; iv < g.intervals; ++iv) {
-            double z;
-            uint32_t draws = 0;
-            if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
-                ok = false;
-                break;
-            }
-            const uint32_t step_idx = g.step_base + iv;
-            double growth;
-            const double before = spot;
-            const double next = gbm_advance(spot, t.diffusion[step_idx], t.log_drift[step_idx], z, growth);
-            if (!isfinite(next)) {
-                ok = false;
-                break;
-            }
-            spot = next;
-            if (GREEKS) {
-                if (before == 0.0) {
-                    tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
-                } else {
-                    const double dt = t.dt[step_idx];
-                    tan[0] = tan[0] * growth;
-                    tan[1] = fma(tan[1], growth, next * fma(t.sqrt_dt[step_idx], z, -g.sigma * dt));
-                    tan[2] = fma(tan[2], growth, next * dt);
-                    tan[3] = fma(tan[3], growth, -next * dt);
-                }
-            }
-            const uint32_t node = g.node_base + iv + 1;
-            const uint32_t flags = t.node_flags[node];
-            const bool cash = flags & kNodeCash;
-            const bool terminal = flags & kNodeTerminal;
-            if (cash && terminal) {
-                if (GREEKS && g.policy == 0 && !(spot > t.cash[node])) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
-                spot = apply_cash<double>(g.policy, t.cash[node], spot);
-            }
-            if ((flags & kNodeObserved) && row < last) {
-                const Real S = static_cast<Real>(spot);
-                const Real disc = static_cast<Real>(t.disc0[g.row_base + row]);
-#pragma unroll
-                for (int i = 0; i < TS; ++i) {
-                    if (!(active & (1u << i)) || !(S > K[i])) continue;
-                    const Model &m = models[g.model_base + (local0 + i) * obs + row];
-                    if (m.kind == kRuleRegression) {
-                        const Real intr = S - K[i];
-                        const Real cont = predict(m, fp.degree, S / K[i]);
-                        if (!isfinite(cont)) {
-                            record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
-                            active &= ~(1u << i);
-                            continue;
-                        }
-                        if (should_exercise(fp.tie, static_cast<double>(intr), static_cast<double>(cont))) {
-                            pay[i] = intr * disc;
-                            active &= ~(1u << i);
-                            if (!isfinite(pay[i]))
-                                record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPayoff, p));
-                            if (use_lds_hist)
-                                atomicAdd(&hist[i * obs + row], 1u);
-                            else
-                                atomicAdd(&counts[g.count_base + (local0 + i) * obs + row], 1u);
-                            if (GREEKS) {
-                                const double dd = t.disc0[g.row_base + row];
-                                dpay[i][0] = static_cast<Real>(tan[0] * dd);
-                                dpay[i][1] = static_cast<Real>(tan[1] * dd);
-                                dpay[i][2] = static_cast<Real>(tan[2] * dd - t.row_time[g.row_base + row] *
-                                                                                 static_cast<double>(pay[i]));
-                                dpay[i][3] = static_cast<Real>(tan[3] * dd);
-                            }
-                        }
-                    } else if (m.kind == kRuleNoSupport) {
-                        atomicAdd(&nosupport[g.count_base + (local0 + i) * obs + row], 1u);
-                    }
-                }
-            }
-            if (flags & kNodeObserved) row += 1;
-            if (cash && !terminal) {
-                if (GREEKS && g.policy == 0 && !(spot > t.cash[node])) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
-                spot = apply_cash<double>(g.policy, t.cash[node], spot);
+    PathStream s;
+    s.init(g->eval_seed, g->eval_domain, p);
+    double spot = g->S0;
+    double tan[4] = {1.0, 0.0, 0.0, 0.0};
+    uint32_t row = origin_row;   // the origin row holds no regression rule
+    for (uint32_t iv = 0; iv < intervals; ++iv) {
+        double z = 0.0;
+        uint32_t draws = 0;
+        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+        const uint32_t si = step_base + iv;
+        double growth;
+        const double next = gbm_advance(spot, *uniform(t.diffusion + si), *uniform(t.log_drift + si), z, growth);
+        if (!isfinite(next)) ok = false;
+        if (!ok) active = 0u;
+        const double before = spot;
+        spot = ok ? next : spot;
+        if (GREEKS) {
+            if (before == 0.0) {
+                tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
+            } else {
+                const double dt = *uniform(t.dt + si);
+                tan[0] = tan[0] * growth;
+                tan[1] = fma(tan[1], growth, spot * fma(*uniform(t.sqrt_dt + si), z, -sigma * dt));
+                tan[2] = fma(tan[2], growth, spot * dt);
+                tan[3] = fma(tan[3], growth, -spot * dt);
             }
         }
-        if (!ok) {
-            atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
-            active = 0;
-        } else {
-            const Real S = static_cast<Real>(spot);
-            const Real disc = static_cast<Real>(t.disc0[g.row_base + last]);
+        const uint32_t node = node_base + iv + 1;
+        const uint32_t flags = *uniform(t.node_flags + node);
+        const bool cash = flags & kNodeCash;
+        const bool terminal = flags & kNodeTerminal;
+        const double amount = cash ? *uniform(t.cash + node) : 0.0;
+        if (cash && terminal) {
+            if (GREEKS && policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
+            spot = apply_cash<double>(policy, amount, spot);
+        }
+        if ((flags & kNodeObserved) && row < last) {
+            const double disc = *uniform(t.disc0 + row_base + row);
+            const double t_row = GREEKS ? *uniform(t.row_time + row_base + row) : 0.0;
 #pragma unroll
             for (int i = 0; i < TS; ++i) {
-                if (!(active & (1u << i))) continue;
-                pay[i] = intrinsic(S, K[i]) * disc;
-                if (!isfinite(pay[i]))
-                    record_error(&errors[s_first + i], error_key(kEvalStepBase + last, kStageEvalPayoff, p));
-                if (use_lds_hist)
-                    atomicAdd(&hist[i * obs + last], 1u);
-                else
-                    atomicAdd(&counts[g.count_base + (local0 + i) * obs + last], 1u);
-                if (GREEKS && S > K[i]) {
-                    const double dd = t.disc0[g.row_base + last];
-                    dpay[i][0] = static_cast<Real>(tan[0] * dd);
-                    dpay[i][1] = static_cast<Real>(tan[1] * dd);
-                    dpay[i][2] = static_cast<Real>(tan[2] * dd - t.row_time[g.row_base + last] *
-                                                                     static_cast<double>(pay[i]));
-                    dpay[i][3] = static_cast<Real>(tan[3] * dd);
+                if (i >= ns) break;
+                const FOHIP_UNIFORM Model *m = uniform(tile_models + i * obs + row);
+                const int kind = m->kind;   // wave-uniform
+                const double K = m->K;      // set on every fitted row
+                const bool cand = ((active >> i) & 1u) && spot > K;
+                // branch-free: the continuation is formed for every row and used only where a
+                // regression rule exists
+                const bool reg = kind == kRuleRegression;
+                const double intr = spot - K;
+                const double cont = FAST ? continuation_fast(m, spot) : predict_ref(m, degree, spot / K);
+                const bool finite = isfinite(cont);
+                const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
+                const bool bad = reg && cand && !finite;
+                const double pv = intr * disc;
+                pay[i] = ex ? pv : pay[i];
+                if (GREEKS) {
+                    dpay[i][0] = ex ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
+                    dpay[i][1] = ex ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
+                    dpay[i][2] = ex ? static_cast<Tan>(tan[2] * disc - t_row * pv) : dpay[i][2];
+                    dpay[i][3] = ex ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
                 }
+                active &= ~(((ex || bad) ? 1u : 0u) << i);
+                const unsigned long long exm = __ballot(ex);
+                const unsigned long long nsm = __ballot(kind == kRuleNoSupport && cand);
+                const unsigned long long rare = __ballot(bad || (ex && !isfinite(pv)));
+                if (rare != 0ull) {   // non-finite continuation or payoff
+                    if (bad)
+                        record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
+                    if (ex && !isfinite(pv))
+                        record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPayoff, p));
+                }
+                if ((exm | nsm) != 0ull && lane == 0) {
+                    if (exm) {
+                        const uint32_t c = static_cast<uint32_t>(__popcll(exm));
+                        if (use_lds_hist)
+                            atomicAdd(&hist[i * obs + row], c);
+                        else
+                            atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
+                    }
+                    if (nsm)
+                        atomicAdd(&nosupport[count_base + (local0 + i) * obs + row],
+                                  static_cast<uint32_t>(__popcll(nsm)));
+                }
+            }
+        }
+        if (flags & kNodeObserved) row += 1;
+        if (cash && !terminal) {
+            if (GREEKS && policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
+            spot = apply_cash<double>(policy, amount, spot);
+        }
+    }
+    if (p < n && !ok) atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
+    {
+        const double disc = *uniform(t.disc0 + row_base + last);
+        const double t_last = *uniform(t.row_time + row_base + last);
+#pragma unroll
+        for (int i = 0; i < TS; ++i) {
+            if (i >= ns) break;
+            const double K = *uniform(t.strikes + s_first + i);
+            const bool hold = (active >> i) & 1u;
+            const double pv = (spot > K ? spot - K : 0.0) * disc;
+            pay[i] = hold ? pv : pay[i];
+            if (hold && !isfinite(pv))
+                record_error(&errors[s_first + i], error_key(kEvalStepBase + last, kStageEvalPayoff, p));
+            const unsigned long long hm = __ballot(hold);
+            if (hm != 0ull && lane == 0) {
+                const uint32_t c = static_cast<uint32_t>(__popcll(hm));
+                if (use_lds_hist)
+                    atomicAdd(&hist[i * obs + last], c);
+                else
+                    atomicAdd(&counts[count_base + (local0 + i) * obs + last], c);
+            }
+            if (GREEKS) {
+                const bool itm = hold && spot > K;
+                dpay[i][0] = itm ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
+                dpay[i][1] = itm ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
+                dpay[i][2] = itm ? static_cast<Tan>(tan[2] * disc - t_last * pv) : dpay[i][2];
+                dpay[i][3] = itm ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
             }
         }
     }
     // per-strike block reduction of (sum v, sum v^2[, sum dv_a, sum dv_a^2])
+    const bool contributes_any = p < n && ok;
 #pragma unroll
     for (int i = 0; i < TS; ++i) {
         if (i >= ns) break;
-        const bool contributes = valid && (live & (1u << i));
+        const bool contributes = contributes_any && (live & (1u << i));
         double vals[NACC];
-        const double v = contributes ? static_cast<double>(pay[i]) : 0.0;
+        const double v = contributes ? pay[i] : 0.0;
         vals[0] = v;
         vals[1] = v * v;
         if (GREEKS) {
@@ -1040,7 +1142,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
         }
         block_sum<NACC>(vals, scratch);
         if (threadIdx.x == 0) {
-            double *out = partials + static_cast<uint64_t>(g.part_base_e + (local0 + i) * g.chunks_e + item.y) *
+            double *out = partials + static_cast<uint64_t>(g->part_base_e + (local0 + i) * g->chunks_e + item.y) *
                                          kPartialStride;
 #pragma unroll
             for (int a = 0; a < NACC; ++a) out[a] = vals[a];
@@ -1050,7 +1152,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
         __syncthreads();
         for (uint32_t idx = threadIdx.x; idx < static_cast<uint32_t>(ns) * obs; idx += kBlock) {
             const uint32_t c = hist[idx];
-            if (c) atomicAdd(&counts[g.count_base + (local0 + idx / obs) * obs + idx % obs], c);
+            if (c) atomicAdd(&counts[count_base + (local0 + idx / obs) * obs + idx % obs], c);
         }
     }
 }
@@ -1263,13 +1365,19 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
     return hipGetLastError();
 }
 
-hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
                                 void *values, const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fit_bulk_step, dim3(xcd_blocks(nitems, max_strikes)), dim3(kBlock), 0, st, t, fp, step,
-                       items, nitems, max_strikes, static_cast<const float *>(paths), static_cast<float *>(values),
-                       models, partials, errors);
+    const uint32_t sets = (max_strikes + kBulkStrikes - 1) / kBulkStrikes;
+    const dim3 grid(xcd_blocks(nitems, sets));
+    if (f32)
+        hipLaunchKernelGGL(k_fit_bulk_step<float>, grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets,
+                           static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
+    else
+        hipLaunchKernelGGL(k_fit_bulk_step<double>, grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets,
+                           static_cast<const double *>(paths), static_cast<double *>(values), models, partials,
+                           errors);
     return hipGetLastError();
 }
 
@@ -1282,19 +1390,23 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
     return hipGetLastError();
 }
 
-hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, const uint2 *items,
-                            uint32_t nitems, uint32_t max_strikes, const void *paths, const void *values,
-                            const Model *models, double *partials, RowError *errors) {
+hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
+                            const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
+                            const void *values, const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
     const dim3 grid(xcd_blocks(nitems, max_strikes));
     if (f32)
-        hipLaunchKernelGGL(k_fit_final<float>, grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
+        hipLaunchKernelGGL((k_fit_final<float, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors, 1);
-    else
-        hipLaunchKernelGGL(k_fit_final<double>, grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
+                           errors);
+    else if (fused)
+        hipLaunchKernelGGL((k_fit_final<double, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
-                           errors, 0);
+                           errors);
+    else
+        hipLaunchKernelGGL((k_fit_final<double, false>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
+                           static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
+                           errors);
     return hipGetLastError();
 }
 
@@ -1316,7 +1428,7 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     return tile;
 }
 
-template <typename Real, int TS, bool G>
+template <bool FAST, int TS, bool G>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
@@ -1324,20 +1436,20 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    hipLaunchKernelGGL((k_evaluate<Real, TS, G>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+    hipLaunchKernelGGL((k_evaluate<FAST, TS, G>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
                        errors, sim_fail, use_lds);
     return hipGetLastError();
 }
 
-template <typename Real>
+template <bool FAST>
 static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
                                 RowError *errors, unsigned long long *sim_fail) {
 #define FOHIP_EVAL_CASE(TSV, GV)                                                                                   \
     if (tile == TSV && greeks == GV)                                                                               \
-        return launch_eval_t<Real, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, \
+        return launch_eval_t<FAST, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, \
                                             nosupport, errors, sim_fail);
     FOHIP_EVAL_CASE(1, false)
     FOHIP_EVAL_CASE(4, false)
@@ -1349,15 +1461,15 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, cons
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile, const DevTables &t,
+hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, uint32_t tile, const DevTables &t,
                            const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                            uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
     if (nitems == 0) return hipSuccess;
-    if (f32)
-        return launch_eval_r<float>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
+    if (fast)
+        return launch_eval_r<true>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
                                     counts, nosupport, errors, sim_fail);
-    return launch_eval_r<double>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
+    return launch_eval_r<false>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
                                  counts, nosupport, errors, sim_fail);
 }
 

@@ -38,21 +38,22 @@ hipError_t launch_fit_moments(hipStream_t st, const DevTables &t, const FitParam
 hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
                                      uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
                                      double *partials, Model *models, RowError *errors);
-// FP32 bulk: one fused HBM pass per backward step (apply, Welford + Gram via pivoted power sums)
-hipError_t launch_fit_bulk_step(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
+// FP32 bulk: one fused HBM pass per backward step (apply, Welford + Gram via pivoted power sums);
+// items are (group, kBulkChunk-path chunk)
+hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
                                 void *values, const Model *models, double *partials, RowError *errors);
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
-hipError_t launch_fit_final(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const Model *models,
                             double *partials, RowError *errors);
 hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
                                     const uint32_t *rows, uint32_t nrows, double *partials,
                                     Model *models, RowError *errors);
-hipError_t launch_evaluate(hipStream_t st, bool f32, bool greeks, uint32_t tile,
+hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, uint32_t tile,
                            const DevTables &t, const FitParams &fp, const uint2 *items,
                            uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                            const Model *models, double *partials, uint32_t *counts,

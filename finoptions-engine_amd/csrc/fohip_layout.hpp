@@ -4,7 +4,7 @@
 //   training paths   time-major [obs][nt] Real at path_base (coalesced per observation row)
 //   values           [strikes][nt] Real at value_base: the backward-induction cashflow per strike
 //   models           [strikes][obs] Model at model_base: fitted rule per (strike, observation row)
-//   partials         [strikes][chunks] x kPartialStride doubles: per-1024-path chunk reductions
+//   partials         [strikes][chunks] x kPartialStride doubles: per-chunk reductions
 // Step, node and row tables are concatenated over groups and indexed by the *_base fields.
 #pragma once
 
@@ -15,6 +15,9 @@ namespace fohip {
 constexpr int kChunk = 1024;            // paths per reduction chunk (fixes every reduction tree)
 constexpr int kPathsPerThread = 4;      // kChunk / threads per block
 constexpr int kEvalChunk = 256;         // evaluation paths per chunk: one path per thread
+constexpr int kBulkPathsPerThread = 8;  // fused fit pass: paths per thread
+constexpr int kBulkChunk = 2048;        // fused fit pass: paths per reduction chunk
+constexpr int kBulkStrikes = 8;         // fused fit pass: strikes per block
 constexpr int kPartialStride = 16;      // doubles per chunk partial
 constexpr int kMaxDegree = 4;
 
@@ -29,14 +32,22 @@ enum RuleKind : int32_t {
     kRulePending = 5          // regression being fitted
 };
 
-struct Model {
+struct alignas(64) Model {
+    // Evaluation form of an accepted regression, in spot units (first 72 bytes, two scalar
+    // loads): continuation = hz0 + z (hz1 + z (hz2 + z (hz3 + z hz4))) with z = fma(S, a, b),
+    // a = 1 / (scale K), b = -mean / scale; the same polynomial as the three-term recurrence
+    // below expanded into monomials of the standardized moneyness. K is set on every fitted row.
+    double hz[kMaxDegree + 1];
+    double a, b;
+    double K;
+    int32_t kind;
+    int32_t degree;
     double mean, scale;
     double alpha[kMaxDegree], beta[kMaxDegree], coef[kMaxDegree + 1], norm[kMaxDegree + 1];
     double condition;
     double prev_norm;
     double estimate;          // origin decision: estimated continuation
     uint32_t itm;             // in-the-money training paths
-    int32_t kind;
     int32_t rank;
     int32_t exercise;         // origin decision
 };
@@ -64,7 +75,7 @@ struct DevGroup {
     uint32_t part_base_e;     // evaluation partial slot base
     uint32_t count_base;      // exercise-count slot of (strike_begin, row 0)
     int32_t policy;           // cash-dividend policy
-    uint32_t pad;
+    uint32_t chunks_b;        // fused fit chunks (kBulkChunk paths each)
 };
 
 // Node flags

@@ -46,16 +46,22 @@ _MAX_PATHS: Final = (2**32) - 1
 class HipPrecision(str, Enum):
     """Numerical method of a HIP executor.
 
-    ``F64``: every path, payoff, regression pass and reduction in binary64 (the plain method that
-    matches the reference CPU path). ``F32_BULK``: paths, payoffs and basis evaluation in binary32;
-    moment/Gram accumulation, the per-node solve and price/SE reductions in binary64.
+    ``F64``: the reference's algorithm and arithmetic in binary64 (a Welford pass and d + 1
+    Stieltjes passes per exercise node); matches the reference CPU path.
+    ``FUSED_F64``: one fused pass per node (pivoted power sums, moment-form Stieltjes solve) over
+    binary64 spots and cashflows, binary64 exercise decisions, binary32 Greek tangents. Exercise
+    decisions agree with ``F64``, so prices agree far inside a standard error.
+    ``F32_BULK``: ``FUSED_F64`` with binary32 spots and cashflows. Rounding flips rare near-tie
+    exercise decisions and backward induction amplifies them: most strikes stay within 0.01 SE of
+    ``F64``, single strikes can move by 0.1-0.5 SE.
     """
 
     F64 = "f64"
     F32_BULK = "f32_bulk"
+    FUSED_F64 = "fused_f64"
 
 
-_PRECISION_CODE = {HipPrecision.F64: 0, HipPrecision.F32_BULK: 1}
+_PRECISION_CODE = {HipPrecision.F64: 0, HipPrecision.F32_BULK: 1, HipPrecision.FUSED_F64: 2}
 
 
 class _Handle:

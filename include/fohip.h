@@ -116,13 +116,19 @@ typedef struct {
 } fohip_error;
 
 /* ---- executor ---- */
-enum { FOHIP_PRECISION_F64 = 0, FOHIP_PRECISION_F32_BULK = 1 };
+/* F64: the reference's algorithm and arithmetic (Welford pass + d + 1 Stieltjes passes per node).
+ * FUSED_F64: one fused HBM pass per node (pivoted power sums, moment-form Stieltjes solve), binary64
+ *   spots and cashflows, binary64 exercise decisions; prices within 1e-10 of F64 in the absence of
+ *   near-tie decisions, Greek tangents accumulated in binary32.
+ * F32_BULK: FUSED_F64 with binary32 storage of spots and cashflows. Binary32 decision inputs flip
+ *   rare near-tie exercise decisions, which backward induction amplifies: typically below 0.01
+ *   SE, occasionally 0.1-0.5 SE on single strikes. */
+enum { FOHIP_PRECISION_F64 = 0, FOHIP_PRECISION_F32_BULK = 1, FOHIP_PRECISION_FUSED_F64 = 2 };
 
 typedef struct fohip_executor fohip_executor;
 
 /* Binds one HIP device explicitly. Fails with FOHIP_E_NO_DEVICE when the device does not exist;
- * there is no CPU fallback. `precision` selects plain FP64 or FP32 bulk (FP32 paths, payoff and
- * basis; FP64 Gram/solve and price/SE reductions). */
+ * there is no CPU fallback. `precision` is one of FOHIP_PRECISION_*. */
 int fohip_executor_create(int32_t device, int32_t precision, fohip_executor **out,
                           fohip_error *err);
 int fohip_executor_destroy(fohip_executor *executor);

@@ -1381,6 +1381,92 @@ int or_grouped_models(const or_grouped_input *in, uint32_t g, or_node_fit *fits_
  * fixed. Tangents of S w.r.t. (S0, sigma, r, q) follow the exact GBM transition and the dividend
  * jump (liquidator: dS_after = dS_before when S_before > D, else 0; survivor: unchanged). The rate
  * enters through logD(t) = -r t and the drift; the yield through Q(t) = q t. */
+/* Test utility, no reference counterpart: the evaluation of policy.rs:642-736 applied to group g of
+ * `eval_in` under the policy fitted on group g of `fit_in` (the reference's split between
+ * fit_policy and evaluate, policy.rs:306-336). stop_mode 0 decides stopping rows from the frozen
+ * rules, 1 does the same and records them in stops[strike * ne + path], 2 holds the recorded rows
+ * fixed. prices[strike] = mean discounted payoff. Both requests must share mesh and strikes; only
+ * S0, sigma, r and q may differ. */
+int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_input *eval_in, uint32_t g,
+                             int32_t *stops, int stop_mode, double *prices, or_error *err) {
+    zig_build();
+    group_plan_t gf, ge;
+    group_plan(fit_in, g, &gf);
+    group_plan(eval_in, g, &ge);
+    plan_t pf, pe;
+    plan_build(&gf.in, &pf);
+    plan_build(&ge.in, &pe);
+    uint32_t nt = fit_in->training_path_counts ? fit_in->training_path_counts[g] : fit_in->training_paths;
+    uint32_t ne = eval_in->evaluation_path_counts ? eval_in->evaluation_path_counts[g] : eval_in->evaluation_paths;
+    uint32_t first = fit_in->group_offsets[g], last = fit_in->group_offsets[g + 1];
+    uint32_t strikes = last - first;
+    double *train = malloc(sizeof(double) * pf.obs * (size_t)nt);
+    double *values = malloc(sizeof(double) * nt);
+    double *xb = malloc(sizeof(double) * nt), *yb = malloc(sizeof(double) * nt);
+    uint32_t *ib = malloc(sizeof(uint32_t) * nt);
+    or_node_fit *fits = calloc((size_t)strikes * pf.obs, sizeof(or_node_fit));
+    double *eval = NULL;
+    int rc = simulate_population(&pf, fit_in->initial_spots[g], nt, fit_in->training_seed,
+                                 fit_in->training_domain_start + g, NULL, train, err, PH_TRAIN_SIM);
+    for (uint32_t s = 0; !rc && s < strikes; ++s) {
+        or_estimator est = {fit_in->strikes[first + s], fit_in->degree, fit_in->rank_tolerance,
+                            fit_in->condition_limit, fit_in->tie_rule, fit_in->insufficient_support};
+        rc = fit_policy(&pf, gf.logd, fit_in->initial_spots[g], &est, train, nt,
+                        fits + (size_t)s * pf.obs, values, xb, yb, ib, err);
+    }
+    if (rc) goto out;
+    eval = malloc(sizeof(double) * pe.obs * (size_t)ne);
+    rc = simulate_population(&pe, eval_in->initial_spots[g], ne, eval_in->evaluation_seed,
+                             eval_in->evaluation_domain_start + g, NULL, eval, err, PH_EVAL_SIM);
+    if (rc) goto out;
+    {
+        const uint32_t rows = pe.obs - 1;
+        const double S0 = eval_in->initial_spots[g];
+        for (uint32_t sidx = 0; sidx < strikes; ++sidx) {
+            const or_node_fit *f = fits + (size_t)sidx * pf.obs;
+            const double K = fit_in->strikes[first + sidx];
+            if (rows > 0 && f[0].kind == 0 && f[0].exercise) { /* origin exercise: deterministic */
+                prices[sidx] = S0 > K ? S0 - K : 0.0;
+                continue;
+            }
+            double sum = 0.0;
+            for (uint32_t path = 0; path < ne; ++path) {
+                int64_t stop = -1;
+                if (stop_mode == 2) {
+                    stop = stops[(size_t)sidx * ne + path];
+                } else {
+                    for (uint32_t r = 0; r < rows && stop < 0; ++r) {
+                        if (f[r].kind != 3) continue;
+                        double spot = eval[(size_t)r * ne + path];
+                        if (spot <= K) continue;
+                        double cont = or_predict(&f[r].model, spot / K);
+                        if (should_exercise(fit_in->tie_rule, spot - K, cont)) stop = r;
+                    }
+                    if (stop < 0) stop = rows;
+                    if (stop_mode == 1) stops[(size_t)sidx * ne + path] = (int32_t)stop;
+                }
+                double spot = eval[(size_t)stop * ne + path];
+                double pay = spot > K ? spot - K : 0.0;
+                sum += pay * exp(ge.logd[pe.obs_nodes[stop]]);
+            }
+            prices[sidx] = sum / ne;
+        }
+    }
+out:
+    free(train);
+    free(values);
+    free(xb);
+    free(yb);
+    free(ib);
+    free(fits);
+    free(eval);
+    plan_free(&pf);
+    plan_free(&pe);
+    group_plan_free(&gf);
+    group_plan_free(&ge);
+    return rc;
+}
+
 int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *delta, double *vega,
                                double *rho, double *dq, or_error *err) {
     zig_build();

@@ -184,6 +184,11 @@ int or_price_grouped(const or_grouped_input *in, or_batch_output *out, int threa
 int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t group, double *delta,
                                double *vega, double *rho, double *dq, or_error *err);
 
+/* Test utility: group g of eval_in priced under the policy fitted on group g of fit_in;
+ * stop_mode 0 decide, 1 decide and record stops[strike * ne + path], 2 hold recorded rows. */
+int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_input *eval_in, uint32_t g,
+                             int32_t *stops, int stop_mode, double *prices, or_error *err);
+
 /* Fitted models of one group (strikes x exercise rows) for coefficient-level parity checks. */
 int or_grouped_models(const or_grouped_input *in, uint32_t group, or_node_fit *fits,
                       uint32_t capacity_per_strike, or_error *err);

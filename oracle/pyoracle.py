@@ -588,5 +588,27 @@ def grouped_pathwise_greeks(req: GroupedRequest, group: int) -> dict[str, np.nda
     return res
 
 
+def grouped_frozen_prices(fit_req: GroupedRequest, eval_req: GroupedRequest, group: int, *,
+                          stops: np.ndarray | None = None, record: bool = False) -> dict:
+    """Prices of ``eval_req``'s group under the policy fitted on ``fit_req``'s group. With
+    ``record`` the stopping rows are returned; passing ``stops`` holds them fixed."""
+    fs, fkeep = fit_req.struct()
+    es, ekeep = eval_req.struct()
+    a, b = int(fit_req.group_offsets[group]), int(fit_req.group_offsets[group + 1])
+    ne = int(eval_req.evaluation_path_counts[group]) if eval_req.evaluation_path_counts is not None \
+        else int(eval_req.evaluation_paths)
+    prices = np.zeros(b - a)
+    mode = 2 if stops is not None else (1 if record else 0)
+    buf = (np.ascontiguousarray(stops, dtype=np.int32) if stops is not None
+           else np.zeros((b - a) * ne if record else 1, dtype=np.int32))
+    err = OrError()
+    rc = lib().or_grouped_frozen_prices(C.byref(fs), C.byref(es), C.c_uint32(group),
+                                        _p(buf, C.c_int32), C.c_int(mode),
+                                        _p(prices, C.c_double), C.byref(err))
+    if rc:
+        raise OracleError(err)
+    return {"prices": prices, "stops": buf.reshape(b - a, ne) if mode else None}
+
+
 def threads_available() -> int:
     return len(os.sched_getaffinity(0))

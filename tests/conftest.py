@@ -50,6 +50,13 @@ def hip_f32():
     return fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
 
 
+@pytest.fixture(scope="session")
+def hip_fused():
+    import finoptions_hip as fh
+
+    return fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
+
+
 # ------------------------------------------------------------------ canonical request builders
 
 def c1_inputs():

@@ -353,13 +353,66 @@ def test_uniform_mesh_union_with_cash_times(oracle):
     assert t.tolist() == [0.0, 1.0 * (1 / 3), 1.0 * (2 / 3), 1.0]
 
 
-def test_pathwise_greeks_match_frozen_policy_finite_differences(oracle):
-    cols = _c1_like_cols()
-    req = oracle_grouped(cols, base_intervals=4, training_paths=8192, evaluation_paths=8192,
-                         seed=5)
-    greeks = oracle.grouped_pathwise_greeks(req, 0)
-    # delta and vega pathwise estimators are positive for a call and bounded (delta <= 1)
-    assert 0.0 < greeks["delta"][0] < 1.0
-    assert greeks["vega"][0] > 0.0
-    assert greeks["rho"][0] > 0.0
-    assert greeks["dividend_yield"][0] < 0.0
+def _greek_request(oracle, **kw):
+    base = dict(group_offsets=np.array([0, 3], dtype=np.uint32), initial_spots=np.array([100.0]),
+                strikes=np.array([90.0, 100.0, 110.0]), maturities=np.array([1.0]),
+                rates=np.array([0.03]), volatilities=np.array([0.25]), training_paths=4096,
+                evaluation_paths=4096, training_seed=5, evaluation_seed=5,
+                training_domain_start=1, evaluation_domain_start=2,
+                dividend_yields=np.array([0.02]), base_intervals=8, insufficient_support=1)
+    base.update(kw)
+    return oracle.GroupedRequest(**base)
+
+
+GREEK_INPUTS = {"delta": "initial_spots", "vega": "volatilities", "rho": "rates",
+                "dividend_yield": "dividend_yields"}
+
+
+@pytest.mark.parametrize("scenario", ["yield", "cash-liquidator", "cash-survivor",
+                                      "large-liquidator-dividend"])
+def test_pathwise_greeks_are_the_derivative_of_the_stopped_payoff(oracle, scenario):
+    """The pathwise estimator is d/dtheta of the mean discounted payoff with every path's stopping
+    row held fixed (frozen policy, frozen stopping times). A central difference of that function
+    of (S0, sigma, r, q) - evaluated by plain re-simulation with common random numbers - must
+    match the tangent recursion to O(h^2). A dropped drift term, a wrong sign in the discount
+    term of rho or a tangent that survives a liquidating dividend each fail this by orders of
+    magnitude."""
+    import dataclasses
+
+    kw = {}
+    if scenario.startswith("cash"):
+        kw = dict(dividend_yields=None, cash_offsets=np.array([0, 2], dtype=np.int64),
+                  cash_times=np.array([0.3, 0.7]), cash_amounts=np.array([3.0, 2.0]),
+                  cash_policy=0 if scenario == "cash-liquidator" else 1)
+    elif scenario == "large-liquidator-dividend":
+        # about 1 path in 40 is liquidated to zero; its tangent must vanish with it
+        kw = dict(dividend_yields=None, cash_offsets=np.array([0, 1], dtype=np.int64),
+                  cash_times=np.array([0.5]), cash_amounts=np.array([60.0]), cash_policy=0,
+                  strikes=np.array([30.0, 40.0, 50.0]))
+    base = _greek_request(oracle, **kw)
+    pathwise = oracle.grouped_pathwise_greeks(base, 0)
+    stops = oracle.grouped_frozen_prices(base, base, 0, record=True)["stops"]
+    for greek, field in GREEK_INPUTS.items():
+        value = getattr(base, field)
+        if value is None:
+            continue
+        # small enough that no path is expected to cross the payoff kink at S = K (expected
+        # crossings ~ paths x density x h x |dS/dtheta| < 0.01), large enough for roundoff ~eps/h
+        h = (1e-6 if greek == "delta" else 1e-5) * abs(float(value[0]))
+        up = dataclasses.replace(base, **{field: value + h})
+        down = dataclasses.replace(base, **{field: value - h})
+        fd = (oracle.grouped_frozen_prices(base, up, 0, stops=stops)["prices"]
+              - oracle.grouped_frozen_prices(base, down, 0, stops=stops)["prices"]) / (2 * h)
+        np.testing.assert_allclose(pathwise[greek], fd, rtol=1e-6, atol=1e-6,
+                                   err_msg=f"{scenario} {greek}")
+    assert 0.0 < pathwise["delta"][0] <= 1.0
+
+
+def test_frozen_policy_evaluation_reproduces_the_priced_request(oracle):
+    """Fitting and evaluating the same request through the frozen-policy entry reproduces the
+    grouped pricing (same fits and stopping decisions; the mean is a plain sum here and a
+    Welford mean there, so the last bits differ)."""
+    req = _greek_request(oracle)
+    frozen = oracle.grouped_frozen_prices(req, req, 0)["prices"]
+    priced = oracle.price_grouped(req)["prices"]
+    np.testing.assert_allclose(frozen, priced, rtol=1e-13)
