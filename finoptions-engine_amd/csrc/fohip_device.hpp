@@ -290,6 +290,37 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double *scratch) {
     __syncthreads();
 }
 
+// Wave sums of N <= 16 doubles per lane through a transpose in LDS: every lane stores its N values,
+// then lane 4q + j (j < 4) adds entries 16j .. 16j + 15 of value q in index order and the quad
+// combines its four partials by two commutative exchanges. Lane 4q returns the total of value q
+// (other lanes return unspecified values). `buf` is this wave's private area of N * 66 doubles.
+// The order of every addition is fixed by lane indices, so the sums are reproducible.
+template <int N>
+__device__ __forceinline__ double wave_sums_lds(const double (&v)[N], double *buf) {
+    static_assert(N <= 16, "at most 16 values");
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < N; ++q) buf[q * 66 + lane] = v[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int q = lane >> 2, part = lane & 3;
+    double sum = 0.0;
+    if (q < N) {
+        const double2 *row = reinterpret_cast<const double2 *>(buf + q * 66 + part * 16);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double2 pair = row[k];
+            sum += pair.x;
+            sum += pair.y;
+        }
+    }
+    sum += dpp_partner<0xB1>(sum);   // quad_perm [1,0,3,2]
+    sum += dpp_partner<0x4E>(sum);   // quad_perm [2,3,0,1]
+    __builtin_amdgcn_wave_barrier();   // buf may be rewritten by the next call
+    return sum;
+}
+
 // Chan et al. pairwise merge of (count, mean, M2) with a formula symmetric in its operands.
 struct Moments {
     double n, mean, m2;

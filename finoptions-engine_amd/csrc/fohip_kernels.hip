@@ -611,8 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
 // prices by several tenths of a standard error; binary32 storage alone flips rare near-tie
 // decisions, which backward induction then amplifies.
 // Block = (2048-path chunk, set of up to 8 strikes): every thread keeps its 8 paths' spot values
-// of rows r and r + 1 in registers for all strikes of the set; each wave reduces its sums by DPP
-// exchanges into LDS, and one barrier at the end combines the waves in fixed order.
+// of rows r and r + 1 in registers for all strikes of the set; each wave reduces its sums through
+// an LDS transpose, and one barrier at the end combines the waves in fixed order.
 template <typename Store>
 __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
                                                           const uint2 *__restrict__ items, uint32_t nitems,
@@ -621,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
                                                           double *__restrict__ partials,
                                                           RowError *__restrict__ errors) {
     __shared__ double red[kWaves][kBulkStrikes][16];
+    __shared__ double transpose[kWaves][14 * 66];
     uint32_t wi, set;
     if (!xcd_item_column(nitems, sets, wi, set)) return;
     const uint2 item = load_item(items, wi);
@@ -718,13 +719,9 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
                 }
             }
         }
-#pragma unroll
-        for (int q = 0; q < 14; ++q) acc[q] = wave_total(acc[q]);
-        if (lane == 0) {
-#pragma unroll
-            for (int q = 0; q < 14; ++q) red[wave][i][q] = acc[q];
-            red[wave][i][14] = pivot;
-        }
+        const double total = wave_sums_lds<14>(acc, transpose[wave]);
+        if ((lane & 3) == 0 && lane < 56) red[wave][i][lane >> 2] = total;
+        if (lane == 0) red[wave][i][14] = pivot;
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < ns * 15; idx += kBlock) {
@@ -786,29 +783,37 @@ __device__ __forceinline__ double binom9(int j, int k) {
 }
 
 // Support decision, exact location/scale and standardized moments from the pivoted power sums
-// (binomial recentring), then the Stieltjes solve above.
-__global__ __launch_bounds__(kBlock) void k_fit_bulk_reduce(DevTables t, FitParams fp, uint32_t step,
-                                                            const uint32_t *__restrict__ rows,
-                                                            double *__restrict__ partials,
-                                                            Model *__restrict__ models,
-                                                            RowError *__restrict__ errors) {
-    __shared__ double scratch[14 * kWaves];
+// (binomial recentring), then the Stieltjes solve above. One wave per strike row reduces the chunk
+// partials of (strike row, exercise row obs - 2 - step) in fixed order (lane-strided chunk sums,
+// then an LDS transpose) and its lane 0 fits the rule.
+__global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams fp, uint32_t step,
+                                                        const uint32_t *__restrict__ rows,
+                                                        double *__restrict__ partials, Model *__restrict__ models,
+                                                        RowError *__restrict__ errors) {
+    __shared__ double buf[14 * 66];
     const uint32_t srow = rows[blockIdx.x];
     const DevGroup &g = t.groups[t.strike_group[srow]];
     if (step >= g.fit_rows || row_dead(errors, srow)) return;
+    const int lane = threadIdx.x & 63;
     const uint32_t local = srow - g.strike_begin;
     const uint32_t r = g.obs - 2 - step;
     double acc[14];
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    for (uint32_t c = threadIdx.x; c < g.chunks_b; c += kBlock) {
+    for (uint32_t c = lane; c < g.chunks_b; c += 64) {
         const double *in = partial_slot(partials, g, local, c);
 #pragma unroll
         for (int i = 0; i < 14; ++i) acc[i] += in[i];
     }
+    const double total = wave_sums_lds<14>(acc, buf);
+    if ((lane & 3) == 0 && lane < 56) buf[lane >> 2] = total;   // totals of values 0..13
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) acc[i] = buf[i];
     const double pivot = partial_slot(partials, g, local, 0)[14];
-    block_sum<14>(acc, scratch);
-    if (threadIdx.x != 0) return;
     Model &m = models[g.model_base + local * g.obs + r];
     const double nd = acc[0];
     if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(nd), t.strikes[srow])) return;
@@ -1054,35 +1059,35 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
                 const bool finite = isfinite(cont);
                 const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
                 const bool bad = reg && cand && !finite;
-                const double pv = intr * disc;
-                pay[i] = ex ? pv : pay[i];
-                if (GREEKS) {
-                    dpay[i][0] = ex ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
-                    dpay[i][1] = ex ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
-                    dpay[i][2] = ex ? static_cast<Tan>(tan[2] * disc - t_row * pv) : dpay[i][2];
-                    dpay[i][3] = ex ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
-                }
-                active &= ~(((ex || bad) ? 1u : 0u) << i);
-                const unsigned long long exm = __ballot(ex);
-                const unsigned long long nsm = __ballot(kind == kRuleNoSupport && cand);
-                const unsigned long long rare = __ballot(bad || (ex && !isfinite(pv)));
-                if (rare != 0ull) {   // non-finite continuation or payoff
+                // state changes only where a lane stops: one wave-uniform branch
+                const unsigned long long stopm = __ballot(ex || bad);
+                if (stopm != 0ull) {
+                    const double pv = intr * disc;
+                    pay[i] = ex ? pv : pay[i];
+                    if (GREEKS) {
+                        dpay[i][0] = ex ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
+                        dpay[i][1] = ex ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
+                        dpay[i][2] = ex ? static_cast<Tan>(tan[2] * disc - t_row * pv) : dpay[i][2];
+                        dpay[i][3] = ex ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
+                    }
+                    active &= ~(((ex || bad) ? 1u : 0u) << i);
                     if (bad)
                         record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
                     if (ex && !isfinite(pv))
                         record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPayoff, p));
-                }
-                if ((exm | nsm) != 0ull && lane == 0) {
-                    if (exm) {
+                    const unsigned long long exm = __ballot(ex);
+                    if (exm != 0ull && lane == 0) {
                         const uint32_t c = static_cast<uint32_t>(__popcll(exm));
                         if (use_lds_hist)
                             atomicAdd(&hist[i * obs + row], c);
                         else
                             atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
                     }
-                    if (nsm)
-                        atomicAdd(&nosupport[count_base + (local0 + i) * obs + row],
-                                  static_cast<uint32_t>(__popcll(nsm)));
+                }
+                if (kind == kRuleNoSupport) {   // wave-uniform
+                    const unsigned long long cm = __ballot(cand);
+                    if (cm != 0ull && lane == 0)
+                        atomicAdd(&nosupport[count_base + (local0 + i) * obs + row], static_cast<uint32_t>(__popcll(cm)));
                 }
             }
         }
@@ -1385,8 +1390,7 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors) {
     if (nrows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fit_bulk_reduce, dim3(nrows), dim3(kBlock), 0, st, t, fp, step, rows, partials, models,
-                       errors);
+    hipLaunchKernelGGL(k_fit_bulk_reduce, dim3(nrows), dim3(64), 0, st, t, fp, step, rows, partials, models, errors);
     return hipGetLastError();
 }
 

@@ -38,8 +38,8 @@ hipError_t launch_fit_moments(hipStream_t st, const DevTables &t, const FitParam
 hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const FitParams &fp,
                                      uint32_t step, int k, const uint32_t *rows, uint32_t nrows,
                                      double *partials, Model *models, RowError *errors);
-// FP32 bulk: one fused HBM pass per backward step (apply, Welford + Gram via pivoted power sums);
-// items are (group, kBulkChunk-path chunk)
+// Fused method: one HBM pass per backward step (apply, Welford + Gram via pivoted power sums) over
+// (group, kBulkChunk-path chunk) items, then one wave per strike row reduces and fits.
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
                                 void *values, const Model *models, double *partials, RowError *errors);
