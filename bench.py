@@ -31,6 +31,10 @@ sys.path.insert(0, str(ROOT / "finoptions-engine_amd"))
 
 METRIC = "LSM path-steps/sec incl. delta/vega vs plain FP64, 1 GPU, |bias|<0.1 SE"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at this workload, scaled by
+# a copy-kernel calibration (profiles/c2_fused_hbm_traffic.md); average over the 49 launches
+MEASURED_TRAFFIC = {"k_fit_bulk_step": {"read": 543.2e6, "written": 101.1e6,
+                                        "source": "profiles/c2_fused_hbm_traffic.md"}}
 INTERVALS = 50
 STRIKES = 64
 
@@ -171,7 +175,10 @@ def main() -> int:
     achieved = alg / (hbm_avg_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "kernel": hbm_name, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_us": round(hbm_avg_ms * 1e3, 2),
+                "traffic": (MEASURED_TRAFFIC[hbm_name]["read"] + MEASURED_TRAFFIC[hbm_name]["written"]
+                            if hbm_name in MEASURED_TRAFFIC and paths == 1 << 20 else None),
+                "traffic_source": MEASURED_TRAFFIC.get(hbm_name, {}).get("source"),
+                "avg_launch_us": round(hbm_avg_ms * 1e3, 2),
                 "algorithmic_bytes_per_launch": alg,
                 "dominant_kernel": name, "dominant_avg_launch_us": round(avg_ms * 1e3, 2),
                 "dominant_share": round(total_ms / sum(v[1] for v in stats.values()), 3)}
