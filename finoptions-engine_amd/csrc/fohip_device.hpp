@@ -241,6 +241,18 @@ __device__ __forceinline__ bool should_exercise(int tie, double intrinsic, doubl
     return tie == 0 ? intrinsic > continuation : intrinsic >= continuation;
 }
 
+// Tangents of the observed stock S = F + A(node) w.r.t. (S0, sigma, r, q) from those of the
+// simulated state (d/dF0, d/dsigma_F, drift d/dr, d/dq), by the chain rule through F0 = S0 - A0(r)
+// and sigma_F(S0, sigma, r), plus dA(node)/dr. The spot models have c = (0, 1, 0, 0) and dA = 0,
+// which returns the state tangents unchanged.
+__device__ __forceinline__ void observed_tangents(const double (&tan)[4], const FOHIP_UNIFORM double *c, double dA,
+                                                  double (&o)[4]) {
+    o[0] = tan[0] + c[0] * tan[1];
+    o[1] = c[1] * tan[1];
+    o[2] = tan[2] + c[2] * tan[0] + c[3] * tan[1] + dA;
+    o[3] = tan[3];
+}
+
 // ------------------------------------------------------- deterministic block reductions
 
 constexpr int kBlock = 256;      // threads per block

@@ -453,6 +453,12 @@ struct GroupPlan {
     uint64_t nt = 0, ne = 0;
     uint64_t train_seed = 0, eval_seed = 0;
     uint32_t train_domain = 0, eval_domain = 0;
+    // escrowed cash dividends: the plan simulates the prepaid forward from `start` with no cash
+    // events; exercise nodes observe F + prepaid[node]; chain = Greek chain factors
+    bool escrowed = false;
+    double start = 0.0;
+    std::vector<double> prepaid, prepaid_dr;   // per node
+    double chain[4] = {0.0, 1.0, 0.0, 0.0};
 };
 
 struct RunSpec {
@@ -476,7 +482,7 @@ struct RowOut {
 
 struct GroupTables {
     std::vector<DevGroup> groups;
-    std::vector<double> log_drift, diffusion, dt, sqrt_dt, cash, disc0, disc_step, row_time;
+    std::vector<double> log_drift, diffusion, dt, sqrt_dt, cash, disc0, disc_step, row_time, row_escrow, row_escrow_dr;
     std::vector<float> log_drift32, diffusion32;
     std::vector<uint32_t> node_flags, row_node;
 };
@@ -509,6 +515,9 @@ static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<d
         const uint32_t nodes = static_cast<uint32_t>(p.times.size());
         g.S0 = p.S0;
         g.sigma = p.sigma;
+        g.escrowed = p.escrowed ? 1 : 0;
+        g.start = p.escrowed ? p.start : p.S0;
+        for (int k = 0; k < 4; ++k) g.chain[k] = p.escrowed ? p.chain[k] : (k == 1 ? 1.0 : 0.0);
         g.train_seed = p.train_seed;
         g.eval_seed = p.eval_seed;
         g.train_domain = p.train_domain;
@@ -587,6 +596,11 @@ static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<d
             gt.disc_step.push_back(ds);
             gt.row_time.push_back(p.times[node]);
             gt.row_node.push_back(node);
+            // reconstruct_observed_stock (path.rs:844-880): the origin observes S0 and expiry
+            // (the last grid node) adds no prepaid cash
+            const bool adds = p.escrowed && node != 0 && node + 1 != nodes;
+            gt.row_escrow.push_back(adds ? p.prepaid[node] : 0.0);
+            gt.row_escrow_dr.push_back(adds ? p.prepaid_dr[node] : 0.0);
         }
         const uint32_t later_final = g.fit_rows ? p.obs_nodes[g.origin_row] : p.obs_nodes[g.obs - 1];
         double dfinal = 0.0;
@@ -673,6 +687,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         const uint32_t row1 = plans.back().strike_end;
         const uint32_t nrows = row1 - row0;
         std::vector<double> bstrikes(strikes.begin() + row0, strikes.begin() + row1);
+        std::vector<double> binv(bstrikes.size());   // 1/K, correctly rounded as on the device
+        for (size_t i = 0; i < bstrikes.size(); ++i) binv[i] = 1.0 / bstrikes[i];
         for (auto &p : plans) {
             p.strike_begin -= row0;
             p.strike_end -= row0;
@@ -724,7 +740,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             gt.groups.size() * sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8,
             gt.log_drift32.size() * 4, gt.diffusion32.size() * 4, gt.dt.size() * 8, gt.sqrt_dt.size() * 8,
             gt.cash.size() * 8, gt.node_flags.size() * 4, gt.disc0.size() * 8, gt.disc_step.size() * 8,
-            gt.row_time.size() * 8, gt.row_node.size() * 4, bstrikes.size() * 8, strike_group.size() * 4};
+            gt.row_time.size() * 8, gt.row_node.size() * 4, bstrikes.size() * 8, strike_group.size() * 4,
+            gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, binv.size() * 8};
         size_t tables_bytes = 0;
         std::vector<size_t> offs;
         for (size_t b : tables_bytes_list) {
@@ -752,7 +769,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
                                     gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
                                     gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
-                                    gt.row_node.data(), bstrikes.data(), strike_group.data()};
+                                    gt.row_node.data(), bstrikes.data(), strike_group.data(),
+                                    gt.row_escrow.data(), gt.row_escrow_dr.data(), binv.data()};
         for (size_t i = 0; i < offs.size(); ++i)
             if (tables_bytes_list[i]) HIP_TRY(hipMemcpyAsync(tb + offs[i], host_parts[i], tables_bytes_list[i], hipMemcpyHostToDevice, st));
         DevTables t;
@@ -771,6 +789,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         t.row_node = reinterpret_cast<const uint32_t *>(tb + offs[12]);
         t.strikes = reinterpret_cast<const double *>(tb + offs[13]);
         t.strike_group = reinterpret_cast<const uint32_t *>(tb + offs[14]);
+        t.row_escrow = reinterpret_cast<const double *>(tb + offs[15]);
+        t.row_escrow_dr = reinterpret_cast<const double *>(tb + offs[16]);
+        t.inv_strikes = reinterpret_cast<const double *>(tb + offs[17]);
         HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items_t.data(), items_t.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ex->items_e.ptr, items_e.data(), items_e.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(ex->items_b.ptr, items_b.data(), items_b.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
@@ -815,7 +836,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // ---- evaluation
         const bool greeks = spec.greeks != 0;
         const uint32_t tile = eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
-        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, tile, t, fp, ex->items_e.as<uint2>(),
+        bool escrow = false;   // the escrowed instantiation also serves spot-model groups
+        for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
+        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ex->items_e.as<uint2>(),
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
                                 ex->fail_e.as<unsigned long long>()));
@@ -1170,6 +1193,104 @@ int64_t search_time(const std::vector<double> &times, double t) {
 
 }  // namespace
 
+// validate_escrowed_row (input.rs:798-832) with escrowed_origin_prepaid_cash (input.rs:834-860):
+// F0 = S0 - sum_k D_k exp(-r t_k) (schedule order) must be finite and positive, and a spot-matched
+// volatility sigma S0 / F0 finite and positive. `sigma_f` receives the path volatility
+// (path_volatility, pricing.rs:840-860).
+static int validate_escrowed_group(size_t g, double S0, double rate, double sigma, const double *times,
+                                   const double *amounts, size_t count, int32_t convention, double *sigma_f,
+                                   fohip_error *err) {
+    double prepaid_cash = 0.0;
+    for (size_t e = 0; e < count; ++e) {
+        const double discounted = amounts[e] * std::exp(-rate * times[e]);
+        prepaid_cash += discounted;
+        if (!std::isfinite(discounted) || !std::isfinite(prepaid_cash))
+            return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0,
+                        "escrowed prepaid cash is not finite at batch row %zu, event %zu: time=%.17g, amount=%.17g, "
+                        "rate=%.17g",
+                        g, e, times[e], amounts[e], rate);
+    }
+    const double prepaid_forward = S0 - prepaid_cash;
+    if (!std::isfinite(prepaid_forward) || prepaid_forward <= 0.0)
+        return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0,
+                    "escrowed prepaid forward must be strictly positive at batch row %zu; initial spot is %.17g and "
+                    "origin prepaid cash is %.17g",
+                    g, S0, prepaid_cash);
+    double path_sigma = sigma;
+    if (convention == FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN) {
+        path_sigma = sigma * S0 / prepaid_forward;
+        if (!std::isfinite(path_sigma) || path_sigma <= 0.0)
+            return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0,
+                        "escrowed spot-matched volatility is invalid at batch row %zu: supplied=%.17g, initial_spot=%.17g, "
+                        "prepaid_forward=%.17g, transformed=%.17g",
+                        g, sigma, S0, prepaid_forward, path_sigma);
+    }
+    if (sigma_f) *sigma_f = path_sigma;
+    return FOHIP_OK;
+}
+
+// EscrowedGbmPathPlan::try_new (path.rs:628-666) with escrowed_prepaid_cash (path.rs:897-944):
+// prepaid[node] = prepaid[node + 1] exp(logD[node + 1] - logD[node]) + same-node cash (expiry
+// holds a terminal amount), prepaid[0] replaced by sum_k D_k exp(logD[node_k]) in schedule order.
+// Also the rate derivative of every prepaid value and the Greek chain factors of the observed
+// stock through F0 = S0 - prepaid[0] and, spot matched, sigma_F = sigma S0 / F0.
+static int escrowed_plan(GroupPlan &p, const std::vector<double> &times, const std::vector<uint32_t> &cash_node,
+                         const std::vector<double> &cash_a, double S0, double sigma, int32_t convention,
+                         size_t group_row, fohip_error *err) {
+    const size_t nodes = times.size();
+    p.escrowed = true;
+    p.prepaid.assign(nodes, 0.0);
+    p.prepaid_dr.assign(nodes, 0.0);
+    size_t event = cash_a.size();
+    const size_t expiry = nodes - 1;
+    if (event > 0 && cash_node[event - 1] == expiry) {
+        event -= 1;
+        p.prepaid[expiry] = cash_a[event];
+    }
+    for (size_t k = nodes - 1; k-- > 0;) {
+        const double interval_log_discount = p.logd[k + 1] - p.logd[k];
+        const double factor = std::exp(interval_log_discount);
+        const double discounted_future = p.prepaid[k + 1] * factor;
+        double same_node_cash = 0.0;
+        if (event > 0 && cash_node[event - 1] == k) {
+            event -= 1;
+            same_node_cash = cash_a[event];
+        }
+        const double value = discounted_future + same_node_cash;
+        if (!std::isfinite(value))
+            return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0,
+                        "row %zu escrowed prepaid cash is not finite at node %zu", group_row, k);
+        p.prepaid[k] = value;
+        // d factor / dr = -(t[k + 1] - t[k]) factor (logD = -r t)
+        p.prepaid_dr[k] = p.prepaid_dr[k + 1] * factor - p.prepaid[k + 1] * factor * (times[k + 1] - times[k]);
+    }
+    double origin = 0.0, origin_dr = 0.0;
+    for (size_t e = 0; e < cash_a.size(); ++e) {
+        const double pv = cash_a[e] * std::exp(p.logd[cash_node[e]]);
+        origin += pv;
+        origin_dr += -times[cash_node[e]] * pv;
+        if (!std::isfinite(origin))
+            return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0, "row %zu escrowed prepaid cash is not finite at node 0",
+                        group_row);
+    }
+    p.prepaid[0] = origin;
+    p.prepaid_dr[0] = origin_dr;
+    p.prepaid_dr[expiry] = 0.0;
+    const double F0 = S0 - origin;
+    if (!std::isfinite(F0) || F0 <= 0.0)
+        return fail(err, FOHIP_E_ESCROWED_PREPAID_FORWARD, 0,
+                    "escrowed prepaid forward must be strictly positive at batch row %zu; initial spot is %.17g and "
+                    "origin prepaid cash is %.17g",
+                    group_row, S0, origin);
+    p.start = F0;
+    p.sigma = convention == FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN ? sigma * S0 / F0 : sigma;
+    p.chain[0] = convention == FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN ? -sigma * origin / (F0 * F0) : 0.0;
+    p.chain[1] = convention == FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN ? S0 / F0 : 1.0;
+    p.chain[2] = -origin_dr;
+    p.chain[3] = convention == FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN ? sigma * S0 * origin_dr / (F0 * F0) : 0.0;
+    return FOHIP_OK;
+}
+
 static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<GroupPlan> &plans,
                                std::vector<double> &strikes, fohip_error *err) {
     const size_t groups = rq->groups;
@@ -1191,16 +1312,22 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         if (!std::isfinite(rq->strikes[r]) || rq->strikes[r] <= 0.0)
             return fail(err, FOHIP_E_INVALID_ROW_VALUE, 0, "strikes[%zu] must be finite and strictly positive; observed %.17g",
                         r, rq->strikes[r]);
-    if (rq->dividend_model != 0 && rq->dividend_model != 1)
+    if (rq->dividend_model < FOHIP_DIVIDENDS_CONTINUOUS_YIELD || rq->dividend_model > FOHIP_DIVIDENDS_ESCROWED_CASH)
         return fail(err, FOHIP_E_UNSUPPORTED, 0, "unsupported dividend model %d", rq->dividend_model);
-    if (rq->dividend_model == 1) {
+    const bool has_cash_schedule = rq->dividend_model != FOHIP_DIVIDENDS_CONTINUOUS_YIELD;
+    const bool escrowed = rq->dividend_model == FOHIP_DIVIDENDS_ESCROWED_CASH;
+    if (has_cash_schedule) {
         if (!rq->cash_offsets) return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "cash-dividend offsets are missing");
         int rc = fohip_validate_ragged_offsets(rq->cash_offsets, groups + 1, rq->cash_events, err);
         if (rc) return rc;
         if (rq->maturity_boundary != 0 && rq->maturity_boundary != 1)
             return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "unsupported cash maturity boundary");
-        if (rq->cash_policy != 0 && rq->cash_policy != 1)
+        if (!escrowed && rq->cash_policy != 0 && rq->cash_policy != 1)
             return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown dividend policy %d", rq->cash_policy);
+        if (escrowed && rq->escrowed_volatility != FOHIP_ESCROWED_PREPAID_FORWARD &&
+            rq->escrowed_volatility != FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN)
+            return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown escrowed volatility convention %d",
+                        rq->escrowed_volatility);
     }
     for (size_t g = 0; g < groups; ++g) {
         const double S0 = rq->initial_spots[g], T = rq->maturities[g], r = rq->rates[g], v = rq->volatilities[g];
@@ -1239,6 +1366,12 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
                 int rc = fohip_validate_cash_dividend_amount(rq->cash_amounts[e], err);
                 if (rc) return rc;
             }
+            if (escrowed) {
+                const int64_t first = rq->cash_offsets[g] - cb, count = rq->cash_offsets[g + 1] - cb - first;
+                int rc = validate_escrowed_group(g, S0, r, v, rq->cash_times + first, rq->cash_amounts + first,
+                                                 static_cast<size_t>(count), rq->escrowed_volatility, nullptr, err);
+                if (rc) return rc;
+            }
         }
     }
     // method-level checks (lib.rs:1302-1400, input.rs:201-238)
@@ -1270,7 +1403,7 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
     }
     if (rq->mesh_kind == FOHIP_MESH_UNIFORM && rq->base_intervals == 0)
         return fail(err, FOHIP_E_EXERCISE_MESH, 0, "uniform exercise mesh needs at least one base interval");
-    if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES && rq->dividend_model != 1)
+    if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES && !has_cash_schedule)
         return fail(err, FOHIP_E_EXERCISE_MESH, 0, "CashDividendExerciseMesh requires cash-dividend inputs");
     if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
         if (!rq->exercise_offsets) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "explicit exercise offsets are missing");
@@ -1284,7 +1417,7 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         GroupPlan &p = plans[g];
         const double T = rq->maturities[g], rate = rq->rates[g];
         std::vector<double> cash_t, cash_a;
-        if (rq->dividend_model == 1) {
+        if (has_cash_schedule) {
             const int64_t cb = rq->cash_offsets[0];
             cash_t.assign(rq->cash_times + (rq->cash_offsets[g] - cb), rq->cash_times + (rq->cash_offsets[g + 1] - cb));
             cash_a.assign(rq->cash_amounts + (rq->cash_offsets[g] - cb), rq->cash_amounts + (rq->cash_offsets[g + 1] - cb));
@@ -1381,7 +1514,7 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
             if (times.size() < 2) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu mesh has no interval", group_row);
         }
         // build_row_curves (pricing.rs:862-891)
-        const double yield = rq->dividend_model == 0 ? rq->dividend_yields[g] : 0.0;
+        const double yield = has_cash_schedule ? 0.0 : rq->dividend_yields[g];
         p.logd.resize(times.size());
         p.q.resize(times.size());
         for (size_t i = 0; i < times.size(); ++i) {
@@ -1399,11 +1532,19 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         // build_row_schedule (pricing.rs:893-965)
         p.cash.assign(times.size(), 0.0);
         p.has_cash.assign(times.size(), 0);
+        std::vector<uint32_t> cash_node(cash_t.size());
         for (size_t e = 0; e < cash_t.size(); ++e) {
             const int64_t idx = search_time(times, cash_t[e]);
             if (idx < 0) return fail(err, FOHIP_E_CASH_SCHEDULE, 0, "row %zu cash event %zu has no grid node", group_row, e);
+            cash_node[e] = static_cast<uint32_t>(idx);
+            if (escrowed) continue;   // the risky prepaid forward carries no cash events
             p.cash[idx] = cash_a[e];
             p.has_cash[idx] = 1;
+        }
+        if (escrowed) {
+            int rc = escrowed_plan(p, times, cash_node, cash_a, rq->initial_spots[g], rq->volatilities[g],
+                                   rq->escrowed_volatility, group_row, err);
+            if (rc) return rc;
         }
         p.obs_nodes.clear();
         if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
@@ -1420,8 +1561,8 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         }
         p.obs_nodes.push_back(static_cast<uint32_t>(times.size() - 1));
         p.S0 = rq->initial_spots[g];
-        p.sigma = rq->volatilities[g];
-        p.policy = rq->dividend_model == 1 ? rq->cash_policy : 0;
+        if (!escrowed) p.sigma = rq->volatilities[g];   // escrowed: sigma_F, set by escrowed_plan
+        p.policy = rq->dividend_model == FOHIP_DIVIDENDS_CASH ? rq->cash_policy : 0;
         p.strike_begin = rq->group_offsets[g];
         p.strike_end = rq->group_offsets[g + 1];
         p.nt = nt_of(g);
@@ -1562,11 +1703,13 @@ extern "C" int fohip_grouped_diagnostics(fohip_executor *ex, const fohip_grouped
 
 // ======================================================================== path export/probes
 
-extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_plan *pl, double S0, uint64_t paths,
-                                         uint64_t seed, uint32_t domain, double *out, fohip_error *err) {
-    clear_error(err);
-    if (!ex || !pl || !out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
-    std::lock_guard<std::mutex> lock(ex->mu);
+// GbmPathPlan::simulate_seeded (crates/finoptions-sim/src/executor.rs:34, seeded.rs:38-77) of one
+// plan into `device_out` (caller device memory, binary32 when f32) or, when it is null, into the
+// executor's path buffer copied to `host_out` as binary64. Work queued on `after` (a caller stream)
+// completes first; the call returns once the states are complete.
+static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double S0, uint64_t paths, uint64_t seed,
+                           uint32_t domain, bool f32, void *device_out, hipStream_t after, double *host_out,
+                           fohip_error *err) {
     int rc = fohip_validate_event_schedule(pl->times, pl->nodes, pl->dividend_nodes, pl->dividend_amounts, pl->dividends,
                                            pl->exercise_nodes, pl->exercise_count, pl->expiry_node, err);
     if (!rc) rc = fohip_validate_log_discount_curve(pl->times, pl->log_discounts, pl->nodes, err);
@@ -1606,8 +1749,23 @@ extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_pl
     int64_t bad_group, bad_node;
     build_tables(plans, none, false, gt, path_elems, value_elems, model_count, part_t, part_e, count_slots, err,
                  bad_group, bad_node);
-    const bool f32 = ex->precision == FOHIP_PRECISION_F32_BULK;
     const size_t real = f32 ? 4 : 8;
+    if (device_out) {
+        // the destination must be device memory of this executor's device holding the whole matrix
+        hipPointerAttribute_t attr;
+        void *base = nullptr;
+        size_t range = 0;
+        const bool ok = hipPointerGetAttributes(&attr, device_out) == hipSuccess && attr.type == hipMemoryTypeDevice &&
+                        attr.device == ex->device &&
+                        hipMemGetAddressRange(&base, &range, device_out) == hipSuccess &&
+                        static_cast<char *>(device_out) + path_elems * real <= static_cast<char *>(base) + range;
+        if (!ok) {
+            (void)hipGetLastError();
+            return fail(err, FOHIP_E_INVALID_ARGUMENT, 0,
+                        "device output must be device memory on device %d holding %llu elements", ex->device,
+                        (unsigned long long)path_elems);
+        }
+    }
     const size_t tb_bytes[] = {sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8, gt.log_drift32.size() * 4,
                                gt.diffusion32.size() * 4, gt.cash.size() * 8, gt.node_flags.size() * 4};
     const void *tb_src[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
@@ -1619,13 +1777,22 @@ extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_pl
         total += (tb_bytes[i] + 255) & ~size_t(255);
     }
     HIP_TRY(ex->tables.ensure(total));
-    HIP_TRY(ex->paths.ensure(path_elems * real));
+    if (!device_out) HIP_TRY(ex->paths.ensure(path_elems * real));
+    void *dst = device_out ? device_out : ex->paths.ptr;
     HIP_TRY(ex->fail_t.ensure(8));
     std::vector<uint2> items;
     for (uint32_t c = 0; c < gt.groups[0].chunks_t; ++c) items.push_back(make_uint2(0, c));
     HIP_TRY(ex->items_t.ensure(items.size() * sizeof(uint2)));
     char *tb = ex->tables.as<char>();
     hipStream_t st = ex->stream;
+    if (after && after != st) {
+        hipEvent_t ready;
+        HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(ready, after);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ready, 0);
+        (void)hipEventDestroy(ready);
+        HIP_TRY(e);
+    }
     for (int i = 0; i < 7; ++i) HIP_TRY(hipMemcpyAsync(tb + offs[i], tb_src[i], tb_bytes[i], hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(ex->fail_t.ptr, 0xFF, 8, st));
@@ -1638,17 +1805,19 @@ extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_pl
     t.diffusion32 = reinterpret_cast<const float *>(tb + offs[4]);
     t.cash = reinterpret_cast<const double *>(tb + offs[5]);
     t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[6]);
-    HIP_TRY(launch_simulate(st, f32, t, ex->items_t.as<uint2>(), static_cast<uint32_t>(items.size()), ex->paths.ptr, 0,
+    HIP_TRY(launch_simulate(st, f32, t, ex->items_t.as<uint2>(), static_cast<uint32_t>(items.size()), dst, 0,
                             ex->fail_t.as<unsigned long long>()));
     unsigned long long failed = 0;
     HIP_TRY(hipMemcpyAsync(&failed, ex->fail_t.ptr, 8, hipMemcpyDeviceToHost, st));
-    if (f32) {
-        std::vector<float> tmp(path_elems);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), ex->paths.ptr, path_elems * 4, hipMemcpyDeviceToHost, st));
+    if (device_out) {
         HIP_TRY(hipStreamSynchronize(st));
-        for (uint64_t i = 0; i < path_elems; ++i) out[i] = tmp[i];
+    } else if (f32) {
+        std::vector<float> tmp(path_elems);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), dst, path_elems * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < path_elems; ++i) host_out[i] = tmp[i];
     } else {
-        HIP_TRY(hipMemcpyAsync(out, ex->paths.ptr, path_elems * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(host_out, dst, path_elems * 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
     if (failed != kNoError) {
@@ -1657,6 +1826,27 @@ extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_pl
                     "GBM path transition at path %llu produced a non-finite spot", failed);
     }
     return FOHIP_OK;
+}
+
+extern "C" int fohip_simulate_gbm_seeded(fohip_executor *ex, const fohip_path_plan *pl, double S0, uint64_t paths,
+                                         uint64_t seed, uint32_t domain, double *out, fohip_error *err) {
+    clear_error(err);
+    if (!ex || !pl || !out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    return simulate_export(ex, pl, S0, paths, seed, domain, ex->precision == FOHIP_PRECISION_F32_BULK, nullptr,
+                           nullptr, out, err);
+}
+
+extern "C" int fohip_simulate_gbm_seeded_device(fohip_executor *ex, const fohip_path_plan *pl, double S0,
+                                                uint64_t paths, uint64_t seed, uint32_t domain, void *device_out,
+                                                int32_t dtype, void *stream, fohip_error *err) {
+    clear_error(err);
+    if (!ex || !pl || !device_out) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    if (dtype != FOHIP_DTYPE_F64 && dtype != FOHIP_DTYPE_F32)
+        return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "unknown output dtype %d", dtype);
+    std::lock_guard<std::mutex> lock(ex->mu);
+    return simulate_export(ex, pl, S0, paths, seed, domain, dtype == FOHIP_DTYPE_F32, device_out,
+                           static_cast<hipStream_t>(stream), nullptr, err);
 }
 
 extern "C" int fohip_philox_blocks(fohip_executor *ex, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,

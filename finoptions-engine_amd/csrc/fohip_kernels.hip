@@ -107,10 +107,10 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     PathStream s;
     s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p);
     Store *out = paths_out + g.path_base + p;
-    double spot = g.S0;
+    double spot = g.start;
     uint32_t row = 0;
     if (g.origin_row) {
-        out[0] = static_cast<Store>(spot);
+        out[0] = static_cast<Store>(g.S0);
         row = 1;
     }
     bool ok = true;
@@ -134,7 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
         const bool terminal = flags & kNodeTerminal;
         if (cash && terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
         if (flags & kNodeObserved) {
-            out[static_cast<uint64_t>(row) * n] = static_cast<Store>(spot);
+            // escrowed groups observe F + A(node) (path.rs:844-880); A is 0 at expiry
+            const double observed = g.escrowed ? spot + t.row_escrow[g.row_base + row] : spot;
+            out[static_cast<uint64_t>(row) * n] = static_cast<Store>(observed);
             row += 1;
         }
         if (cash && !terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
@@ -675,47 +677,72 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
 #pragma unroll
         for (int q = 0; q < 14; ++q) acc[q] = 0.0;
         const FOHIP_UNIFORM Model *mn = uniform(models + model_row + local * obs);
-        const bool apply = step > 0 && mn->kind == kRuleRegression;
+        const bool apply = step > 0 && mn->kind == kRuleRegression;   // wave-uniform
         const double pivot = apply ? mn->mean : 1.0;
         const double K = *uniform(t.strikes + srow);
-        const double invK = 1.0 / K;
+        const double invK = *uniform(t.inv_strikes + srow);
         if (alive & (1u << i)) {
             Store *V = values + value_base + static_cast<uint64_t>(local) * n;
+            // Straight-line slots: lanes past n hold zero spots, which are never in the money and
+            // never exercise, so only the stores need the bound.
+            double v[kBulkPathsPerThread];
+            bool bad[kBulkPathsPerThread];
+            if (step == 0) {
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const uint32_t p = base + k * kBlock;
+                    const double s1 = s1v[k];
+                    const Store vf = static_cast<Store>((s1 > K ? s1 - K : 0.0) * d_next);
+                    if (p < n) V[p] = vf;
+                    v[k] = vf;
+                    bad[k] = false;
+                }
+            } else if (apply) {
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const uint32_t p = base + k * kBlock;
+                    const double s1 = s1v[k];
+                    const double cont = continuation_fast(mn, s1);
+                    const bool cand = s1 > K;
+                    const bool finite = isfinite(cont);
+                    const bool ex = cand && finite && should_exercise(fp.tie, s1 - K, cont);
+                    const Store vf = static_cast<Store>((s1 - K) * d_next);
+                    if (ex) V[p] = vf;
+                    v[k] = ex ? static_cast<double>(vf) : static_cast<double>(vv[k]);
+                    bad[k] = cand && !finite;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    v[k] = vv[k];
+                    bad[k] = false;
+                }
+            }
+            // failures are rare: one wave-uniform test for all slots
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) any = any || bad[k] || !isfinite(v[k]);
+            if (__ballot(any) != 0ull) {
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const uint32_t p = base + k * kBlock;
+                    if (bad[k]) record_error(&errors[srow], error_key(step - 1, kStageApply, p));
+                    if (p < n && !isfinite(v[k])) record_error(&errors[srow], error_key(step, kStageDiscount, p));
+                }
+            }
+            // pivoted power sums over in-the-money paths: an out-of-the-money lane adds exact zeros
 #pragma unroll
             for (int k = 0; k < kBulkPathsPerThread; ++k) {
-                const uint32_t p = base + k * kBlock;
-                if (p >= n) break;
-                const double s1 = s1v[k];
-                double v;
-                if (step == 0) {
-                    const Store vf = static_cast<Store>((s1 > K ? s1 - K : 0.0) * d_next);
-                    V[p] = vf;
-                    v = vf;
-                } else {
-                    v = vv[k];
-                    if (apply && s1 > K) {
-                        const double cont = continuation_fast(mn, s1);
-                        if (!isfinite(cont)) {
-                            record_error(&errors[srow], error_key(step - 1, kStageApply, p));
-                        } else if (should_exercise(fp.tie, s1 - K, cont)) {
-                            const Store vf = static_cast<Store>((s1 - K) * d_next);
-                            V[p] = vf;
-                            v = vf;
-                        }
-                    }
-                }
-                if (!isfinite(v)) record_error(&errors[srow], error_key(step, kStageDiscount, p));
                 const double s0 = s0v[k];
-                if (s0 > K) {
-                    const double u = fma(s0, invK, -pivot);
-                    const double y = v * inv_d;
-                    double pw = 1.0;
+                const bool itm = s0 > K;
+                const double u = fma(s0, invK, -pivot);
+                const double y = itm ? v[k] * inv_d : 0.0;
+                double pw = itm ? 1.0 : 0.0;
 #pragma unroll
-                    for (int j = 0; j <= 8; ++j) {
-                        acc[j] += pw;
-                        if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
-                        pw *= u;
-                    }
+                for (int j = 0; j <= 8; ++j) {
+                    acc[j] += pw;
+                    if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
+                    pw *= u;
                 }
             }
         }
@@ -888,7 +915,6 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
     const bool stored_at_origin = kBulk && g.fit_rows > 0;
     const double disc = stored_at_origin ? 1.0 : g.disc_final;
     const double d_last = stored_at_origin ? t.disc0[g.row_base + last_row] : 1.0;
-    const double invK = 1.0 / K;
     double sum[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
@@ -951,7 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
 // with its result discarded), so rules and tables are read with scalar loads.
 // With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along the transitions and each
 // strike's stopping row is held fixed (pathwise estimator under the frozen policy).
-template <bool FAST, int TS, bool GREEKS>
+template <bool FAST, int TS, bool GREEKS, bool ESCROW>
 __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
@@ -1007,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
     }
     PathStream s;
     s.init(g->eval_seed, g->eval_domain, p);
-    double spot = g->S0;
+    double spot = g->start;   // S0, or the escrowed prepaid forward F0
     double tan[4] = {1.0, 0.0, 0.0, 0.0};
     uint32_t row = origin_row;   // the origin row holds no regression rule
     for (uint32_t iv = 0; iv < intervals; ++iv) {
@@ -1044,18 +1070,20 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
         if ((flags & kNodeObserved) && row < last) {
             const double disc = *uniform(t.disc0 + row_base + row);
             const double t_row = GREEKS ? *uniform(t.row_time + row_base + row) : 0.0;
+            // the exercise decision observes the stock: F + A(node) for escrowed groups
+            const double S = ESCROW ? spot + *uniform(t.row_escrow + row_base + row) : spot;
 #pragma unroll
             for (int i = 0; i < TS; ++i) {
                 if (i >= ns) break;
                 const FOHIP_UNIFORM Model *m = uniform(tile_models + i * obs + row);
                 const int kind = m->kind;   // wave-uniform
                 const double K = m->K;      // set on every fitted row
-                const bool cand = ((active >> i) & 1u) && spot > K;
+                const bool cand = ((active >> i) & 1u) && S > K;
                 // branch-free: the continuation is formed for every row and used only where a
                 // regression rule exists
                 const bool reg = kind == kRuleRegression;
-                const double intr = spot - K;
-                const double cont = FAST ? continuation_fast(m, spot) : predict_ref(m, degree, spot / K);
+                const double intr = S - K;
+                const double cont = FAST ? continuation_fast(m, S) : predict_ref(m, degree, S / K);
                 const bool finite = isfinite(cont);
                 const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
                 const bool bad = reg && cand && !finite;
@@ -1065,10 +1093,12 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
                     const double pv = intr * disc;
                     pay[i] = ex ? pv : pay[i];
                     if (GREEKS) {
-                        dpay[i][0] = ex ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
-                        dpay[i][1] = ex ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
-                        dpay[i][2] = ex ? static_cast<Tan>(tan[2] * disc - t_row * pv) : dpay[i][2];
-                        dpay[i][3] = ex ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
+                        double o[4] = {tan[0], tan[1], tan[2], tan[3]};
+                        if (ESCROW) observed_tangents(tan, uniform(g->chain), *uniform(t.row_escrow_dr + row_base + row), o);
+                        dpay[i][0] = ex ? static_cast<Tan>(o[0] * disc) : dpay[i][0];
+                        dpay[i][1] = ex ? static_cast<Tan>(o[1] * disc) : dpay[i][1];
+                        dpay[i][2] = ex ? static_cast<Tan>(o[2] * disc - t_row * pv) : dpay[i][2];
+                        dpay[i][3] = ex ? static_cast<Tan>(o[3] * disc) : dpay[i][3];
                     }
                     active &= ~(((ex || bad) ? 1u : 0u) << i);
                     if (bad)
@@ -1101,6 +1131,8 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
     {
         const double disc = *uniform(t.disc0 + row_base + last);
         const double t_last = *uniform(t.row_time + row_base + last);
+        double o[4] = {tan[0], tan[1], tan[2], tan[3]};   // expiry observes F_T itself (no prepaid cash)
+        if (GREEKS && ESCROW) observed_tangents(tan, uniform(g->chain), 0.0, o);
 #pragma unroll
         for (int i = 0; i < TS; ++i) {
             if (i >= ns) break;
@@ -1120,10 +1152,10 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
             }
             if (GREEKS) {
                 const bool itm = hold && spot > K;
-                dpay[i][0] = itm ? static_cast<Tan>(tan[0] * disc) : dpay[i][0];
-                dpay[i][1] = itm ? static_cast<Tan>(tan[1] * disc) : dpay[i][1];
-                dpay[i][2] = itm ? static_cast<Tan>(tan[2] * disc - t_last * pv) : dpay[i][2];
-                dpay[i][3] = itm ? static_cast<Tan>(tan[3] * disc) : dpay[i][3];
+                dpay[i][0] = itm ? static_cast<Tan>(o[0] * disc) : dpay[i][0];
+                dpay[i][1] = itm ? static_cast<Tan>(o[1] * disc) : dpay[i][1];
+                dpay[i][2] = itm ? static_cast<Tan>(o[2] * disc - t_last * pv) : dpay[i][2];
+                dpay[i][3] = itm ? static_cast<Tan>(o[3] * disc) : dpay[i][3];
             }
         }
     }
@@ -1267,7 +1299,7 @@ __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint3
     const DevGroup &g = t.groups[group];
     PathStream s;
     s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, path);
-    double spot = g.S0;
+    double spot = g.start;
     out[0] = -1.0;
     for (uint32_t i = 0; i < g.intervals; ++i) {
         double z;
@@ -1432,7 +1464,7 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     return tile;
 }
 
-template <bool FAST, int TS, bool G>
+template <bool FAST, int TS, bool G, bool E>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
@@ -1440,21 +1472,23 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    hipLaunchKernelGGL((k_evaluate<FAST, TS, G>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
                        errors, sim_fail, use_lds);
     return hipGetLastError();
 }
 
 template <bool FAST>
-static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, const DevTables &t, const FitParams &fp,
+static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
                                 RowError *errors, unsigned long long *sim_fail) {
 #define FOHIP_EVAL_CASE(TSV, GV)                                                                                   \
     if (tile == TSV && greeks == GV)                                                                               \
-        return launch_eval_t<FAST, TSV, GV>(st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, \
-                                            nosupport, errors, sim_fail);
+        return escrow ? launch_eval_t<FAST, TSV, GV, true>(st, t, fp, items, nitems, max_strikes, max_obs, models,  \
+                                                           partials, counts, nosupport, errors, sim_fail)          \
+                      : launch_eval_t<FAST, TSV, GV, false>(st, t, fp, items, nitems, max_strikes, max_obs, models, \
+                                                            partials, counts, nosupport, errors, sim_fail);
     FOHIP_EVAL_CASE(1, false)
     FOHIP_EVAL_CASE(4, false)
     FOHIP_EVAL_CASE(16, false)
@@ -1465,16 +1499,16 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, uint32_t tile, cons
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, uint32_t tile, const DevTables &t,
+hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile, const DevTables &t,
                            const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                            uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
     if (nitems == 0) return hipSuccess;
     if (fast)
-        return launch_eval_r<true>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
-                                    counts, nosupport, errors, sim_fail);
-    return launch_eval_r<false>(st, greeks, tile, t, fp, items, nitems, max_strikes, max_obs, models, partials,
-                                 counts, nosupport, errors, sim_fail);
+        return launch_eval_r<true>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
+                                    partials, counts, nosupport, errors, sim_fail);
+    return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
+                                 partials, counts, nosupport, errors, sim_fail);
 }
 
 hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models, const uint32_t *counts,

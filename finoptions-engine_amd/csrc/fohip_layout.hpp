@@ -76,6 +76,15 @@ struct DevGroup {
     uint32_t count_base;      // exercise-count slot of (strike_begin, row 0)
     int32_t policy;           // cash-dividend policy
     uint32_t chunks_b;        // fused fit chunks (kBulkChunk paths each)
+    // Escrowed cash dividends (path.rs:618-944): the simulated state is the risky prepaid
+    // forward F, started at start = S0 - A0, and an exercise row observes F + row_escrow[row].
+    // Spot models: escrowed = 0, start = S0.
+    int32_t escrowed;
+    uint32_t pad1;
+    double start;
+    // Greek chain factors of the observed stock (identity for the spot models): dsigma_F/dS0,
+    // dsigma_F/dsigma, dF0/dr, dsigma_F/dr.
+    double chain[4];
 };
 
 // Node flags
@@ -99,6 +108,9 @@ struct DevTables {
     const uint32_t *row_node;     // [row]
     const double *strikes;        // [strike row]
     const uint32_t *strike_group; // [strike row]
+    const double *row_escrow;     // [observation row] prepaid cash observed (escrowed groups)
+    const double *row_escrow_dr;  // [observation row] its derivative w.r.t. the rate
+    const double *inv_strikes;    // [strike row] 1 / K
 };
 
 // Per-strike-row failure record: the smallest key wins (earliest failure in processing order).

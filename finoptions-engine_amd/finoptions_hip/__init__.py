@@ -13,6 +13,8 @@ from .batch_lsm import (
     CashDividendBatch,
     CashDividendExerciseMesh,
     ContinuousDividendYieldBatch,
+    EscrowedCashDividendBatch,
+    EscrowedVolatilityConvention,
     ExerciseMesh,
     ExplicitExerciseMesh,
     LsmBatchResult,
@@ -49,11 +51,13 @@ from .model import (
 )
 from .simulation import (
     CpuExecutor,
+    DeviceGbmPaths,
     GbmPaths,
     HipExecutor,
     HipPrecision,
     RandomStream,
     simulate_gbm_seeded,
+    simulate_gbm_seeded_tensor,
 )
 
 __all__ = [
@@ -63,6 +67,8 @@ __all__ = [
     "CashDividendExerciseMesh",
     "ConstantVolatility",
     "ContinuousDividendYieldBatch",
+    "EscrowedCashDividendBatch",
+    "EscrowedVolatilityConvention",
     "CpuExecutor",
     "CumulativeDividendYieldCurve",
     "DividendPolicy",
@@ -72,6 +78,7 @@ __all__ = [
     "ExerciseMesh",
     "ExerciseTieRule",
     "ExplicitExerciseMesh",
+    "DeviceGbmPaths",
     "GbmPaths",
     "Greek",
     "HipExecutor",
@@ -102,4 +109,5 @@ __all__ = [
     "price_lsm_call_batch",
     "price_lsm_call_grouped_batch",
     "simulate_gbm_seeded",
+    "simulate_gbm_seeded_tensor",
 ]

@@ -7,6 +7,7 @@ The library is loaded from this package directory only. A missing or unloadable 
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
 from pathlib import Path
 
@@ -137,6 +138,7 @@ class FohipGroupedRequest(C.Structure):
         ("rank_tolerance", C.c_double), ("condition_limit", C.c_double),
         ("tie_rule", C.c_int32), ("insufficient_support", C.c_int32),
         ("greeks", C.c_uint32),
+        ("escrowed_volatility", C.c_int32),
     ]
 
 
@@ -171,7 +173,8 @@ class FohipPathPlan(C.Structure):
 EXPORTED = (
     "fohip_abi_version", "fohip_device_count", "fohip_executor_create", "fohip_executor_destroy",
     "fohip_executor_info", "fohip_set_tuning", "fohip_price_lsm_call", "fohip_price_lsm_grouped",
-    "fohip_grouped_diagnostics", "fohip_simulate_gbm_seeded", "fohip_philox_blocks",
+    "fohip_grouped_diagnostics", "fohip_simulate_gbm_seeded", "fohip_simulate_gbm_seeded_device",
+    "fohip_philox_blocks",
     "fohip_stream_normals", "fohip_validate_time_grid", "fohip_validate_log_discount_curve",
     "fohip_validate_cumulative_yield_curve", "fohip_validate_event_schedule",
     "fohip_validate_constant_volatility", "fohip_validate_cash_dividend_amount",
@@ -180,6 +183,26 @@ EXPORTED = (
 )
 
 _lib: C.CDLL | None = None
+
+
+def _share_torch_hip_runtime() -> None:
+    """Bind libfohip.so to the HIP runtime that torch ships, when torch is installed.
+
+    torch bundles its own libamdhip64 under the soname of the system runtime (libamdhip64.so.7).
+    Loaded first, libfohip.so would map the system copy, and torch would later map a second
+    runtime that finds no device. Opening torch's copy first makes the dynamic loader resolve this
+    library's dependency to it, so a process keeps one HIP runtime whichever package it imports
+    first. torch itself is not imported.
+    """
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    runtime = Path(spec.origin).parent / "lib" / "libamdhip64.so"
+    if runtime.exists():
+        try:
+            C.CDLL(str(runtime), mode=os.RTLD_NOW | os.RTLD_LOCAL)
+        except OSError:
+            pass   # the system runtime then serves this library alone
 
 
 def lib() -> C.CDLL:
@@ -191,6 +214,7 @@ def lib() -> C.CDLL:
         msg = (f"libfohip.so is not built at {LIB_PATH}; run finoptions-engine_amd/build_lib.py. "
                "There is no CPU fallback.")
         raise ExecutionError(msg)
+    _share_torch_hip_runtime()
     try:
         handle = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | os.RTLD_LOCAL)
     except OSError as exc:
@@ -210,6 +234,9 @@ def lib() -> C.CDLL:
                                                  C.c_size_t, _u32p, E]
     handle.fohip_simulate_gbm_seeded.argtypes = [C.c_void_p, C.POINTER(FohipPathPlan), C.c_double,
                                                  C.c_uint64, C.c_uint64, C.c_uint32, _dp, E]
+    handle.fohip_simulate_gbm_seeded_device.argtypes = [
+        C.c_void_p, C.POINTER(FohipPathPlan), C.c_double, C.c_uint64, C.c_uint64, C.c_uint32,
+        C.c_void_p, C.c_int32, C.c_void_p, E]
     handle.fohip_philox_blocks.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                            C.c_uint32, C.c_uint32, _u32p, E]
     handle.fohip_stream_normals.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,

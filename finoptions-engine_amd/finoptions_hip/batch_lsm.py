@@ -45,6 +45,13 @@ _MAX_U32: Final = (2**32) - 1
 _MIN_EVALUATION_PATHS: Final = 2
 
 
+class EscrowedVolatilityConvention(str, Enum):
+    """Meaning of the volatility vector for the escrowed prepaid-forward approximation."""
+
+    PREPAID_FORWARD = "prepaid_forward"
+    SPOT_MATCHED_AT_ORIGIN = "spot_matched_at_origin"
+
+
 class OriginExercise(str, Enum):
     """Whether valuation origin is an explicit common exercise decision."""
 
@@ -145,6 +152,31 @@ class CashDividendBatch:
 
 
 @dataclass(frozen=True, slots=True)
+class EscrowedCashDividendBatch:
+    """Risky prepaid forward plus the deterministic value of later cash dividends.
+
+    An escrowed-dividend approximation, distinct from the exact spot cash-jump model of
+    :class:`CashDividendBatch`: the simulated state is the prepaid forward F (no dividends), started
+    at F0 = S0 - sum_k D_k exp(-r t_k), and an exercise date observes F + the prepaid value of the
+    cash still to come (that date's amount included). ``volatility_convention`` states whether the
+    volatility column belongs to F already or is a spot volatility to be rescaled by S0 / F0
+    (python/finoptions/batch_lsm.py:171-195 of the reference).
+    """
+
+    schedules: RaggedCashDividends
+    volatility_convention: EscrowedVolatilityConvention
+    maturity_boundary: MaturityBoundary
+
+    def __post_init__(self) -> None:
+        if type(self.schedules) is not RaggedCashDividends:
+            raise InputError("schedules must be RaggedCashDividends")
+        if type(self.volatility_convention) is not EscrowedVolatilityConvention:
+            raise InputError("volatility_convention must be EscrowedVolatilityConvention")
+        if type(self.maturity_boundary) is not MaturityBoundary:
+            raise InputError("maturity_boundary must be MaturityBoundary")
+
+
+@dataclass(frozen=True, slots=True)
 class RaggedPathCounts:
     """Per-group (per-contract for a row batch) training and evaluation path counts.
 
@@ -166,7 +198,10 @@ class RaggedPathCounts:
             raise InputError("ragged batches require at least two evaluation paths per group")
 
 
-BatchDividends = Union[ContinuousDividendYieldBatch, CashDividendBatch]
+BatchDividends = Union[ContinuousDividendYieldBatch, CashDividendBatch, EscrowedCashDividendBatch]
+_DIVIDEND_TYPES: Final = (ContinuousDividendYieldBatch, CashDividendBatch, EscrowedCashDividendBatch)
+_DIVIDEND_TYPE_MESSAGE: Final = (
+    "dividends must be ContinuousDividendYieldBatch, CashDividendBatch, or EscrowedCashDividendBatch")
 ExerciseMesh = Union[UniformExerciseMesh, CashDividendExerciseMesh, ExplicitExerciseMesh]
 
 
@@ -188,8 +223,8 @@ class LsmStrikeGroups:
                              ("maturities", self.maturities), ("rates", self.rates),
                              ("volatilities", self.volatilities)):
             validate_vector(values, field=name, dtype="float64")
-        if type(self.dividends) not in (ContinuousDividendYieldBatch, CashDividendBatch):
-            raise InputError("dividends must be ContinuousDividendYieldBatch or CashDividendBatch")
+        if type(self.dividends) not in _DIVIDEND_TYPES:
+            raise InputError(_DIVIDEND_TYPE_MESSAGE)
 
 
 _COLUMNS: Final = (
@@ -254,6 +289,37 @@ def _mesh_fields(mesh: ExerciseMesh, groups: int) -> dict:
         "mesh must be UniformExerciseMesh, CashDividendExerciseMesh, or ExplicitExerciseMesh")
 
 
+def _dividend_fields(req: FohipGroupedRequest, dividends: BatchDividends, groups: int) -> None:
+    """Dividend model fields of the C request (the three reference variants,
+    python/finoptions/batch_lsm.py:596-627)."""
+    if type(dividends) is ContinuousDividendYieldBatch:
+        if dividends.annualized_decimals.size != groups:
+            raise InputError(
+                f"dividend_model has {dividends.annualized_decimals.size} rows; expected {groups}")
+        req.dividend_model = 0
+        req.dividend_yields = ptr(dividends.annualized_decimals, C.c_double)
+        return
+    if type(dividends) not in (CashDividendBatch, EscrowedCashDividendBatch):
+        raise InputError(_DIVIDEND_TYPE_MESSAGE)
+    sched = dividends.schedules
+    if sched.rows != groups:
+        raise InputError(f"dividend_model has {sched.rows} rows; expected {groups}")
+    req.cash_offsets = ptr(sched.offsets, C.c_int64)
+    req.cash_times = ptr(sched.times, C.c_double)
+    req.cash_amounts = ptr(sched.amounts, C.c_double)
+    req.cash_events = sched.times.size
+    req.maturity_boundary = (0 if dividends.maturity_boundary is MaturityBoundary.STRICTLY_BEFORE
+                             else 1)
+    if type(dividends) is CashDividendBatch:
+        req.dividend_model = 1
+        req.cash_policy = 0 if dividends.policy is DividendPolicy.LIQUIDATOR else 1
+    else:
+        req.dividend_model = 2
+        req.escrowed_volatility = (
+            0 if dividends.volatility_convention is EscrowedVolatilityConvention.PREPAID_FORWARD
+            else 1)
+
+
 def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates, volatilities,
            dividends: BatchDividends, mesh: ExerciseMesh, executor: HipExecutor, method: Lsm,
            greeks: tuple[Greek, ...], *, what: str,
@@ -284,26 +350,7 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
     req.volatilities = ptr(volatilities, C.c_double)
     req.strikes = ptr(strikes, C.c_double)
     req.rows = rows
-    if type(dividends) is ContinuousDividendYieldBatch:
-        if dividends.annualized_decimals.size != groups:
-            raise InputError(
-                f"dividend_model has {dividends.annualized_decimals.size} rows; expected {groups}")
-        req.dividend_model = 0
-        req.dividend_yields = ptr(dividends.annualized_decimals, C.c_double)
-    elif type(dividends) is CashDividendBatch:
-        sched = dividends.schedules
-        if sched.rows != groups:
-            raise InputError(f"dividend_model has {sched.rows} rows; expected {groups}")
-        req.dividend_model = 1
-        req.cash_offsets = ptr(sched.offsets, C.c_int64)
-        req.cash_times = ptr(sched.times, C.c_double)
-        req.cash_amounts = ptr(sched.amounts, C.c_double)
-        req.cash_events = sched.times.size
-        req.cash_policy = 0 if dividends.policy is DividendPolicy.LIQUIDATOR else 1
-        req.maturity_boundary = (0 if dividends.maturity_boundary is MaturityBoundary.STRICTLY_BEFORE
-                                 else 1)
-    else:
-        raise InputError("dividends must be ContinuousDividendYieldBatch or CashDividendBatch")
+    _dividend_fields(req, dividends, groups)
     req.mesh_kind = mf["mesh_kind"]
     req.base_intervals = mf["base_intervals"]
     req.origin_exercise = mf["origin_exercise"]
@@ -471,18 +518,7 @@ def _request_for(groups: LsmStrikeGroups, mesh: ExerciseMesh, method: Lsm):
     req.volatilities = ptr(groups.volatilities, C.c_double)
     req.strikes = ptr(groups.strikes, C.c_double)
     req.rows = groups.strikes.size
-    d = groups.dividends
-    if type(d) is ContinuousDividendYieldBatch:
-        req.dividend_model = 0
-        req.dividend_yields = ptr(d.annualized_decimals, C.c_double)
-    else:
-        req.dividend_model = 1
-        req.cash_offsets = ptr(d.schedules.offsets, C.c_int64)
-        req.cash_times = ptr(d.schedules.times, C.c_double)
-        req.cash_amounts = ptr(d.schedules.amounts, C.c_double)
-        req.cash_events = d.schedules.times.size
-        req.cash_policy = 0 if d.policy is DividendPolicy.LIQUIDATOR else 1
-        req.maturity_boundary = 0 if d.maturity_boundary is MaturityBoundary.STRICTLY_BEFORE else 1
+    _dividend_fields(req, groups.dividends, int(groups.initial_spots.size))
     mf = _mesh_fields(mesh, int(groups.initial_spots.size))
     req.mesh_kind = mf["mesh_kind"]
     req.base_intervals = mf["base_intervals"]
@@ -511,6 +547,8 @@ __all__ = [
     "CashDividendBatch",
     "CashDividendExerciseMesh",
     "ContinuousDividendYieldBatch",
+    "EscrowedCashDividendBatch",
+    "EscrowedVolatilityConvention",
     "ExerciseMesh",
     "ExplicitExerciseMesh",
     "LsmBatchResult",

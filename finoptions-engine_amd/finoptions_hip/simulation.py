@@ -184,22 +184,10 @@ class GbmPaths:
         return self.states[self.observation_nodes.index(node)]
 
 
-def simulate_gbm_seeded(  # noqa: PLR0913 - financial inputs remain independently typed
-    *,
-    initial_spot: float,
-    schedule: EventSchedule,
-    curve: LogDiscountCurve,
-    dividend_yield_curve: CumulativeDividendYieldCurve,
-    volatility: ConstantVolatility,
-    dividend_policy: DividendPolicy,
-    paths: int,
-    random_stream: RandomStream,
-    executor: HipExecutor,
-) -> GbmPaths:
-    """Generate seeded GBM paths on the device and return the time-major observation matrix.
-
-    Mirrors the reference ``simulate_gbm_seeded`` (python/finoptions/simulation.py:262-332).
-    """
+def _path_plan(initial_spot, schedule, curve, dividend_yield_curve, volatility, dividend_policy,
+               paths, random_stream, executor):
+    """Checked inputs of ``simulate_gbm_seeded`` as a C path plan (python/finoptions/
+    simulation.py:262-332 of the reference); returns (plan, arrays kept alive, observation rows)."""
     if type(initial_spot) is not float:
         raise InputError("initial_spot must be a plain float")
     for name, value, cls in (("schedule", schedule, EventSchedule), ("curve", curve, LogDiscountCurve),
@@ -226,7 +214,27 @@ def simulate_gbm_seeded(  # noqa: PLR0913 - financial inputs remain independentl
                          len(schedule.dividends), ptr(en, C.c_uint32), len(schedule.exercise_nodes),
                          schedule.expiry_node, volatility.annualized_decimal,
                          0 if dividend_policy is DividendPolicy.LIQUIDATOR else 1)
-    obs = len(schedule.exercise_nodes) + 1
+    return plan, (times, logd, q, dn, da, en), len(schedule.exercise_nodes) + 1
+
+
+def simulate_gbm_seeded(  # noqa: PLR0913 - financial inputs remain independently typed
+    *,
+    initial_spot: float,
+    schedule: EventSchedule,
+    curve: LogDiscountCurve,
+    dividend_yield_curve: CumulativeDividendYieldCurve,
+    volatility: ConstantVolatility,
+    dividend_policy: DividendPolicy,
+    paths: int,
+    random_stream: RandomStream,
+    executor: HipExecutor,
+) -> GbmPaths:
+    """Generate seeded GBM paths on the device and return the time-major observation matrix.
+
+    Mirrors the reference ``simulate_gbm_seeded`` (python/finoptions/simulation.py:262-332).
+    """
+    plan, _keep, obs = _path_plan(initial_spot, schedule, curve, dividend_yield_curve, volatility,
+                                  dividend_policy, paths, random_stream, executor)
     out = np.empty((obs, paths), dtype=np.float64)
     err = FohipError()
     code = lib().fohip_simulate_gbm_seeded(executor._handle.ptr, C.byref(plan), initial_spot,  # noqa: SLF001
@@ -236,6 +244,59 @@ def simulate_gbm_seeded(  # noqa: PLR0913 - financial inputs remain independentl
     out.flags.writeable = False
     return GbmPaths(observation_nodes=tuple(schedule.exercise_nodes) + (schedule.expiry_node,),
                     states=out)
+
+
+@dataclass(frozen=True, slots=True)
+class DeviceGbmPaths:
+    """Time-major pre-event states left in device memory: ``states`` is a torch tensor of shape
+    [observation_nodes, paths] on the executor's device."""
+
+    observation_nodes: tuple[int, ...]
+    states: object
+
+    @property
+    def path_count(self) -> int:
+        return int(self.states.shape[1])
+
+    def values_at(self, node: int):
+        if type(node) is not int:
+            raise InputError("node must be a plain int")
+        return self.states[self.observation_nodes.index(node)]
+
+
+def simulate_gbm_seeded_tensor(  # noqa: PLR0913 - financial inputs remain independently typed
+    *,
+    initial_spot: float,
+    schedule: EventSchedule,
+    curve: LogDiscountCurve,
+    dividend_yield_curve: CumulativeDividendYieldCurve,
+    volatility: ConstantVolatility,
+    dividend_policy: DividendPolicy,
+    paths: int,
+    random_stream: RandomStream,
+    executor: HipExecutor,
+    dtype: str = "float64",
+) -> DeviceGbmPaths:
+    """``simulate_gbm_seeded`` with the observation matrix written straight into a torch tensor on
+    the executor's device (no host copy), ordered after the work already queued on torch's current
+    stream. ``dtype`` "float32" rounds each binary64 state once."""
+    import torch
+
+    if dtype not in ("float64", "float32"):
+        raise InputError(f"dtype must be 'float64' or 'float32'; observed {dtype!r}")
+    plan, _keep, obs = _path_plan(initial_spot, schedule, curve, dividend_yield_curve, volatility,
+                                  dividend_policy, paths, random_stream, executor)
+    device = torch.device("cuda", executor.device)
+    states = torch.empty((obs, paths), dtype=getattr(torch, dtype), device=device)
+    stream = torch.cuda.current_stream(device)
+    err = FohipError()
+    code = lib().fohip_simulate_gbm_seeded_device(
+        executor._handle.ptr, C.byref(plan), initial_spot, paths, random_stream.seed,  # noqa: SLF001
+        random_stream.domain, C.c_void_p(states.data_ptr()), 0 if dtype == "float64" else 1,
+        C.c_void_p(stream.cuda_stream), C.byref(err))
+    raise_for(code, err, lsm=False)
+    return DeviceGbmPaths(observation_nodes=tuple(schedule.exercise_nodes) + (schedule.expiry_node,),
+                          states=states)
 
 
 def philox_blocks(executor: HipExecutor, *, seed: int, domain: int, path0: int, paths: int,
@@ -263,6 +324,7 @@ def stream_normals(executor: HipExecutor, *, seed: int, domain: int, path0: int,
 
 __all__ = [
     "CpuExecutor",
+    "DeviceGbmPaths",
     "ExecutionError",
     "GbmPaths",
     "HipExecutor",
@@ -271,5 +333,6 @@ __all__ = [
     "SimulationError",
     "philox_blocks",
     "simulate_gbm_seeded",
+    "simulate_gbm_seeded_tensor",
     "stream_normals",
 ]

@@ -14,10 +14,14 @@
  *                                  (crates/finoptions-python/src/lib.rs:1170-1248) ->
  *                                  LsmCallEstimator::price_seeded (crates/finoptions-lsm/src/policy.rs:306)
  *   fohip_price_lsm_grouped        _native.price_lsm_call_grouped_batch_{continuous_yield,
- *                                  cash_dividends}_f64 (crates/finoptions-python/src/lib.rs:1885-2061)
- *                                  -> GroupedBatchLsmCallInputs::price (batch/pricing.rs:422)
+ *                                  cash_dividends,escrowed_cash_dividends}_f64
+ *                                  (crates/finoptions-python/src/lib.rs:1885-2061) and the row
+ *                                  batches _native.price_lsm_call_batch_*_f64 ->
+ *                                  GroupedBatchLsmCallInputs::price (batch/pricing.rs:422),
+ *                                  BatchLsmCallInputs::price (batch/pricing.rs:191)
  *   fohip_simulate_gbm_seeded      _native.simulate_gbm_seeded_f64 (lib.rs:1020-1060) ->
  *                                  GbmPathPlan::simulate_seeded_with (crates/finoptions-sim/src/executor.rs:34)
+ *   fohip_simulate_gbm_seeded_device  the same export into caller device memory (no host copy)
  *   fohip_philox_blocks            philox4x32_10 (crates/finoptions-sim/src/random/philox.rs:32) over
  *                                  PhiloxPathStream counters (philox.rs:64-86); parity probe
  *   fohip_stream_normals           sample_standard_normal (crates/finoptions-sim/src/random/normal.rs:96);
@@ -86,6 +90,8 @@ enum {
     FOHIP_E_CASH_SCHEDULE = 505,
     FOHIP_E_EXERCISE_MESH = 506,
     FOHIP_E_INSUFFICIENT_EVALUATION_PATHS = 507,
+    FOHIP_E_ESCROWED_PREPAID_FORWARD = 508,   /* nonpositive F0, non-finite prepaid cash, or an
+                                                 invalid spot-matched volatility */
     /* 9xx: execution (reference ExecutionError) */
     FOHIP_E_NO_DEVICE = 900,
     FOHIP_E_HIP = 901,
@@ -227,7 +233,10 @@ int fohip_price_lsm_call(fohip_executor *executor, const fohip_call_request *req
                          fohip_call_result *result, fohip_error *err);
 
 /* ---- group-addressed strikes: price_lsm_call_grouped_batch ---- */
-enum { FOHIP_DIVIDENDS_CONTINUOUS_YIELD = 0, FOHIP_DIVIDENDS_CASH = 1 };
+enum { FOHIP_DIVIDENDS_CONTINUOUS_YIELD = 0, FOHIP_DIVIDENDS_CASH = 1, FOHIP_DIVIDENDS_ESCROWED_CASH = 2 };
+/* escrowed cash dividends (EscrowedVolatilityConvention, batch_lsm.py:52-56): the volatility
+ * column is the prepaid forward's, or a spot volatility rescaled by S0 / F0 */
+enum { FOHIP_ESCROWED_PREPAID_FORWARD = 0, FOHIP_ESCROWED_SPOT_MATCHED_AT_ORIGIN = 1 };
 enum { FOHIP_MESH_UNIFORM = 0, FOHIP_MESH_CASH_DIVIDEND_DATES = 1, FOHIP_MESH_EXPLICIT = 2 };
 enum { FOHIP_ORIGIN_EXCLUDED = 0, FOHIP_ORIGIN_INCLUDED = 1 };
 enum { FOHIP_BOUNDARY_STRICTLY_BEFORE = 0, FOHIP_BOUNDARY_AT_OR_BEFORE = 1 };
@@ -243,11 +252,11 @@ typedef struct {
     size_t rows;
     int32_t dividend_model;             /* FOHIP_DIVIDENDS_* */
     const double *dividend_yields;      /* [groups] (continuous yield) */
-    const int64_t *cash_offsets;        /* [groups + 1] Arrow-style offsets (cash) */
+    const int64_t *cash_offsets;        /* [groups + 1] Arrow-style offsets (cash and escrowed) */
     const double *cash_times;
     const double *cash_amounts;
     size_t cash_events;                 /* length of cash_times / cash_amounts */
-    int32_t cash_policy;                /* FOHIP_DIVIDEND_* */
+    int32_t cash_policy;                /* FOHIP_DIVIDEND_* (cash model) */
     int32_t maturity_boundary;          /* FOHIP_BOUNDARY_* */
     int32_t mesh_kind;                  /* FOHIP_MESH_* */
     uint32_t base_intervals;            /* uniform mesh */
@@ -267,6 +276,7 @@ typedef struct {
     int32_t tie_rule;
     int32_t insufficient_support;
     uint32_t greeks;                    /* FOHIP_GREEK_* mask */
+    int32_t escrowed_volatility;        /* FOHIP_ESCROWED_* (escrowed model) */
 } fohip_grouped_request;
 
 /* The 12 row-ordered columns of NativeBatchLsmResult (lib.rs:1417-1430) plus optional Greek
@@ -315,6 +325,18 @@ typedef struct {
 int fohip_simulate_gbm_seeded(fohip_executor *executor, const fohip_path_plan *plan,
                               double initial_spot, uint64_t paths, uint64_t seed, uint32_t domain,
                               double *out, fohip_error *err);
+
+/* The same time-major states written straight into caller device memory on the executor's device
+ * (simulate_gbm_seeded, python/finoptions/simulation.py:262-332, with the observation matrix left
+ * in HBM for torch/DLPack consumers instead of copied to a host array). `device_out` holds
+ * (exercise_count + 1) * paths elements of `dtype` (binary32 rounds each binary64 state once).
+ * The simulation is ordered after the work already queued on `stream` (a hipStream_t, or null)
+ * and the call returns once the states are complete. */
+enum { FOHIP_DTYPE_F64 = 0, FOHIP_DTYPE_F32 = 1 };
+int fohip_simulate_gbm_seeded_device(fohip_executor *executor, const fohip_path_plan *plan,
+                                     double initial_spot, uint64_t paths, uint64_t seed,
+                                     uint32_t domain, void *device_out, int32_t dtype, void *stream,
+                                     fohip_error *err);
 
 /* Raw Philox4x32-10 output words: out[(p * blocks + b) * 4 + w] for path path0 + p, block b. */
 int fohip_philox_blocks(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,

@@ -53,6 +53,7 @@ enum {
     E_OUTSIDE_TRAINING_SUPPORT = 402,
     E_UNREPRESENTABLE_DISCOUNT = 403,
     E_PATH_COUNT_OUTSIDE_DOMAIN = 404,
+    E_ESCROWED_PREPAID_FORWARD = 508,
     E_INTERNAL = 904,
 };
 
@@ -1126,7 +1127,7 @@ int or_group_grid(const or_grouped_input *in, uint32_t g, double *times, uint32_
                   double *div_amounts, uint32_t *div_count) {
     const double *cash = NULL, *amounts = NULL;
     uint32_t ncash = 0;
-    if (in->dividend_model == 1) {
+    if (in->dividend_model != 0) {   /* cash and escrowed models share the event-aligned grid */
         int64_t a = in->cash_offsets[g] - in->cash_offsets[0], b = in->cash_offsets[g + 1] - in->cash_offsets[0];
         cash = in->cash_times + a;
         amounts = in->cash_amounts + a;
@@ -1171,11 +1172,51 @@ typedef struct {
     uint32_t *ex, *div_nodes;
     uint32_t nodes, nex, ndiv;
     or_plan_input in;
+    /* escrowed model (path.rs:618-944): the plan `in` is the risky prepaid forward (no cash
+     * events, zero yield); observations add prepaid[node] */
+    int escrowed;
+    double *prepaid;       /* [nodes] */
+    double start;          /* initial simulated state: S0, or F0 = S0 - prepaid[0] */
+    int prepaid_ok;        /* every prepaid value finite */
 } group_plan_t;
+
+/* path.rs:897-944 escrowed_prepaid_cash: prepaid[node] = prepaid[node + 1] exp(logD[node + 1] -
+ * logD[node]) + same-node cash, expiry holding a terminal amount; prepaid[0] is then replaced by
+ * the origin value sum_k D_k exp(logD[node_k]) in schedule order. Returns 0 when a value is not
+ * finite. */
+static int escrowed_prepaid_cash(uint32_t nodes, const double *logd, uint32_t ndiv,
+                                 const uint32_t *div_nodes, const double *amounts, double *prepaid) {
+    for (uint32_t i = 0; i < nodes; ++i) prepaid[i] = 0.0;
+    uint32_t event = ndiv;
+    const uint32_t expiry = nodes - 1;
+    if (event > 0 && div_nodes[event - 1] == expiry) {
+        event -= 1;
+        prepaid[expiry] = amounts[event];
+    }
+    for (int64_t node = (int64_t)nodes - 2; node >= 0; --node) {
+        double interval_log_discount = logd[node + 1] - logd[node];
+        double discounted_future = prepaid[node + 1] * exp(interval_log_discount);
+        double same_node_cash = 0.0;
+        if (event > 0 && div_nodes[event - 1] == (uint32_t)node) {
+            event -= 1;
+            same_node_cash = amounts[event];
+        }
+        double value = discounted_future + same_node_cash;
+        if (!isfinite(value)) return 0;
+        prepaid[node] = value;
+    }
+    double origin = 0.0;
+    for (uint32_t d = 0; d < ndiv; ++d) {
+        origin += amounts[d] * exp(logd[div_nodes[d]]);
+        if (!isfinite(origin)) return 0;
+    }
+    prepaid[0] = origin;
+    return 1;
+}
 
 static void group_plan(const or_grouped_input *in, uint32_t g, group_plan_t *gp) {
     uint32_t ncash = 0, nex = 0;
-    if (in->dividend_model == 1) ncash = (uint32_t)(in->cash_offsets[g + 1] - in->cash_offsets[g]);  /* base cancels */
+    if (in->dividend_model != 0) ncash = (uint32_t)(in->cash_offsets[g + 1] - in->cash_offsets[g]);  /* base cancels */
     if (in->mesh_kind == 2) nex = (uint32_t)(in->exercise_offsets[g + 1] - in->exercise_offsets[g]);
     uint32_t cap = (in->mesh_kind == 0 ? in->base_intervals : 1) + nex + ncash + 3;
     gp->times = malloc(sizeof(double) * cap);
@@ -1206,6 +1247,26 @@ static void group_plan(const or_grouped_input *in, uint32_t g, group_plan_t *gp)
     gp->in.exercise_count = gp->nex;
     gp->in.exercise_nodes = gp->ex;
     gp->in.expiry_node = gp->nodes - 1;
+    gp->escrowed = in->dividend_model == 2;
+    gp->prepaid = NULL;
+    gp->start = in->initial_spots[g];
+    gp->prepaid_ok = 1;
+    if (gp->escrowed) {
+        /* EscrowedGbmPathPlan::try_new (path.rs:628-666): the risky component carries no cash
+         * events and, as a prepaid forward, no yield; path_volatility (pricing.rs:840-860)
+         * rescales a spot volatility by S0 / F0 with F0 from the schedule-order origin value
+         * (input.rs:834-860, the same sum as prepaid[0]). */
+        gp->prepaid = malloc(sizeof(double) * gp->nodes);
+        gp->prepaid_ok = escrowed_prepaid_cash(gp->nodes, gp->logd, gp->ndiv, gp->div_nodes,
+                                               gp->div_amounts, gp->prepaid);
+        gp->in.dividends = 0;
+        gp->in.dividend_policy = 0;
+        double S0 = in->initial_spots[g];
+        double prepaid_forward = S0 - gp->prepaid[0];
+        gp->start = prepaid_forward;
+        if (in->escrowed_volatility == 1)
+            gp->in.volatility = in->volatilities[g] * S0 / prepaid_forward;
+    }
 }
 
 static void group_plan_free(group_plan_t *gp) {
@@ -1215,6 +1276,56 @@ static void group_plan_free(group_plan_t *gp) {
     free(gp->div_amounts);
     free(gp->logd);
     free(gp->q);
+    free(gp->prepaid);
+}
+
+/* validate_escrowed_row (input.rs:798-832) and initial_prepaid_forward (path.rs:828-842). */
+static int escrowed_check(const or_grouped_input *in, uint32_t g, const group_plan_t *gp,
+                          or_error *err) {
+    if (!gp->escrowed) return 0;
+    double S0 = in->initial_spots[g];
+    if (!gp->prepaid_ok)
+        return set_err(err, E_ESCROWED_PREPAID_FORWARD, PH_NONE, -1, -1, g, S0, 0.0);
+    double prepaid_forward = S0 - gp->prepaid[0];
+    if (!isfinite(prepaid_forward) || prepaid_forward <= 0.0)
+        return set_err(err, E_ESCROWED_PREPAID_FORWARD, PH_NONE, -1, -1, g, S0, gp->prepaid[0]);
+    if (in->escrowed_volatility == 1) {
+        double transformed = gp->in.volatility;
+        if (!isfinite(transformed) || transformed <= 0.0)
+            return set_err(err, E_ESCROWED_PREPAID_FORWARD, PH_NONE, -1, -1, g, transformed, 0.0);
+    }
+    return 0;
+}
+
+/* reconstruct_observed_stock (path.rs:844-880): origin rows observe S0; other exercise rows add
+ * the prepaid cash of their node, expiry (the last grid node) adds nothing. */
+static int escrowed_reconstruct(const group_plan_t *gp, const plan_t *p, double S0, double *out,
+                                uint32_t paths, or_error *err, int phase) {
+    for (uint32_t o = 0; o < p->obs; ++o) {
+        uint32_t node = p->obs_nodes[o];
+        double *row = out + (size_t)o * paths;
+        if (node == 0) {
+            for (uint32_t i = 0; i < paths; ++i) row[i] = S0;
+            continue;
+        }
+        double prepaid_cash = node == gp->nodes - 1 ? 0.0 : gp->prepaid[node];
+        for (uint32_t i = 0; i < paths; ++i) {
+            double stock = row[i] + prepaid_cash;
+            if (!isfinite(stock))
+                return set_err(err, E_NON_FINITE_TRANSITION, phase, node, i, node, row[i], prepaid_cash);
+            row[i] = stock;
+        }
+    }
+    return 0;
+}
+
+/* One population of a group: the spot model directly, or the escrowed prepaid forward from F0
+ * followed by the observed-stock reconstruction (path.rs:724-739). */
+static int group_simulate(const group_plan_t *gp, const plan_t *p, double S0, uint32_t paths,
+                          uint64_t seed, uint32_t domain, double *out, or_error *err, int phase) {
+    int rc = simulate_population(p, gp->start, paths, seed, domain, NULL, out, err, phase);
+    if (!rc && gp->escrowed) rc = escrowed_reconstruct(gp, p, S0, out, paths, err, phase);
+    return rc;
 }
 
 /* price_group_into (pricing.rs:505-589) + try_from_parts (:711-812). */
@@ -1239,8 +1350,10 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
     or_node_fit *fits = calloc((size_t)strikes * p.obs, sizeof(or_node_fit));
     uint32_t *counts = calloc(p.obs, sizeof(uint32_t));
     uint32_t td = in->training_domain_start + g, ed = in->evaluation_domain_start + g;
-    rc = simulate_population(&p, in->initial_spots[g], nt, in->training_seed, td, NULL, train, err,
-                             PH_TRAIN_SIM);
+    rc = escrowed_check(in, g, &gp, err);
+    if (!rc)
+        rc = group_simulate(&gp, &p, in->initial_spots[g], nt, in->training_seed, td, train, err,
+                            PH_TRAIN_SIM);
     if (rc) {
         err->row = first;
         goto done;
@@ -1256,8 +1369,8 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
     train = NULL;
     if (rc) goto done;
     evalm = malloc(sizeof(double) * p.obs * (size_t)ne);
-    rc = simulate_population(&p, in->initial_spots[g], ne, in->evaluation_seed, ed, NULL, evalm,
-                             err, PH_EVAL_SIM);
+    rc = group_simulate(&gp, &p, in->initial_spots[g], ne, in->evaluation_seed, ed, evalm, err,
+                        PH_EVAL_SIM);
     if (rc) {
         err->row = first;
         goto done;
@@ -1351,8 +1464,10 @@ int or_grouped_models(const or_grouped_input *in, uint32_t g, or_node_fit *fits_
     double *xb = malloc(sizeof(double) * nt), *yb = malloc(sizeof(double) * nt);
     uint32_t *ib = malloc(sizeof(uint32_t) * nt);
     or_node_fit *fits = calloc(p.obs, sizeof(or_node_fit));
-    int rc = simulate_population(&p, in->initial_spots[g], nt, in->training_seed,
-                                 in->training_domain_start + g, NULL, train, err, PH_TRAIN_SIM);
+    int rc = escrowed_check(in, g, &gp, err);
+    if (!rc)
+        rc = group_simulate(&gp, &p, in->initial_spots[g], nt, in->training_seed,
+                            in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < last - first; ++s) {
         or_estimator est = {in->strikes[first + s], in->degree, in->rank_tolerance,
                             in->condition_limit, in->tie_rule, in->insufficient_support};
@@ -1406,8 +1521,11 @@ int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_in
     uint32_t *ib = malloc(sizeof(uint32_t) * nt);
     or_node_fit *fits = calloc((size_t)strikes * pf.obs, sizeof(or_node_fit));
     double *eval = NULL;
-    int rc = simulate_population(&pf, fit_in->initial_spots[g], nt, fit_in->training_seed,
-                                 fit_in->training_domain_start + g, NULL, train, err, PH_TRAIN_SIM);
+    int rc = escrowed_check(fit_in, g, &gf, err);
+    if (!rc) rc = escrowed_check(eval_in, g, &ge, err);
+    if (!rc)
+        rc = group_simulate(&gf, &pf, fit_in->initial_spots[g], nt, fit_in->training_seed,
+                            fit_in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < strikes; ++s) {
         or_estimator est = {fit_in->strikes[first + s], fit_in->degree, fit_in->rank_tolerance,
                             fit_in->condition_limit, fit_in->tie_rule, fit_in->insufficient_support};
@@ -1416,8 +1534,8 @@ int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_in
     }
     if (rc) goto out;
     eval = malloc(sizeof(double) * pe.obs * (size_t)ne);
-    rc = simulate_population(&pe, eval_in->initial_spots[g], ne, eval_in->evaluation_seed,
-                             eval_in->evaluation_domain_start + g, NULL, eval, err, PH_EVAL_SIM);
+    rc = group_simulate(&ge, &pe, eval_in->initial_spots[g], ne, eval_in->evaluation_seed,
+                        eval_in->evaluation_domain_start + g, eval, err, PH_EVAL_SIM);
     if (rc) goto out;
     {
         const uint32_t rows = pe.obs - 1;
@@ -1483,8 +1601,10 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
     double *xb = malloc(sizeof(double) * nt), *yb = malloc(sizeof(double) * nt);
     uint32_t *ib = malloc(sizeof(uint32_t) * nt);
     or_node_fit *fits = calloc((size_t)strikes * p.obs, sizeof(or_node_fit));
-    int rc = simulate_population(&p, in->initial_spots[g], nt, in->training_seed,
-                                 in->training_domain_start + g, NULL, train, err, PH_TRAIN_SIM);
+    int rc = escrowed_check(in, g, &gp, err);
+    if (!rc)
+        rc = group_simulate(&gp, &p, in->initial_spots[g], nt, in->training_seed,
+                            in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < strikes; ++s) {
         or_estimator est = {in->strikes[first + s], in->degree, in->rank_tolerance,
                             in->condition_limit, in->tie_rule, in->insufficient_support};
@@ -1498,19 +1618,51 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
     free(ib);
     if (rc) goto out;
     {
-        const double S0 = in->initial_spots[g], sigma = in->volatilities[g], rate = in->rates[g];
+        const double S0 = in->initial_spots[g], sigma = gp.in.volatility, rate = in->rates[g];
         const uint32_t rows = p.obs - 1;
         const uint32_t ed = in->evaluation_domain_start + g;
         double *sums = calloc((size_t)strikes * 4, sizeof(double));
         double *spot_obs = malloc(sizeof(double) * p.obs * 5);
+        /* Escrowed chain factors. The tangents t follow the simulated state F (d/dF0, d/dsigma_F,
+         * drift d/dr, d/dq). The observed stock is F + A_node(r) with F0 = S0 - A0(r) and, spot
+         * matched, sigma_F = sigma S0 / F0. So dS/dS0 = t0 + t1 dsigma_F/dS0,
+         * dS/dsigma = t1 dsigma_F/dsigma, dS/dr = t2 + t0 dF0/dr + t1 dsigma_F/dr + dA_node/dr.
+         * The spot models use the identity factors. */
+        double c_s = 0.0, c_v = 1.0, c_r0 = 0.0, c_rs = 0.0;
+        double *dprepaid = calloc(gp.nodes, sizeof(double));
+        if (gp.escrowed) {
+            /* d/dr of the prepaid recursion: d exp(logD[n+1] - logD[n]) / dr = -(t[n+1] - t[n]) x
+             * the factor; the origin value differentiates sum_k D_k exp(-r t_k). */
+            for (int64_t n = (int64_t)gp.nodes - 2; n >= 0; --n) {
+                double e = exp(gp.logd[n + 1] - gp.logd[n]);
+                double h = gp.times[n + 1] - gp.times[n];
+                dprepaid[n] = dprepaid[n + 1] * e - gp.prepaid[n + 1] * e * h;
+            }
+            double d0 = 0.0;
+            for (uint32_t d = 0; d < gp.ndiv; ++d) {
+                uint32_t node = gp.div_nodes[d];
+                d0 += -gp.times[node] * gp.div_amounts[d] * exp(gp.logd[node]);
+            }
+            dprepaid[0] = d0;
+            dprepaid[gp.nodes - 1] = 0.0;   /* expiry observes no prepaid cash */
+            const double A0 = gp.prepaid[0], F0 = gp.start;
+            c_r0 = -d0;
+            if (in->escrowed_volatility == 1) {
+                const double sv = in->volatilities[g];
+                c_s = -sv * A0 / (F0 * F0);
+                c_v = S0 / F0;
+                c_rs = sv * S0 * d0 / (F0 * F0);
+            }
+        }
         for (uint32_t path = 0; path < ne && !rc; ++path) {
             stream_t s;
             stream_init(&s, in->evaluation_seed, ed, path, p.intervals);
-            double S = S0, t[4] = {1.0, 0.0, 0.0, 0.0};
+            double S = gp.start, t[4] = {1.0, 0.0, 0.0, 0.0};
             uint32_t observation = 0;
             if (p.obs_nodes[0] == 0) {
-                spot_obs[0] = S;
-                for (int k = 0; k < 4; ++k) spot_obs[1 + k] = t[k];
+                spot_obs[0] = S0;
+                spot_obs[1] = 1.0;
+                for (int k = 1; k < 4; ++k) spot_obs[1 + k] = 0.0;
                 observation = 1;
             }
             for (uint32_t iv = 0; iv < p.intervals; ++iv) {
@@ -1543,8 +1695,14 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
                         for (int k = 0; k < 4; ++k) t[k] = 0.0;
                 }
                 if (observation < p.obs && p.obs_nodes[observation] == node) {
-                    spot_obs[observation * 5] = S;
-                    for (int k = 0; k < 4; ++k) spot_obs[observation * 5 + 1 + k] = t[k];
+                    const int expiry = node == gp.nodes - 1;
+                    const double add = gp.escrowed && !expiry ? gp.prepaid[node] : 0.0;
+                    double *o = spot_obs + observation * 5;
+                    o[0] = S + add;
+                    o[1] = t[0] + c_s * t[1];
+                    o[2] = c_v * t[1];
+                    o[3] = t[2] + c_r0 * t[0] + c_rs * t[1] + dprepaid[node];
+                    o[4] = t[3];
                     observation += 1;
                 }
                 if (has_cash && !terminal_cash) {
@@ -1595,10 +1753,50 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
         }
         free(sums);
         free(spot_obs);
+        free(dprepaid);
     }
 out:
     free(fits);
     plan_free(&p);
+    group_plan_free(&gp);
+    return rc;
+}
+
+int or_grouped_paths(const or_grouped_input *in, uint32_t g, int evaluation, double *out,
+                     or_error *err) {
+    zig_build();
+    group_plan_t gp;
+    group_plan(in, g, &gp);
+    plan_t p;
+    plan_build(&gp.in, &p);
+    uint32_t n = evaluation
+                     ? (in->evaluation_path_counts ? in->evaluation_path_counts[g] : in->evaluation_paths)
+                     : (in->training_path_counts ? in->training_path_counts[g] : in->training_paths);
+    int rc = escrowed_check(in, g, &gp, err);
+    if (!rc)
+        rc = group_simulate(&gp, &p, in->initial_spots[g], n,
+                            evaluation ? in->evaluation_seed : in->training_seed,
+                            (evaluation ? in->evaluation_domain_start : in->training_domain_start) + g,
+                            out, err, evaluation ? PH_EVAL_SIM : PH_TRAIN_SIM);
+    plan_free(&p);
+    group_plan_free(&gp);
+    return rc;
+}
+
+int or_escrowed_prepaid_cash(const or_grouped_input *in, uint32_t g, double *out,
+                             uint32_t capacity, or_error *err) {
+    group_plan_t gp;
+    group_plan(in, g, &gp);
+    int rc = (int)gp.nodes;
+    if (!gp.escrowed) {
+        rc = -set_err(err, E_INTERNAL, PH_NONE, -1, -1, g, 0.0, 0.0);
+    } else if (!gp.prepaid_ok) {
+        rc = -set_err(err, E_ESCROWED_PREPAID_FORWARD, PH_NONE, -1, -1, g, 0.0, 0.0);
+    } else if (gp.nodes > capacity) {
+        rc = -1;
+    } else {
+        memcpy(out, gp.prepaid, sizeof(double) * gp.nodes);
+    }
     group_plan_free(&gp);
     return rc;
 }

@@ -140,7 +140,7 @@ typedef struct {
     const double *volatilities;        /* [groups] */
     uint32_t rows;
     const double *strikes;             /* [rows] */
-    int32_t dividend_model;            /* 0 continuous yield, 1 cash dividends */
+    int32_t dividend_model;            /* 0 continuous yield, 1 cash dividends, 2 escrowed cash */
     const double *dividend_yields;     /* [groups] */
     const int64_t *cash_offsets;       /* [groups + 1] */
     const double *cash_times;
@@ -159,6 +159,7 @@ typedef struct {
     int32_t degree;
     double rank_tolerance, condition_limit;
     int32_t tie_rule, insufficient_support;
+    int32_t escrowed_volatility;       /* model 2: 0 prepaid forward, 1 spot matched at origin */
 } or_grouped_input;
 
 typedef struct {
@@ -188,6 +189,16 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t group, doubl
  * stop_mode 0 decide, 1 decide and record stops[strike * ne + path], 2 hold recorded rows. */
 int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_input *eval_in, uint32_t g,
                              int32_t *stops, int stop_mode, double *prices, or_error *err);
+
+/* The observation matrix [obs][paths] of group g's training (evaluation = 0) or evaluation
+ * population, as the fit and the evaluation see it (escrowed groups: reconstructed stock). */
+int or_grouped_paths(const or_grouped_input *in, uint32_t group, int evaluation, double *out,
+                     or_error *err);
+
+/* Escrowed prepaid cash per grid node of group g (path.rs:897-944): out[0] is the origin value,
+ * out[node] the value observed at exercise nodes; returns the node count or a negative code. */
+int or_escrowed_prepaid_cash(const or_grouped_input *in, uint32_t group, double *out,
+                             uint32_t capacity, or_error *err);
 
 /* Fitted models of one group (strikes x exercise rows) for coefficient-level parity checks. */
 int or_grouped_models(const or_grouped_input *in, uint32_t group, or_node_fit *fits,

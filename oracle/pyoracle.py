@@ -141,6 +141,7 @@ class OrGroupedInput(C.Structure):
         ("condition_limit", C.c_double),
         ("tie_rule", C.c_int32),
         ("insufficient_support", C.c_int32),
+        ("escrowed_volatility", C.c_int32),
     ]
 
 
@@ -487,6 +488,15 @@ class GroupedRequest:
     insufficient_support: int = 0
     training_path_counts: np.ndarray | None = None
     evaluation_path_counts: np.ndarray | None = None
+    # escrowed cash-dividend model for the cash columns: None (spot cash model), 0 prepaid-forward
+    # volatility, 1 spot volatility matched at the origin
+    escrowed: int | None = None
+
+    @property
+    def dividend_model(self) -> int:
+        if self.cash_offsets is None:
+            return 0
+        return 1 if self.escrowed is None else 2
 
     def struct(self) -> tuple[OrGroupedInput, list]:
         f64 = lambda a: np.ascontiguousarray(a if a is not None else [0.0], dtype=np.float64)
@@ -505,14 +515,15 @@ class GroupedRequest:
         s = OrGroupedInput(
             groups, _p(keep[0], C.c_uint32), _p(keep[1], C.c_double), _p(keep[2], C.c_double),
             _p(keep[3], C.c_double), _p(keep[4], C.c_double), len(self.strikes),
-            _p(keep[5], C.c_double), 0 if self.cash_offsets is None else 1,
+            _p(keep[5], C.c_double), self.dividend_model,
             _p(keep[6], C.c_double), _p(keep[7], C.c_int64), _p(keep[8], C.c_double),
             _p(keep[9], C.c_double), int(self.cash_policy), int(self.mesh_kind),
             int(self.base_intervals), int(self.origin_exercise), _p(keep[10], C.c_int64),
             _p(keep[11], C.c_double), int(self.training_paths), int(self.evaluation_paths),
             cptr[0], cptr[1], int(self.training_seed), int(self.evaluation_seed), int(self.training_domain_start),
             int(self.evaluation_domain_start), int(self.degree), float(self.rank_tolerance),
-            float(self.condition_limit), int(self.tie_rule), int(self.insufficient_support))
+            float(self.condition_limit), int(self.tie_rule), int(self.insufficient_support),
+            int(self.escrowed or 0))
         return s, keep
 
 
@@ -608,6 +619,36 @@ def grouped_frozen_prices(fit_req: GroupedRequest, eval_req: GroupedRequest, gro
     if rc:
         raise OracleError(err)
     return {"prices": prices, "stops": buf.reshape(b - a, ne) if mode else None}
+
+
+def grouped_paths(req: GroupedRequest, group: int, *, evaluation: bool = False) -> np.ndarray:
+    """Observation matrix [obs, paths] of one group's training or evaluation population, as the
+    fit and the evaluation read it (escrowed groups: the reconstructed stock)."""
+    s, keep = req.struct()
+    counts = req.evaluation_path_counts if evaluation else req.training_path_counts
+    n = int(counts[group]) if counts is not None else int(
+        req.evaluation_paths if evaluation else req.training_paths)
+    obs = len(group_grid(req, group)["exercise_nodes"]) + 1
+    out = np.zeros((obs, n))
+    err = OrError()
+    rc = lib().or_grouped_paths(C.byref(s), C.c_uint32(group), C.c_int(int(evaluation)),
+                                _p(out, C.c_double), C.byref(err))
+    if rc:
+        raise OracleError(err)
+    return out
+
+
+def escrowed_prepaid_cash(req: GroupedRequest, group: int) -> np.ndarray:
+    """Prepaid cash per grid node of an escrowed group (index 0: origin value)."""
+    s, keep = req.struct()
+    cap = len(group_grid(req, group)["times"])
+    out = np.zeros(cap)
+    err = OrError()
+    n = lib().or_escrowed_prepaid_cash(C.byref(s), C.c_uint32(group), _p(out, C.c_double),
+                                       C.c_uint32(cap), C.byref(err))
+    if n < 0:
+        raise OracleError(err)
+    return out[:n]
 
 
 def threads_available() -> int:

@@ -113,16 +113,23 @@ def oracle_grouped(cols: dict, *, base_intervals: int, training_paths: int,
         volatilities=cols["volatilities"], dividend_yields=cols.get("yields"),
         cash_offsets=cols.get("cash_offsets"), cash_times=cols.get("cash_times"),
         cash_amounts=cols.get("cash_amounts"), cash_policy=cols.get("cash_policy", 0),
-        training_paths=training_paths, evaluation_paths=evaluation_paths, training_seed=seed,
+        escrowed=cols.get("escrowed"), training_paths=training_paths, evaluation_paths=evaluation_paths, training_seed=seed,
         evaluation_seed=seed, training_domain_start=train_domain,
-        evaluation_domain_start=eval_domain, mesh_kind=0, base_intervals=base_intervals,
-        origin_exercise=origin, **kw)
+        evaluation_domain_start=eval_domain, mesh_kind=kw.pop("mesh_kind", 0),
+        base_intervals=base_intervals, origin_exercise=kw.pop("origin_exercise", origin), **kw)
 
 
 def hip_groups(cols: dict):
     import finoptions_hip as fh
 
-    if "cash_offsets" in cols:
+    if "cash_offsets" in cols and cols.get("escrowed") is not None:
+        dividends = fh.EscrowedCashDividendBatch(
+            schedules=fh.RaggedCashDividends(offsets=cols["cash_offsets"], times=cols["cash_times"],
+                                             amounts=cols["cash_amounts"]),
+            volatility_convention=fh.EscrowedVolatilityConvention.PREPAID_FORWARD
+            if cols["escrowed"] == 0 else fh.EscrowedVolatilityConvention.SPOT_MATCHED_AT_ORIGIN,
+            maturity_boundary=fh.MaturityBoundary.STRICTLY_BEFORE)
+    elif "cash_offsets" in cols:
         dividends = fh.CashDividendBatch(
             schedules=fh.RaggedCashDividends(offsets=cols["cash_offsets"], times=cols["cash_times"],
                                              amounts=cols["cash_amounts"]),

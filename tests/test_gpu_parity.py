@@ -93,6 +93,29 @@ def test_fp64_paths_match_the_oracle(hip_f64, oracle):
     assert err < 1e-13
 
 
+def test_device_tensor_export_is_the_host_export(hip_f64):
+    """The observation matrix left in HBM as a torch tensor is bitwise the host export (binary64),
+    or its single rounding (binary32); torch work queued before the call is ordered first."""
+    import finoptions_hip as fh
+    import torch
+
+    inp = c1_inputs()
+    kw = dict(initial_spot=100.0, schedule=inp["schedule"], curve=inp["curve"],
+              dividend_yield_curve=inp["dividend_yield_curve"], volatility=inp["volatility"],
+              dividend_policy=inp["dividend_policy"], paths=40_000,
+              random_stream=fh.RandomStream(seed=7, domain=1), executor=hip_f64)
+    host = fh.simulate_gbm_seeded(**kw)
+    dev = fh.simulate_gbm_seeded_tensor(**kw)
+    assert dev.states.device.type == "cuda" and dev.states.dtype == torch.float64
+    assert dev.observation_nodes == host.observation_nodes
+    assert np.array_equal(dev.states.cpu().numpy(), host.states)
+    single = fh.simulate_gbm_seeded_tensor(**kw, dtype="float32")
+    assert np.array_equal(single.states.cpu().numpy(), host.states.astype(np.float32))
+    assert torch.equal(dev.values_at(2), dev.states[1])
+    with pytest.raises(fh.InputError):
+        fh.simulate_gbm_seeded_tensor(**kw, dtype="int8")
+
+
 def test_c1_plain_fp64_matches_the_oracle(hip_f64, oracle):
     import finoptions_hip as fh
 

@@ -369,7 +369,8 @@ GREEK_INPUTS = {"delta": "initial_spots", "vega": "volatilities", "rho": "rates"
 
 
 @pytest.mark.parametrize("scenario", ["yield", "cash-liquidator", "cash-survivor",
-                                      "large-liquidator-dividend"])
+                                      "large-liquidator-dividend", "escrowed-prepaid-forward",
+                                      "escrowed-spot-matched"])
 def test_pathwise_greeks_are_the_derivative_of_the_stopped_payoff(oracle, scenario):
     """The pathwise estimator is d/dtheta of the mean discounted payoff with every path's stopping
     row held fixed (frozen policy, frozen stopping times). A central difference of that function
@@ -384,6 +385,12 @@ def test_pathwise_greeks_are_the_derivative_of_the_stopped_payoff(oracle, scenar
         kw = dict(dividend_yields=None, cash_offsets=np.array([0, 2], dtype=np.int64),
                   cash_times=np.array([0.3, 0.7]), cash_amounts=np.array([3.0, 2.0]),
                   cash_policy=0 if scenario == "cash-liquidator" else 1)
+    elif scenario.startswith("escrowed"):
+        # S0 moves F0 = S0 - A0 (and sigma_F when spot matched); r moves A(t), F0, sigma_F, the
+        # drift and the discount
+        kw = dict(dividend_yields=None, cash_offsets=np.array([0, 2], dtype=np.int64),
+                  cash_times=np.array([0.3, 0.7]), cash_amounts=np.array([3.0, 2.0]),
+                  escrowed=0 if scenario == "escrowed-prepaid-forward" else 1)
     elif scenario == "large-liquidator-dividend":
         # about 1 path in 40 is liquidated to zero; its tangent must vanish with it
         kw = dict(dividend_yields=None, cash_offsets=np.array([0, 1], dtype=np.int64),
