@@ -262,7 +262,7 @@ def main() -> int:
                        "pivoted power sums, moment-form Stieltjes solve), binary64 exercise "
                        "decisions, binary32 Greek tangents"),
             "insufficient_support": "continue",
-            "greeks": "pathwise delta+vega, frozen policy, paths regenerated from Philox",
+            "greeks": "pathwise delta+vega, frozen policy, evaluation paths rebuilt from normals drawn once per pricing",
             "path_steps_per_step": path_steps,
             "parallelism": "single GPU",
         },

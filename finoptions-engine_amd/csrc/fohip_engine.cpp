@@ -260,7 +260,7 @@ struct fohip_executor {
     size_t batch_budget = 0;   // bytes for paths + values per batch
     std::mutex mu;
     DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, items_e,
-        items_b, rows, eval_out, summary, replay, probe_a, probe_b, probe_c;
+        items_b, rows, eval_out, summary, replay, probe_a, probe_b, probe_c, znorm;
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
     std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
@@ -388,7 +388,7 @@ extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     (void)hipStreamSynchronize(ex->stream);
     for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
                       &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t, &ex->items_e, &ex->items_b, &ex->rows,
-                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c})
+                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm})
         b->release();
     for (auto &p : ex->pending) {
         (void)hipEventDestroy(p.start);
@@ -667,7 +667,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         size_t begin = 0, bytes = 0;
         for (size_t g = 0; g < all.size(); ++g) {
             const GroupPlan &p = all[g];
-            const size_t need = (p.obs_nodes.size() + (p.strike_end - p.strike_begin)) * p.nt * real;
+            size_t need = (p.obs_nodes.size() + (p.strike_end - p.strike_begin)) * p.nt * real;
+            if (spec.fused && p.strike_end - p.strike_begin > 1)   // stored evaluation normals
+                need += (p.times.size() - 1) * p.ne * sizeof(double);
             if (g > begin && bytes + need > ex->batch_budget) {
                 batches.push_back({begin, g});
                 begin = g;
@@ -698,6 +700,11 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         int64_t bad_group, bad_node;
         build_tables(plans, bstrikes, f32, gt, path_elems, value_elems, model_count, part_t, part_e, count_slots, err,
                      bad_group, bad_node);
+        uint64_t znorm_elems = 0;   // evaluation normals, [intervals][ne] per group
+        for (DevGroup &dg : gt.groups) {
+            dg.znorm_base = znorm_elems;
+            znorm_elems += static_cast<uint64_t>(dg.intervals) * dg.ne;
+        }
         const uint32_t ngroups = static_cast<uint32_t>(plans.size());
         std::vector<uint32_t> strike_group(nrows);
         uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
@@ -838,10 +845,21 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         const uint32_t tile = eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
         bool escrow = false;   // the escrowed instantiation also serves spot-model groups
         for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
+        // Several strike tiles read one set of evaluation normals from HBM instead of each
+        // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams).
+        const double *znorm = nullptr;
+        if (spec.fused && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0) {
+            HIP_TRY(ex->znorm.ensure(znorm_elems * sizeof(double)));
+            PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ex->items_e.as<uint2>(),
+                                                               static_cast<uint32_t>(items_e.size()),
+                                                               ex->znorm.as<double>(),
+                                                               ex->fail_e.as<unsigned long long>()));
+            znorm = ex->znorm.as<double>();
+        }
         PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ex->items_e.as<uint2>(),
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
-                                ex->fail_e.as<unsigned long long>()));
+                                ex->fail_e.as<unsigned long long>(), znorm));
         PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
         PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
                                    ex->summary.as<RowSummary>()));

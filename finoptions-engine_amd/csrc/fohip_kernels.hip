@@ -977,12 +977,13 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
 // with its result discarded), so rules and tables are read with scalar loads.
 // With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along the transitions and each
 // strike's stopping row is held fixed (pathwise estimator under the frozen policy).
-template <bool FAST, int TS, bool GREEKS, bool ESCROW>
+template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED>
 __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
                                                      uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
-                                                     unsigned long long *__restrict__ sim_fail, int use_lds_hist) {
+                                                     unsigned long long *__restrict__ sim_fail, int use_lds_hist,
+                                                     const double *__restrict__ znorm) {
     using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ uint32_t hist[];   // [TS][obs] when use_lds_hist
     __shared__ double sx[257], sf[257];
@@ -1004,7 +1005,7 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
     const uint32_t local0 = s_first - strike_begin;
     const Model *tile_models = models + g->model_base + local0 * obs;   // [strike][row]
     const int degree = fp.degree, tie = fp.tie;
-    load_zig(sx, sf);
+    if (!STORED) load_zig(sx, sf);
     if (use_lds_hist) {
         for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
         __syncthreads();
@@ -1032,14 +1033,22 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
         for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Tan(0);
     }
     PathStream s;
-    s.init(g->eval_seed, g->eval_domain, p);
+    if (!STORED) s.init(g->eval_seed, g->eval_domain, p);
+    // STORED: this path's normals, time-major, written once by k_eval_normals (NaN marks an
+    // exhausted draw budget)
+    const double *zp = znorm + g->znorm_base + p;
     double spot = g->start;   // S0, or the escrowed prepaid forward F0
     double tan[4] = {1.0, 0.0, 0.0, 0.0};
     uint32_t row = origin_row;   // the origin row holds no regression rule
     for (uint32_t iv = 0; iv < intervals; ++iv) {
         double z = 0.0;
-        uint32_t draws = 0;
-        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+        if (STORED) {
+            if (p < n) z = zp[static_cast<uint64_t>(iv) * n];
+            if (!(z == z)) ok = false;
+        } else {
+            uint32_t draws = 0;
+            if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+        }
         const uint32_t si = step_base + iv;
         double growth;
         const double next = gbm_advance(spot, *uniform(t.diffusion + si), *uniform(t.log_drift + si), z, growth);
@@ -1290,6 +1299,32 @@ __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model
     out[srow] = rs;
 }
 
+// Evaluation normals of every (interval, path) of the evaluation population, time-major
+// ([intervals][ne] per group), drawn from each path's stream exactly as k_evaluate draws them; the
+// strike tiles of the evaluation pass then read them instead of regenerating Philox and ZIGNOR
+// per tile. A path whose draw budget runs out stores NaN from that interval on and reports itself.
+__global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint2 *__restrict__ items,
+                                                         double *__restrict__ znorm,
+                                                         unsigned long long *__restrict__ sim_fail) {
+    __shared__ double sx[257], sf[257];
+    load_zig(sx, sf);
+    const uint2 item = load_item(items, blockIdx.x);
+    const DevGroup &g = t.groups[item.x];
+    const uint32_t n = g.ne, p = item.y * kEvalChunk + threadIdx.x;
+    if (p >= n) return;
+    PathStream s;
+    s.init(g.eval_seed, g.eval_domain, p);
+    double *out = znorm + g.znorm_base + p;
+    bool ok = true;
+    for (uint32_t iv = 0; iv < g.intervals; ++iv) {
+        double z = 0.0;
+        uint32_t draws = 0;
+        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+        out[static_cast<uint64_t>(iv) * n] = ok ? z : __builtin_nan("");
+    }
+    if (!ok) atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
+}
+
 // Serial replay of one failed path (the reference replays the lowest failing index serially to
 // report a deterministic error, seeded.rs:155-186): out = {interval, spot, normal, kind}.
 __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint32_t path, double *out) {
@@ -1464,17 +1499,17 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     return tile;
 }
 
-template <bool FAST, int TS, bool G, bool E>
+template <bool FAST, int TS, bool G, bool E, bool Z>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
-                                unsigned long long *sim_fail) {
+                                unsigned long long *sim_fail, const double *znorm) {
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
-                       errors, sim_fail, use_lds);
+                       errors, sim_fail, use_lds, znorm);
     return hipGetLastError();
 }
 
@@ -1482,33 +1517,48 @@ template <bool FAST>
 static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
-                                RowError *errors, unsigned long long *sim_fail) {
-#define FOHIP_EVAL_CASE(TSV, GV)                                                                                   \
-    if (tile == TSV && greeks == GV)                                                                               \
-        return escrow ? launch_eval_t<FAST, TSV, GV, true>(st, t, fp, items, nitems, max_strikes, max_obs, models,  \
-                                                           partials, counts, nosupport, errors, sim_fail)          \
-                      : launch_eval_t<FAST, TSV, GV, false>(st, t, fp, items, nitems, max_strikes, max_obs, models, \
-                                                            partials, counts, nosupport, errors, sim_fail);
-    FOHIP_EVAL_CASE(1, false)
-    FOHIP_EVAL_CASE(4, false)
-    FOHIP_EVAL_CASE(16, false)
-    FOHIP_EVAL_CASE(1, true)
-    FOHIP_EVAL_CASE(4, true)
-    FOHIP_EVAL_CASE(8, true)
+                                RowError *errors, unsigned long long *sim_fail, const double *znorm) {
+    // stored normals serve the fused method's multi-tile launches; a single tile regenerates
+#define FOHIP_EVAL_ARGS st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, nosupport, errors, \
+                        sim_fail, znorm
+#define FOHIP_EVAL_CASE(TSV, GV, ZV)                                                                               \
+    if (tile == TSV && greeks == GV && (znorm != nullptr) == ZV)                                                   \
+        return escrow ? launch_eval_t<FAST, TSV, GV, true, ZV>(FOHIP_EVAL_ARGS)                                    \
+                      : launch_eval_t<FAST, TSV, GV, false, ZV>(FOHIP_EVAL_ARGS);
+    FOHIP_EVAL_CASE(1, false, false)
+    FOHIP_EVAL_CASE(4, false, false)
+    FOHIP_EVAL_CASE(16, false, false)
+    FOHIP_EVAL_CASE(1, true, false)
+    FOHIP_EVAL_CASE(4, true, false)
+    FOHIP_EVAL_CASE(8, true, false)
+    if (FAST) {
+        FOHIP_EVAL_CASE(4, false, true)
+        FOHIP_EVAL_CASE(16, false, true)
+        FOHIP_EVAL_CASE(4, true, true)
+        FOHIP_EVAL_CASE(8, true, true)
+    }
 #undef FOHIP_EVAL_CASE
+#undef FOHIP_EVAL_ARGS
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile, const DevTables &t,
                            const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                            uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
-                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail) {
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm) {
     if (nitems == 0) return hipSuccess;
     if (fast)
         return launch_eval_r<true>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                    partials, counts, nosupport, errors, sim_fail);
+                                    partials, counts, nosupport, errors, sim_fail, znorm);
     return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                 partials, counts, nosupport, errors, sim_fail);
+                                 partials, counts, nosupport, errors, sim_fail, nullptr);
+}
+
+hipError_t launch_eval_normals(hipStream_t st, const DevTables &t, const uint2 *items, uint32_t nitems, double *znorm,
+                               unsigned long long *sim_fail) {
+    if (nitems == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_eval_normals, dim3(nitems), dim3(kBlock), 0, st, t, items, znorm, sim_fail);
+    return hipGetLastError();
 }
 
 hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models, const uint32_t *counts,

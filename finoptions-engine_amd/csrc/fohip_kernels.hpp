@@ -57,7 +57,11 @@ hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, 
                            const DevTables &t, const FitParams &fp, const uint2 *items,
                            uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                            const Model *models, double *partials, uint32_t *counts,
-                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail);
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail,
+                           const double *znorm);
+// Evaluation normals [intervals][ne] per group at DevGroup::znorm_base (one block per kEvalChunk paths).
+hipError_t launch_eval_normals(hipStream_t st, const DevTables &t, const uint2 *items, uint32_t nitems,
+                               double *znorm, unsigned long long *sim_fail);
 hipError_t launch_eval_reduce(hipStream_t st, const DevTables &t, const uint32_t *rows,
                               uint32_t nrows, const double *partials, int greeks, double *out);
 

@@ -85,6 +85,7 @@ struct DevGroup {
     // Greek chain factors of the observed stock (identity for the spot models): dsigma_F/dS0,
     // dsigma_F/dsigma, dF0/dr, dsigma_F/dr.
     double chain[4];
+    uint64_t znorm_base;      // element offset of the stored evaluation normals [intervals][ne]
 };
 
 // Node flags
