@@ -114,6 +114,20 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def copy_calibration() -> int:
+    """Known-size traffic for the HBM counters: each launch reads and writes exactly 256 MiB."""
+    import torch
+
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, 0.0, out=b)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 x {4 * n >> 20} MiB read and written; check {float(b[0])}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +137,12 @@ def main() -> int:
     ap.add_argument("--cpu-sample-paths", type=int, default=1 << 16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the reference copy for the rocprofv3 FETCH_SIZE / WRITE_SIZE scale "
+                         "factors: three elementwise copies of 256 MiB (64 Mi float32)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         log("bench.py measures one GPU only: every box in this setup has a single MI355X and the "
             "multi-GPU sharding of this workload is a design note, not a measured path. "

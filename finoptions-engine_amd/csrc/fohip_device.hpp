@@ -227,18 +227,27 @@ __device__ __forceinline__ double predict_ref(MP m, int degree, double x) {
 
 // Evaluation form of an accepted rule at spot S: z = fma(S, 1/(scale K), -mean/scale), Horner in
 // z. Unused degrees carry zero coefficients, so the fixed four FMAs serve every degree.
+// c * z + u with u wave-uniform: the VOP3 form reads u from a scalar register directly (the
+// accumulating VOP2 form the compiler otherwise picks first copies u into the result register).
+__device__ __forceinline__ double fma_uniform_addend(double c, double z, double u) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(c), "v"(z), "s"(u));
+    return r;
+}
+
 template <typename MP>
 __device__ __forceinline__ double continuation_fast(MP m, double spot) {
     const double z = fma(spot, m->a, m->b);
-    double c = m->hz[4];
-    c = fma(c, z, m->hz[3]);
-    c = fma(c, z, m->hz[2]);
-    c = fma(c, z, m->hz[1]);
-    return fma(c, z, m->hz[0]);
+    double c = fma(m->hz[4], z, m->hz[3]);
+    c = fma_uniform_addend(c, z, m->hz[2]);
+    c = fma_uniform_addend(c, z, m->hz[1]);
+    return fma_uniform_addend(c, z, m->hz[0]);
 }
 
+// ExerciseTieRule::should_exercise (policy.rs:21-26): strictly greater, or greater or equal. Written
+// as two comparisons joined by lane-mask logic rather than a select between them.
 __device__ __forceinline__ bool should_exercise(int tie, double intrinsic, double continuation) {
-    return tie == 0 ? intrinsic > continuation : intrinsic >= continuation;
+    return (intrinsic > continuation) | ((tie != 0) & (intrinsic == continuation));
 }
 
 // Tangents of the observed stock S = F + A(node) w.r.t. (S0, sigma, r, q) from those of the

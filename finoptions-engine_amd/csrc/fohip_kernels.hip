@@ -457,7 +457,7 @@ __device__ int jacobi_singular_values(double (&mat)[25], double (&sv)[5]) {
 
 // Expands sum_k coef_k p_k(z) into monomials of z (p_0 = 1, p_1 = z - alpha_0,
 // p_{k+1} = (z - alpha_k) p_k - beta_k p_{k-1}) and stores z's affine map.
-__device__ void set_evaluation_form(Model &m, int degree) {
+__device__ __forceinline__ void set_evaluation_form(Model &m, int degree) {
     double previous[5] = {0, 0, 0, 0, 0}, current[5] = {1, 0, 0, 0, 0};
 #pragma unroll
     for (int j = 0; j < 5; ++j) m.hz[j] = 0.0;
@@ -813,6 +813,9 @@ __device__ __forceinline__ double binom9(int j, int k) {
 // (binomial recentring), then the Stieltjes solve above. One wave per strike row reduces the chunk
 // partials of (strike row, exercise row obs - 2 - step) in fixed order (lane-strided chunk sums,
 // then an LDS transpose) and its lane 0 fits the rule.
+__device__ __forceinline__ void solve_node_from_sums(Model &m, const FitParams &fp, RowError *err, uint32_t step,
+                                                     double strike, const double (&acc)[14], double pivot);
+
 __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams fp, uint32_t step,
                                                         const uint32_t *__restrict__ rows,
                                                         double *__restrict__ partials, Model *__restrict__ models,
@@ -841,9 +844,19 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = buf[i];
     const double pivot = partial_slot(partials, g, local, 0)[14];
-    Model &m = models[g.model_base + local * g.obs + r];
+    // The rule is solved in registers and stored once: solving in place would turn every field
+    // update into a global store and every re-read into a dependent global load.
+    Model m = {};
+    solve_node_from_sums(m, fp, &errors[srow], step, t.strikes[srow], acc, pivot);
+    models[g.model_base + local * g.obs + r] = m;
+}
+
+// Support decision, binomial recentring of the pivoted power sums and moment-form Stieltjes solve
+// of one (strike, row) of the fused method; `acc` holds the 9 + 5 power sums about `pivot`.
+__device__ __forceinline__ void solve_node_from_sums(Model &m, const FitParams &fp, RowError *err, uint32_t step,
+                                                     double strike, const double (&acc)[14], double pivot) {
     const double nd = acc[0];
-    if (!decide_support(m, fp, &errors[srow], step, static_cast<uint32_t>(nd), t.strikes[srow])) return;
+    if (!decide_support(m, fp, err, step, static_cast<uint32_t>(nd), strike)) return;
     double mu[9], nu[5];
     double mean = 0.0, scale = 1.0;
     if (fp.degree == 0) {
@@ -858,7 +871,7 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
         scale = sqrt(fmax(m2, 0.0) / nd);
         if (!isfinite(mean) || !isfinite(scale) || scale <= 0.0) {
             m.kind = kRuleNone;
-            fail_row(&errors[srow], step, FOHIP_E_DEGENERATE_PREDICTOR, fp.degree, scale, 0.0);
+            fail_row(err, step, FOHIP_E_DEGENERATE_PREDICTOR, fp.degree, scale, 0.0);
             return;
         }
         // sum (x - mean)^j = sum_k C(j, k) (-d)^(j - k) P_k, divided by scale^j
@@ -884,7 +897,7 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
         mu[1] = 0.0;   // exactly centred
     }
     begin_fit(m, mean, scale);
-    stieltjes_from_moments(m, fp, &errors[srow], step, mu, nu);
+    stieltjes_from_moments(m, fp, err, step, mu, nu);
 }
 
 // After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
@@ -1081,6 +1094,19 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
             const double t_row = GREEKS ? *uniform(t.row_time + row_base + row) : 0.0;
             // the exercise decision observes the stock: F + A(node) for escrowed groups
             const double S = ESCROW ? spot + *uniform(t.row_escrow + row_base + row) : spot;
+            // discounted observed tangents, shared by every strike that stops on this row
+            double od[4] = {0.0, 0.0, 0.0, 0.0};
+            if (GREEKS) {
+                double o[4] = {tan[0], tan[1], tan[2], tan[3]};
+                if (ESCROW) observed_tangents(tan, uniform(g->chain), *uniform(t.row_escrow_dr + row_base + row), o);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) od[a] = o[a] * disc;
+            }
+            Tan od0 = static_cast<Tan>(od[0]), od1 = static_cast<Tan>(od[1]), od3 = static_cast<Tan>(od[3]);
+            double od2 = od[2];
+            if (GREEKS) {   // formed once per row: keep the compiler from recomputing them per strike
+                asm volatile("" : "+v"(od0), "+v"(od1), "+v"(od3), "+v"(od2));
+            }
 #pragma unroll
             for (int i = 0; i < TS; ++i) {
                 if (i >= ns) break;
@@ -1102,12 +1128,10 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
                     const double pv = intr * disc;
                     pay[i] = ex ? pv : pay[i];
                     if (GREEKS) {
-                        double o[4] = {tan[0], tan[1], tan[2], tan[3]};
-                        if (ESCROW) observed_tangents(tan, uniform(g->chain), *uniform(t.row_escrow_dr + row_base + row), o);
-                        dpay[i][0] = ex ? static_cast<Tan>(o[0] * disc) : dpay[i][0];
-                        dpay[i][1] = ex ? static_cast<Tan>(o[1] * disc) : dpay[i][1];
-                        dpay[i][2] = ex ? static_cast<Tan>(o[2] * disc - t_row * pv) : dpay[i][2];
-                        dpay[i][3] = ex ? static_cast<Tan>(o[3] * disc) : dpay[i][3];
+                        dpay[i][0] = ex ? od0 : dpay[i][0];
+                        dpay[i][1] = ex ? od1 : dpay[i][1];
+                        dpay[i][2] = ex ? static_cast<Tan>(od2 - t_row * pv) : dpay[i][2];
+                        dpay[i][3] = ex ? od3 : dpay[i][3];
                     }
                     active &= ~(((ex || bad) ? 1u : 0u) << i);
                     if (bad)
