@@ -837,6 +837,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                 PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
             }
         }
+        if (spec.fused)   // rank/condition gates of every rule of the backward pass at once
+            PROF_LAUNCH("k_fit_gate", launch_fit_gate(st, t, fp, rows, nrows, models, static_cast<uint32_t>(model_count),
+                                                      errors));
         PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
                                  ex->values.ptr, models, ex->part_t.as<double>(), errors));
         PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));

@@ -483,8 +483,77 @@ __device__ __forceinline__ void set_evaluation_form(Model &m, int degree) {
     m.b = degree == 0 ? 0.0 : -m.mean * inv_scale;
 }
 
-// Rank gate, condition number and acceptance of a completed fit.
+// Rank gate and condition number of a completed fit (diagnose_design, regression.rs:540-577).
+// Returns 0 and sets rank/condition when the fit is accepted, else a FOHIP_E_* code with its
+// error detail.
+__device__ __forceinline__ int gate_fit(Model &m, const FitParams &fp, int64_t &index, double &observed,
+                                        double &limit) {
+    double tr[25], sv[5];
+    const int columns = fp.degree + 1;
+    index = 0;
+    observed = limit = 0.0;
+    const int bad = orthonormal_transform(fp.degree, m, tr);
+    if (bad) {
+        index = bad - 1;
+        return FOHIP_E_NON_FINITE_FIT;
+    }
+    int jr = 0;
+    switch (columns) {
+        case 1: jr = jacobi_singular_values<1>(tr, sv); break;
+        case 2: jr = jacobi_singular_values<2>(tr, sv); break;
+        case 3: jr = jacobi_singular_values<3>(tr, sv); break;
+        case 4: jr = jacobi_singular_values<4>(tr, sv); break;
+        default: jr = jacobi_singular_values<5>(tr, sv); break;
+    }
+    if (jr < 0) {
+        index = columns;
+        return FOHIP_E_JACOBI_NOT_CONVERGED;
+    }
+    if (jr > 0) {
+        index = jr - 1;
+        return FOHIP_E_NON_FINITE_FIT;
+    }
+    const double largest = sv[0], threshold = fp.rank_tol * largest;
+    int rank = 0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) rank += (c < columns && sv[c] > threshold) ? 1 : 0;
+    if (rank < columns) {
+        index = rank;
+        observed = columns;
+        limit = fp.rank_tol;
+        return FOHIP_E_RANK_DEFICIENT;
+    }
+    double smallest = sv[0];
+#pragma unroll
+    for (int c = 1; c < 5; ++c)
+        if (c < columns) smallest = sv[c];
+    const double condition = largest / smallest;
+    if (!isfinite(condition)) {
+        index = fp.degree;
+        return FOHIP_E_NON_FINITE_FIT;
+    }
+    if (condition > fp.cond_limit) {
+        index = fp.degree;
+        observed = condition;
+        limit = fp.cond_limit;
+        return FOHIP_E_CONDITION_LIMIT;
+    }
+    m.rank = rank;
+    m.condition = condition;
+    return 0;
+}
+
+// Rank gate, condition number and acceptance of a completed fit. With DEFER the rule is accepted
+// now and the gate runs later (k_fit_gate): the next node's fit uses only the coefficients, and a
+// rule that fails the gate fails its row with the same error key.
+template <bool DEFER>
 __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, uint32_t step) {
+    if (DEFER) {
+        m.gate = kGatePending;
+        set_evaluation_form(m, fp.degree);
+        m.kind = kRuleRegression;
+        return;
+    }
     double tr[25], sv[5];
     const int columns = fp.degree + 1;
     const int bad = orthonormal_transform(fp.degree, m, tr);
@@ -542,6 +611,7 @@ __device__ void diagnose_and_accept(Model &m, const FitParams &fp, RowError *e, 
 }
 
 // One Stieltjes step (regression.rs:448-514) from the reduced moments of p_k.
+template <bool DEFER>
 __device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint32_t step, int k, double norm,
                                double projection, double weighted) {
     if (!isfinite(norm) || !isfinite(projection) || !isfinite(weighted)) {
@@ -571,7 +641,7 @@ __device__ void stieltjes_step(Model &m, const FitParams &fp, RowError *e, uint3
         }
         m.prev_norm = norm;
     } else {
-        diagnose_and_accept(m, fp, e, step);
+        diagnose_and_accept<DEFER>(m, fp, e, step);
     }
 }
 
@@ -596,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
         sums[2] += in[2];
     }
     block_sum<3>(sums, scratch);
-    if (threadIdx.x == 0) stieltjes_step(m, fp, &errors[srow], step, k, sums[0], sums[1], sums[2]);
+    if (threadIdx.x == 0) stieltjes_step<false>(m, fp, &errors[srow], step, k, sums[0], sums[1], sums[2]);
 }
 
 // Fused backward step j over exercise row r = obs - 2 - j in ONE pass over HBM.
@@ -780,7 +850,7 @@ __device__ void stieltjes_from_moments(Model &m, const FitParams &fp, RowError *
                 weighted += a_cur[i] * a_cur[j] * mu[i + j + 1 < 9 ? i + j + 1 : 8];
             }
         }
-        stieltjes_step(m, fp, e, step, k, norm, projection, weighted);
+        stieltjes_step<true>(m, fp, e, step, k, norm, projection, weighted);
         if (m.kind != kRulePending) return;
         double a_next[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1323,6 +1393,48 @@ __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model
     out[srow] = rs;
 }
 
+// Deferred rank/condition gate of the fused method (diagnose_design, regression.rs:540-577): one
+// thread per rule accepted during the backward pass. A rule that fails keeps its code for
+// k_fit_gate_scan and stops being a regression.
+__global__ __launch_bounds__(kBlock) void k_fit_gate(FitParams fp, Model *__restrict__ models, uint32_t count) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    if (models[i].kind != kRuleRegression || models[i].gate != kGatePending) return;
+    Model m = models[i];
+    int64_t index = 0;
+    double observed = 0.0, limit = 0.0;
+    const int code = gate_fit(m, fp, index, observed, limit);
+    if (code != 0) {
+        m.kind = kRuleNone;
+        m.gate = code;
+        m.gate_index = static_cast<int32_t>(index);
+        m.gate_observed = observed;
+        m.gate_limit = limit;
+    } else {
+        m.gate = kGateDone;
+    }
+    models[i] = m;
+}
+
+// The earliest failed gate of each strike row in processing order (exercise rows descending)
+// becomes the row's regression failure, unless an earlier failure is already recorded.
+__global__ __launch_bounds__(kBlock) void k_fit_gate_scan(DevTables t, const uint32_t *__restrict__ rows,
+                                                          uint32_t nrows, const Model *__restrict__ models,
+                                                          RowError *__restrict__ errors) {
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= nrows) return;
+    const uint32_t srow = rows[idx];
+    const DevGroup &g = t.groups[t.strike_group[srow]];
+    const uint32_t local = srow - g.strike_begin;
+    for (uint32_t step = 0; step < g.fit_rows; ++step) {
+        const Model &m = models[g.model_base + local * g.obs + (g.obs - 2 - step)];
+        if (m.gate > 0) {
+            fail_row(&errors[srow], step, m.gate, m.gate_index, m.gate_observed, m.gate_limit);
+            break;
+        }
+    }
+}
+
 // Evaluation normals of every (interval, path) of the evaluation population, time-major
 // ([intervals][ne] per group), drawn from each path's stream exactly as k_evaluate draws them; the
 // strike tiles of the evaluation pass then read them instead of regenerating Philox and ZIGNOR
@@ -1576,6 +1688,17 @@ hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, 
                                     partials, counts, nosupport, errors, sim_fail, znorm);
     return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
                                  partials, counts, nosupport, errors, sim_fail, nullptr);
+}
+
+hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows, uint32_t nrows,
+                           Model *models, uint32_t model_count, RowError *errors) {
+    if (model_count > 0)
+        hipLaunchKernelGGL(k_fit_gate, dim3((model_count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, fp, models,
+                           model_count);
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_fit_gate_scan, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, rows, nrows,
+                           models, errors);
+    return hipGetLastError();
 }
 
 hipError_t launch_eval_normals(hipStream_t st, const DevTables &t, const uint2 *items, uint32_t nitems, double *znorm,

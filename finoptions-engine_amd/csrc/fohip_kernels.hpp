@@ -59,6 +59,9 @@ hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, 
                            const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail,
                            const double *znorm);
+// Deferred rank/condition gate of every fused rule, then each row's earliest gate failure.
+hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows,
+                           uint32_t nrows, Model *models, uint32_t model_count, RowError *errors);
 // Evaluation normals [intervals][ne] per group at DevGroup::znorm_base (one block per kEvalChunk paths).
 hipError_t launch_eval_normals(hipStream_t st, const DevTables &t, const uint2 *items, uint32_t nitems,
                                double *znorm, unsigned long long *sim_fail);

@@ -50,7 +50,16 @@ struct alignas(64) Model {
     uint32_t itm;             // in-the-money training paths
     int32_t rank;
     int32_t exercise;         // origin decision
+    // Fused method: a regression is accepted once its coefficients exist and the rank/condition
+    // gate (kGatePending) runs for every rule after the backward pass; a failed gate leaves its
+    // FOHIP_E_* code and detail here for the row's error.
+    int32_t gate;
+    int32_t gate_index;
+    double gate_observed, gate_limit;
 };
+
+constexpr int32_t kGateDone = 0;
+constexpr int32_t kGatePending = -1;
 
 struct DevGroup {
     double S0;

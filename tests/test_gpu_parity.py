@@ -337,6 +337,33 @@ def test_fail_policy_reports_insufficient_support_like_the_oracle(hip_f64, oracl
     assert gpu_err.value.index == cpu_err.value.index
 
 
+def test_condition_limit_failure_is_reported_like_the_oracle(hip_f64, hip_fused, oracle):
+    """A condition limit that some fitted rules exceed fails the request at the first failing
+    (row, node) in processing order. The fused method accepts each rule provisionally and runs
+    the rank/condition gates after the backward pass, so it must report the same failure."""
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=8)
+    req = oracle_grouped(cols, base_intervals=10, training_paths=8192, evaluation_paths=8192,
+                         seed=5, insufficient_support=1, condition_limit=140.0)
+    with pytest.raises(oracle.OracleError) as cpu_err:
+        oracle.price_grouped(req)
+    assert cpu_err.value.code == 303
+    mesh = fh.UniformExerciseMesh(base_intervals=10, origin_exercise=fh.OriginExercise.EXCLUDED)
+    base = hip_method(8192, 8192, 5, support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    method = fh.Lsm(training_paths=base.training_paths, evaluation_paths=base.evaluation_paths,
+                    training_stream=base.training_stream, evaluation_stream=base.evaluation_stream,
+                    insufficient_exercise_support=base.insufficient_exercise_support,
+                    condition_limit=140.0)
+    for hip in (hip_f64, hip_fused):
+        with pytest.raises(fh.LsmError) as gpu_err:
+            fh.price_lsm_call_grouped_batch(groups=hip_groups(cols), mesh=mesh, executor=hip,
+                                            method=method)
+        e, c = gpu_err.value, cpu_err.value
+        assert (e.code, e.row, e.node, e.index) == (c.code, c.row, c.node, c.index)
+        assert abs(e.observed - c.observed) <= 1e-10 * c.observed
+
+
 def _ragged_requests(fh, oracle, contracts, *, seed, path_scale, max_exercise):
     from finoptions_hip.workloads import ragged_batch
 
