@@ -900,7 +900,22 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
     double acc[14];
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
-    for (uint32_t c = lane; c < g.chunks_b; c += 64) {
+    // four chunks' loads in flight per round trip; the sums still add chunks c, c + 64, ... in order
+    uint32_t c = lane;
+    for (; c + 3 * 64 < g.chunks_b; c += 4 * 64) {
+        double in[4][14];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double *slot = partial_slot(partials, g, local, c + u * 64);
+#pragma unroll
+            for (int i = 0; i < 14; ++i) in[u][i] = slot[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int i = 0; i < 14; ++i) acc[i] += in[u][i];
+    }
+    for (; c < g.chunks_b; c += 64) {
         const double *in = partial_slot(partials, g, local, c);
 #pragma unroll
         for (int i = 0; i < 14; ++i) acc[i] += in[i];
