@@ -249,6 +249,36 @@ struct DevBuf {
     template <typename T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Page-locked host memory: copies from and to it are DMA transfers that return at once, where a
+// pageable buffer costs a staged, blocking copy per call. Grown only between batches, after the
+// stream has drained.
+struct PinnedBuf {
+    char *ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipHostFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, size_t(1) << 16);
+        hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&ptr), want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
 }  // namespace
 
 struct fohip_executor {
@@ -259,8 +289,9 @@ struct fohip_executor {
     fohip_tuning tuning = {0, 0};
     size_t batch_budget = 0;   // bytes for paths + values per batch
     std::mutex mu;
-    DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, items_e,
-        items_b, rows, eval_out, summary, replay, probe_a, probe_b, probe_c, znorm;
+    DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, eval_out,
+        summary, replay, probe_a, probe_b, probe_c, znorm;
+    PinnedBuf staging;   // per-batch tables and items on the way in, results on the way out
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
     std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
@@ -387,9 +418,10 @@ extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     (void)hipSetDevice(ex->device);
     (void)hipStreamSynchronize(ex->stream);
     for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
-                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t, &ex->items_e, &ex->items_b, &ex->rows,
+                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t,
                       &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm})
         b->release();
+    ex->staging.release();
     for (auto &p : ex->pending) {
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
@@ -748,7 +780,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             gt.log_drift32.size() * 4, gt.diffusion32.size() * 4, gt.dt.size() * 8, gt.sqrt_dt.size() * 8,
             gt.cash.size() * 8, gt.node_flags.size() * 4, gt.disc0.size() * 8, gt.disc_step.size() * 8,
             gt.row_time.size() * 8, gt.row_node.size() * 4, bstrikes.size() * 8, strike_group.size() * 4,
-            gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, binv.size() * 8};
+            gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, binv.size() * 8,
+            items_t.size() * sizeof(uint2), items_e.size() * sizeof(uint2), items_b.size() * sizeof(uint2),
+            rows_sorted.size() * sizeof(uint32_t)};
         size_t tables_bytes = 0;
         std::vector<size_t> offs;
         for (size_t b : tables_bytes_list) {
@@ -766,10 +800,6 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         HIP_TRY(ex->errors.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowError)));
         HIP_TRY(ex->fail_t.ensure(ngroups * sizeof(unsigned long long)));
         HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
-        HIP_TRY(ex->items_t.ensure(std::max<size_t>(items_t.size(), 1) * sizeof(uint2)));
-        HIP_TRY(ex->items_e.ensure(std::max<size_t>(items_e.size(), 1) * sizeof(uint2)));
-        HIP_TRY(ex->items_b.ensure(std::max<size_t>(items_b.size(), 1) * sizeof(uint2)));
-        HIP_TRY(ex->rows.ensure(std::max<size_t>(rows_sorted.size(), 1) * sizeof(uint32_t)));
         HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
         HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
         char *tb = ex->tables.as<char>();
@@ -777,9 +807,35 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                                     gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
                                     gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
                                     gt.row_node.data(), bstrikes.data(), strike_group.data(),
-                                    gt.row_escrow.data(), gt.row_escrow_dr.data(), binv.data()};
+                                    gt.row_escrow.data(), gt.row_escrow_dr.data(), binv.data(),
+                                    items_t.data(), items_e.data(), items_b.data(), rows_sorted.data()};
+        // results come back through the staging area too, after the tables
+        struct Download {
+            void *host;
+            const void *dev;
+            size_t bytes;
+        };
+        std::vector<unsigned long long> fail_t(ngroups), fail_e(ngroups);
+        std::vector<RowError> row_err(nrows);
+        std::vector<double> eval_out(static_cast<size_t>(nrows) * 16);
+        std::vector<RowSummary> summary(nrows);
+        std::vector<Model> models_h(models_out ? model_count : 0);
+        std::vector<uint32_t> counts_h(counts_out ? count_slots : 0);
+        const Download downloads[] = {
+            {fail_t.data(), ex->fail_t.ptr, ngroups * sizeof(unsigned long long)},
+            {fail_e.data(), ex->fail_e.ptr, ngroups * sizeof(unsigned long long)},
+            {row_err.data(), ex->errors.ptr, nrows * sizeof(RowError)},
+            {eval_out.data(), ex->eval_out.ptr, eval_out.size() * sizeof(double)},
+            {summary.data(), ex->summary.ptr, nrows * sizeof(RowSummary)},
+            {models_h.data(), ex->models.ptr, models_h.size() * sizeof(Model)},
+            {counts_h.data(), ex->counts.ptr, counts_h.size() * sizeof(uint32_t)}};
+        size_t staged = tables_bytes;
+        for (const Download &d : downloads) staged += (d.bytes + 255) & ~size_t(255);
+        // one upload of every table, item list and row list of the batch
+        HIP_TRY(ex->staging.ensure(staged));
         for (size_t i = 0; i < offs.size(); ++i)
-            if (tables_bytes_list[i]) HIP_TRY(hipMemcpyAsync(tb + offs[i], host_parts[i], tables_bytes_list[i], hipMemcpyHostToDevice, st));
+            if (tables_bytes_list[i]) std::memcpy(ex->staging.ptr + offs[i], host_parts[i], tables_bytes_list[i]);
+        HIP_TRY(hipMemcpyAsync(tb, ex->staging.ptr, tables_bytes, hipMemcpyHostToDevice, st));
         DevTables t;
         t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
         t.log_drift = reinterpret_cast<const double *>(tb + offs[1]);
@@ -799,18 +855,23 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         t.row_escrow = reinterpret_cast<const double *>(tb + offs[15]);
         t.row_escrow_dr = reinterpret_cast<const double *>(tb + offs[16]);
         t.inv_strikes = reinterpret_cast<const double *>(tb + offs[17]);
-        HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items_t.data(), items_t.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(ex->items_e.ptr, items_e.data(), items_e.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(ex->items_b.ptr, items_b.data(), items_b.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(ex->rows.ptr, rows_sorted.data(), rows_sorted.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemsetAsync(ex->models.ptr, 0, model_count * sizeof(Model), st));
-        HIP_TRY(hipMemsetAsync(ex->counts.ptr, 0, count_slots * sizeof(uint32_t), st));
-        HIP_TRY(hipMemsetAsync(ex->nosup.ptr, 0, count_slots * sizeof(uint32_t), st));
-        HIP_TRY(hipMemsetAsync(ex->errors.ptr, 0xFF, nrows * sizeof(RowError), st));
-        HIP_TRY(hipMemsetAsync(ex->fail_t.ptr, 0xFF, ngroups * sizeof(unsigned long long), st));
-        HIP_TRY(hipMemsetAsync(ex->fail_e.ptr, 0xFF, ngroups * sizeof(unsigned long long), st));
-        const uint2 *it = ex->items_t.as<uint2>();
-        const uint32_t *rows = ex->rows.as<uint32_t>();
+        const uint2 *it = reinterpret_cast<const uint2 *>(tb + offs[18]);
+        const uint2 *ite = reinterpret_cast<const uint2 *>(tb + offs[19]);
+        const uint2 *itb = reinterpret_cast<const uint2 *>(tb + offs[20]);
+        const uint32_t *rows = reinterpret_cast<const uint32_t *>(tb + offs[21]);
+        // rules zeroed, exercise and no-support counters zeroed, failure keys set to "none"
+        static_assert(sizeof(Model) % 4 == 0 && sizeof(RowError) % 4 == 0, "fills write 32-bit words");
+        FillList fills = {};
+        auto add_fill = [&fills](void *ptr, size_t bytes, uint32_t value) {
+            fills.region[fills.count++] = FillRegion{static_cast<uint32_t *>(ptr), bytes / 4, value, 0};
+        };
+        add_fill(ex->models.ptr, model_count * sizeof(Model), 0u);
+        add_fill(ex->counts.ptr, count_slots * sizeof(uint32_t), 0u);
+        add_fill(ex->nosup.ptr, count_slots * sizeof(uint32_t), 0u);
+        add_fill(ex->errors.ptr, nrows * sizeof(RowError), 0xFFFFFFFFu);
+        add_fill(ex->fail_t.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(ex->fail_e.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        HIP_TRY(launch_fill(st, fills));
         Model *models = ex->models.as<Model>();
         RowError *errors = ex->errors.as<RowError>();
         const FitParams &fp = spec.fp;
@@ -821,7 +882,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         for (uint32_t j = 0; j < max_fit; ++j) {
             const uint32_t ni = step_items[j], nr = step_rows[j];
             if (spec.fused) {   // one fused HBM pass per backward step
-                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, ex->items_b.as<uint2>(), step_items_b[j], max_strikes, ex->paths.ptr,
+                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, ex->paths.ptr,
                                                                     ex->values.ptr, models, ex->part_t.as<double>(),
                                                                     errors));
                 PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
@@ -853,41 +914,34 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         const double *znorm = nullptr;
         if (spec.fused && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0) {
             HIP_TRY(ex->znorm.ensure(znorm_elems * sizeof(double)));
-            PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ex->items_e.as<uint2>(),
+            PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ite,
                                                                static_cast<uint32_t>(items_e.size()),
                                                                ex->znorm.as<double>(),
                                                                ex->fail_e.as<unsigned long long>()));
             znorm = ex->znorm.as<double>();
         }
-        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ex->items_e.as<uint2>(),
+        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ite,
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
                                 ex->fail_e.as<unsigned long long>(), znorm));
         PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
         PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
                                    ex->summary.as<RowSummary>()));
-        // ---- results
-        std::vector<unsigned long long> fail_t(ngroups), fail_e(ngroups);
-        std::vector<RowError> row_err(nrows);
-        std::vector<double> eval_out(static_cast<size_t>(nrows) * 16);
-        std::vector<RowSummary> summary(nrows);
-        HIP_TRY(hipMemcpyAsync(fail_t.data(), ex->fail_t.ptr, ngroups * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(fail_e.data(), ex->fail_e.ptr, ngroups * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(row_err.data(), ex->errors.ptr, nrows * sizeof(RowError), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(eval_out.data(), ex->eval_out.ptr, eval_out.size() * 8, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(summary.data(), ex->summary.ptr, nrows * sizeof(RowSummary), hipMemcpyDeviceToHost, st));
-        std::vector<Model> models_h;
-        std::vector<uint32_t> counts_h;
-        if (models_out) {
-            models_h.resize(model_count);
-            HIP_TRY(hipMemcpyAsync(models_h.data(), ex->models.ptr, model_count * sizeof(Model), hipMemcpyDeviceToHost, st));
+        // ---- results: DMA transfers into the staging area, copied out once the stream drains
+        {
+            size_t off = tables_bytes;
+            for (const Download &d : downloads) {
+                if (d.bytes) HIP_TRY(hipMemcpyAsync(ex->staging.ptr + off, d.dev, d.bytes, hipMemcpyDeviceToHost, st));
+                off += (d.bytes + 255) & ~size_t(255);
+            }
+            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(hipGetLastError());
+            off = tables_bytes;
+            for (const Download &d : downloads) {
+                if (d.bytes) std::memcpy(d.host, ex->staging.ptr + off, d.bytes);
+                off += (d.bytes + 255) & ~size_t(255);
+            }
         }
-        if (counts_out) {
-            counts_h.resize(count_slots);
-            HIP_TRY(hipMemcpyAsync(counts_h.data(), ex->counts.ptr, count_slots * 4, hipMemcpyDeviceToHost, st));
-        }
-        HIP_TRY(hipStreamSynchronize(st));
-        HIP_TRY(hipGetLastError());
         if (ex->profiling) ex->collect();
         // ---- failures, in the reference's processing order: groups in order; within a group the
         // training simulation, then strike rows' fits in row order, then the evaluation simulation,

@@ -144,6 +144,28 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     if (!ok) atomicMin(fail + item.x, static_cast<unsigned long long>(p));
 }
 
+// ------------------------------------------------------------------------------- fills
+
+// blockIdx.y selects the region; the x blocks stride over its words
+__global__ __launch_bounds__(kBlock) void k_fill(FillList fills) {
+    const FillRegion f = fills.region[blockIdx.y];
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < f.words;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock)
+        f.ptr[i] = f.value;
+}
+
+hipError_t launch_fill(hipStream_t st, const FillList &fills) {
+    if (fills.count <= 0) return hipSuccess;
+    if (fills.count > kMaxFills) return hipErrorInvalidValue;
+    uint64_t most = 0;
+    for (int i = 0; i < fills.count; ++i) most = most > fills.region[i].words ? most : fills.region[i].words;
+    if (most == 0) return hipSuccess;
+    const uint64_t blocks = (most + kBlock - 1) / kBlock;
+    const dim3 grid(static_cast<uint32_t>(blocks < 512 ? blocks : 512), static_cast<uint32_t>(fills.count));
+    hipLaunchKernelGGL(k_fill, grid, dim3(kBlock), 0, st, fills);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------- fit passes
 
 __device__ __forceinline__ bool row_dead(const RowError *err, uint32_t s) { return err[s].key != kNoError; }

@@ -18,6 +18,21 @@ struct FitParams {
     double cond_limit;
 };
 
+// Word fills of up to kMaxFills device regions in one launch (the per-pricing initialisation of
+// rules, counters and failure keys); each region is `words` 32-bit words set to `value`.
+constexpr int kMaxFills = 8;
+struct FillRegion {
+    uint32_t *ptr;
+    uint64_t words;
+    uint32_t value;
+    uint32_t pad;
+};
+struct FillList {
+    FillRegion region[kMaxFills];
+    int32_t count;
+};
+hipError_t launch_fill(hipStream_t st, const FillList &fills);
+
 hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
                                 uint32_t paths, uint32_t blocks, uint32_t *out);
 hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
