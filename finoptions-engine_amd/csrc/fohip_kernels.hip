@@ -706,9 +706,14 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
 // decisions, which backward induction then amplifies.
 // Block = (2048-path chunk, set of up to 8 strikes): every thread keeps its 8 paths' spot values
 // of rows r and r + 1 in registers for all strikes of the set; each wave reduces its sums through
-// an LDS transpose, and one barrier at the end combines the waves in fixed order.
+// an LDS transpose, and one barrier at the end combines the waves in fixed order. The kernel is
+// held to 128 VGPRs (4 waves per SIMD; 33.5 KB of LDS lets 4 blocks share a CU), so a C2 step's
+// 4096 blocks run in 4 full rounds of the 1024 block slots.
+__device__ __noinline__ void record_fit_errors(RowError *e, uint32_t step, uint32_t base, uint32_t n,
+                                               uint32_t bad_bits, uint32_t nonfinite_bits);
+
 template <typename Store>
-__global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
+__global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
                                                           const uint2 *__restrict__ items, uint32_t nitems,
                                                           uint32_t sets, const Store *__restrict__ paths,
                                                           Store *__restrict__ values, const Model *__restrict__ models,
@@ -815,12 +820,13 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
 #pragma unroll
             for (int k = 0; k < kBulkPathsPerThread; ++k) any = any || bad[k] || !isfinite(v[k]);
             if (__ballot(any) != 0ull) {
+                uint32_t bad_bits = 0, nonfinite_bits = 0;
 #pragma unroll
                 for (int k = 0; k < kBulkPathsPerThread; ++k) {
-                    const uint32_t p = base + k * kBlock;
-                    if (bad[k]) record_error(&errors[srow], error_key(step - 1, kStageApply, p));
-                    if (p < n && !isfinite(v[k])) record_error(&errors[srow], error_key(step, kStageDiscount, p));
+                    bad_bits |= (bad[k] ? 1u : 0u) << k;
+                    nonfinite_bits |= (!isfinite(v[k]) ? 1u : 0u) << k;
                 }
+                record_fit_errors(&errors[srow], step, base, n, bad_bits, nonfinite_bits);
             }
             // pivoted power sums over in-the-money paths: an out-of-the-money lane adds exact zeros
 #pragma unroll
@@ -848,6 +854,17 @@ __global__ __launch_bounds__(kBlock) void k_fit_bulk_step(DevTables t, FitParams
         const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
         const uint32_t local = s_first + i - strike_begin;
         partials[static_cast<uint64_t>(g->part_base_t + local * g->chunks_t + item.y) * kPartialStride + q] = v;
+    }
+}
+
+// Rare path of the fused fit pass (non-finite predictions or cashflows), kept out of line so that
+// its atomics weigh neither on the register allocation nor on the code of the hot loop.
+__device__ __noinline__ void record_fit_errors(RowError *e, uint32_t step, uint32_t base, uint32_t n,
+                                               uint32_t bad_bits, uint32_t nonfinite_bits) {
+    for (int k = 0; k < kBulkPathsPerThread; ++k) {
+        const uint32_t p = base + k * kBlock;
+        if ((bad_bits >> k) & 1u) record_error(e, error_key(step - 1, kStageApply, p));
+        if (p < n && ((nonfinite_bits >> k) & 1u)) record_error(e, error_key(step, kStageDiscount, p));
     }
 }
 
@@ -1097,8 +1114,20 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_origin(DevTables t, FitPa
 // with its result discarded), so rules and tables are read with scalar loads.
 // With GREEKS, tangents of S w.r.t. (S0, sigma, r, q) ride along the transitions and each
 // strike's stopping row is held fixed (pathwise estimator under the frozen policy).
+// Rare path of the evaluation (a non-finite prediction or payoff), out of line.
+__device__ __noinline__ void record_eval_errors(RowError *e, uint32_t row, uint32_t p, bool bad, bool payoff_bad) {
+    if (bad) record_error(e, error_key(kEvalStepBase + row, kStageEvalPrediction, p));
+    if (payoff_bad) record_error(e, error_key(kEvalStepBase + row, kStageEvalPayoff, p));
+}
+
+// Waves per SIMD asked of an instantiation. The fused Greek evaluation over 8-strike tiles of
+// stored normals runs at 5 (96 VGPRs, a few spilled) rather than 4: the extra wave hides more of
+// the decision chain than the spills cost. Every other form keeps the compiler's own choice.
+template <bool FAST, int TS, bool GREEKS, bool STORED>
+constexpr int eval_waves() { return (FAST && GREEKS && STORED && TS == 8) ? 5 : 1; }
+
 template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED>
-__global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
+__global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
                                                      uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
@@ -1241,10 +1270,9 @@ __global__ __launch_bounds__(kBlock) void k_evaluate(DevTables t, FitParams fp, 
                         dpay[i][3] = ex ? od3 : dpay[i][3];
                     }
                     active &= ~(((ex || bad) ? 1u : 0u) << i);
-                    if (bad)
-                        record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPrediction, p));
-                    if (ex && !isfinite(pv))
-                        record_error(&errors[s_first + i], error_key(kEvalStepBase + row, kStageEvalPayoff, p));
+                    const bool payoff_bad = ex && !isfinite(pv);
+                    if (__ballot(bad || payoff_bad) != 0ull)
+                        record_eval_errors(&errors[s_first + i], row, p, bad, payoff_bad);
                     const unsigned long long exm = __ballot(ex);
                     if (exm != 0ull && lane == 0) {
                         const uint32_t c = static_cast<uint32_t>(__popcll(exm));
