@@ -910,9 +910,10 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         bool escrow = false;   // the escrowed instantiation also serves spot-model groups
         for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
         // Several strike tiles read one set of evaluation normals from HBM instead of each
-        // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams).
+        // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams). Single-
+        // strike tiles always regenerate: that form has no stored-normal kernel.
         const double *znorm = nullptr;
-        if (spec.fused && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0) {
+        if (spec.fused && tile > 1 && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0) {
             HIP_TRY(ex->znorm.ensure(znorm_elems * sizeof(double)));
             PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ite,
                                                                static_cast<uint32_t>(items_e.size()),

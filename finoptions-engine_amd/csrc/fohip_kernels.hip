@@ -1126,7 +1126,23 @@ __device__ __noinline__ void record_eval_errors(RowError *e, uint32_t row, uint3
 template <bool FAST, int TS, bool GREEKS, bool STORED>
 constexpr int eval_waves() { return (FAST && GREEKS && STORED && TS == 8) ? 5 : 1; }
 
-template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED>
+// Decision data of one (row, strike) as staged in LDS: K, a, b, hz0..hz4.
+constexpr int kDecDoubles = 8;
+
+// The fast continuation from a staged entry: the same operations, in the same order, as
+// continuation_fast on the rule itself.
+__device__ __forceinline__ double continuation_staged(const double *d, double spot) {
+    const double z = fma(spot, d[1], d[2]);
+    double c = fma(d[7], z, d[6]);
+    c = fma(c, z, d[5]);
+    c = fma(c, z, d[4]);
+    return fma(c, z, d[3]);
+}
+
+// DEC: the tile's decision data (kDecDoubles per (row, strike), [row][strike]) and each row's rule
+// kinds (4 bits per strike) are staged in LDS after the exercise histogram, so that a decision
+// reads LDS broadcasts instead of waiting on scalar loads of rules that lie obs rules apart.
+template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED, bool DEC>
 __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
@@ -1134,7 +1150,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                                                      unsigned long long *__restrict__ sim_fail, int use_lds_hist,
                                                      const double *__restrict__ znorm) {
     using Tan = typename std::conditional<FAST, float, double>::type;
-    extern __shared__ uint32_t hist[];   // [TS][obs] when use_lds_hist
+    extern __shared__ __align__(16) uint32_t hist[];   // [TS][obs] when use_lds_hist, then DEC's tables
     __shared__ double sx[257], sf[257];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
@@ -1157,6 +1173,29 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     if (!STORED) load_zig(sx, sf);
     if (use_lds_hist) {
         for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
+        __syncthreads();
+    }
+    double *dec = nullptr;
+    uint32_t *dec_kind = nullptr;
+    if constexpr (DEC) {
+        static_assert(TS <= 8, "rule kinds are packed 4 bits per strike into one word per row");
+        dec = reinterpret_cast<double *>(hist + ((TS * obs + 3u) & ~3u));
+        dec_kind = reinterpret_cast<uint32_t *>(dec + static_cast<size_t>(obs) * TS * kDecDoubles);
+        for (uint32_t e = threadIdx.x; e < static_cast<uint32_t>(ns) * obs; e += kBlock) {
+            const uint32_t i = e / obs, r = e - i * obs;
+            const Model *m = tile_models + i * obs + r;
+            double *d = dec + (static_cast<size_t>(r) * TS + i) * kDecDoubles;
+            d[0] = m->K;
+            d[1] = m->a;
+            d[2] = m->b;
+#pragma unroll
+            for (int k = 0; k <= kMaxDegree; ++k) d[3 + k] = m->hz[k];
+        }
+        for (uint32_t r = threadIdx.x; r < obs; r += kBlock) {
+            uint32_t word = 0;
+            for (int i = 0; i < ns; ++i) word |= (static_cast<uint32_t>(tile_models[i * obs + r].kind) & 15u) << (4 * i);
+            dec_kind[r] = word;
+        }
         __syncthreads();
     }
     uint32_t live = 0;
@@ -1243,18 +1282,22 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             if (GREEKS) {   // formed once per row: keep the compiler from recomputing them per strike
                 asm volatile("" : "+v"(od0), "+v"(od1), "+v"(od3), "+v"(od2));
             }
+            const uint32_t kinds_row = DEC ? __builtin_amdgcn_readfirstlane(dec_kind[row]) : 0u;
+            const double *dec_row = DEC ? dec + static_cast<size_t>(row) * TS * kDecDoubles : nullptr;
 #pragma unroll
             for (int i = 0; i < TS; ++i) {
                 if (i >= ns) break;
-                const FOHIP_UNIFORM Model *m = uniform(tile_models + i * obs + row);
-                const int kind = m->kind;   // wave-uniform
-                const double K = m->K;      // set on every fitted row
+                const FOHIP_UNIFORM Model *m = DEC ? nullptr : uniform(tile_models + i * obs + row);
+                const double *d = DEC ? dec_row + i * kDecDoubles : nullptr;
+                const int kind = DEC ? static_cast<int>((kinds_row >> (4 * i)) & 15u) : m->kind;   // wave-uniform
+                const double K = DEC ? d[0] : m->K;   // set on every fitted row
                 const bool cand = ((active >> i) & 1u) && S > K;
                 // branch-free: the continuation is formed for every row and used only where a
                 // regression rule exists
                 const bool reg = kind == kRuleRegression;
                 const double intr = S - K;
-                const double cont = FAST ? continuation_fast(m, S) : predict_ref(m, degree, S / K);
+                const double cont = DEC ? continuation_staged(d, S)
+                                        : (FAST ? continuation_fast(m, S) : predict_ref(m, degree, S / K));
                 const bool finite = isfinite(cont);
                 const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
                 const bool bad = reg && cand && !finite;
@@ -1708,7 +1751,16 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+    // staged decision tables for the multi-tile fused forms when they fit beside the histogram
+    const size_t dec_bytes = static_cast<size_t>(TS) * max_obs * kDecDoubles * sizeof(double) + max_obs * sizeof(uint32_t);
+    const size_t hist_padded = (hist_bytes + 15) & ~size_t(15);
+    if (Z && TS <= 8 && use_lds && hist_padded + dec_bytes <= 49152) {
+        hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, (Z && TS <= 8)>), dim3(xcd_blocks(nitems, tiles)),
+                           dim3(kBlock), hist_padded + dec_bytes, st, t, fp, items, nitems, tiles, models, partials,
+                           counts, nosupport, errors, sim_fail, use_lds, znorm);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, false>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
                        errors, sim_fail, use_lds, znorm);
     return hipGetLastError();

@@ -238,6 +238,34 @@ def test_results_are_deterministic_and_tile_invariant(hip_f64):
         assert np.array_equal(o.expiry_paths, outs[0].expiry_paths)
 
 
+@pytest.mark.parametrize("intervals", [10, 120])
+def test_fused_results_are_tile_invariant(hip_fused, intervals):
+    """The fused evaluation has three forms: single-strike tiles regenerate the normals from the
+    streams; wider tiles read normals drawn once per pricing and stage the tile's decision data in
+    LDS when it fits (8 strikes x 121 rows does not, 4 x 121 does). Every form gives the same
+    bits, with and without Greeks."""
+    import finoptions_hip as fh
+
+    cols = c2_columns(strikes=16)
+    groups = hip_groups(cols)
+    mesh = fh.UniformExerciseMesh(base_intervals=intervals, origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = hip_method(8192, 8192, 23, support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    for greeks, tiles in (((fh.Greek.DELTA, fh.Greek.VEGA), (0, 1, 4, 8)), ((), (0, 1, 4, 16))):
+        outs = []
+        for tile in tiles:
+            hip_fused.tune(eval_strike_tile=tile)
+            outs.append(fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip_fused,
+                                                        method=method, greeks=greeks))
+        hip_fused.tune(eval_strike_tile=0)
+        for tile, o in zip(tiles[1:], outs[1:]):
+            assert np.array_equal(o.prices.view(np.uint64), outs[0].prices.view(np.uint64)), tile
+            assert np.array_equal(o.standard_errors, outs[0].standard_errors), tile
+            assert np.array_equal(o.early_exercise_paths, outs[0].early_exercise_paths), tile
+            assert np.array_equal(o.expiry_paths, outs[0].expiry_paths), tile
+            for g in greeks:
+                assert np.array_equal(o.greeks[g], outs[0].greeks[g]), (tile, g)
+
+
 def test_fused_method_matches_the_oracle(hip_fused, oracle):
     """One fused pass per node (pivoted power sums, moment-form Stieltjes solve) over binary64
     spots and cashflows: the fitted rules agree with the reference's to ~1e-13, so exercise
