@@ -134,7 +134,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--paths", type=int, default=1 << 20)
-    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 17)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
