@@ -287,10 +287,11 @@ struct fohip_executor {
     hipStream_t stream = nullptr;
     char name[256] = {0};
     fohip_tuning tuning = {0, 0};
+    double near_tie_window = 0.0;   // 0: exact ties; w > 0: |intrinsic - continuation| <= w |continuation|
     size_t batch_budget = 0;   // bytes for paths + values per batch
     std::mutex mu;
     DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, eval_out,
-        summary, replay, probe_a, probe_b, probe_c, znorm;
+        summary, replay, probe_a, probe_b, probe_c, znorm, ties;
     PinnedBuf staging;   // per-batch tables and items on the way in, results on the way out
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
@@ -419,7 +420,8 @@ extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     (void)hipStreamSynchronize(ex->stream);
     for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
                       &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t,
-                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm})
+                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm,
+                      &ex->ties})
         b->release();
     ex->staging.release();
     for (auto &p : ex->pending) {
@@ -470,6 +472,13 @@ extern "C" int fohip_set_tuning(fohip_executor *ex, const fohip_tuning *t) {
     return FOHIP_OK;
 }
 
+extern "C" int fohip_set_near_tie_window(fohip_executor *ex, double window) {
+    if (!ex || !std::isfinite(window) || window < 0.0 || window > 1.0) return FOHIP_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ex->mu);
+    ex->near_tie_window = window;
+    return FOHIP_OK;
+}
+
 // ============================================================================ group plans
 
 namespace {
@@ -510,6 +519,7 @@ struct RowOut {
     bool has_sd = false;
     RowSummary sum{};
     double greek[4] = {0, 0, 0, 0}, greek_se[4] = {0, 0, 0, 0};
+    uint64_t ties_training = 0, ties_evaluation = 0;   // near-tie rule applications
 };
 
 struct GroupTables {
@@ -802,6 +812,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
         HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
         HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
+        HIP_TRY(ex->ties.ensure(std::max<uint32_t>(nrows, 1) * 2 * sizeof(unsigned long long)));
         char *tb = ex->tables.as<char>();
         const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
                                     gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
@@ -821,6 +832,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         std::vector<RowSummary> summary(nrows);
         std::vector<Model> models_h(models_out ? model_count : 0);
         std::vector<uint32_t> counts_h(counts_out ? count_slots : 0);
+        std::vector<unsigned long long> ties_h(static_cast<size_t>(nrows) * 2);   // [training rows | evaluation rows]
         const Download downloads[] = {
             {fail_t.data(), ex->fail_t.ptr, ngroups * sizeof(unsigned long long)},
             {fail_e.data(), ex->fail_e.ptr, ngroups * sizeof(unsigned long long)},
@@ -828,7 +840,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             {eval_out.data(), ex->eval_out.ptr, eval_out.size() * sizeof(double)},
             {summary.data(), ex->summary.ptr, nrows * sizeof(RowSummary)},
             {models_h.data(), ex->models.ptr, models_h.size() * sizeof(Model)},
-            {counts_h.data(), ex->counts.ptr, counts_h.size() * sizeof(uint32_t)}};
+            {counts_h.data(), ex->counts.ptr, counts_h.size() * sizeof(uint32_t)},
+            {ties_h.data(), ex->ties.ptr, ties_h.size() * sizeof(unsigned long long)}};
         size_t staged = tables_bytes;
         for (const Download &d : downloads) staged += (d.bytes + 255) & ~size_t(255);
         // one upload of every table, item list and row list of the batch
@@ -871,10 +884,14 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         add_fill(ex->errors.ptr, nrows * sizeof(RowError), 0xFFFFFFFFu);
         add_fill(ex->fail_t.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
         add_fill(ex->fail_e.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(ex->ties.ptr, ties_h.size() * sizeof(unsigned long long), 0u);
         HIP_TRY(launch_fill(st, fills));
+        unsigned long long *ties_t = ex->ties.as<unsigned long long>();
+        unsigned long long *ties_e = ties_t + nrows;
         Model *models = ex->models.as<Model>();
         RowError *errors = ex->errors.as<RowError>();
-        const FitParams &fp = spec.fp;
+        FitParams fp = spec.fp;
+        fp.tie_window = ex->near_tie_window;
         // ---- training simulation
         PROF_LAUNCH("k_simulate", launch_simulate(st, f32, t, it, static_cast<uint32_t>(items_t.size()), ex->paths.ptr, 0,
                                 ex->fail_t.as<unsigned long long>()));
@@ -884,13 +901,13 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             if (spec.fused) {   // one fused HBM pass per backward step
                 PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, ex->paths.ptr,
                                                                     ex->values.ptr, models, ex->part_t.as<double>(),
-                                                                    errors));
+                                                                    errors, ties_t));
                 PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
                                                                         models, errors));
                 continue;
             }
             PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
-                                       ex->part_t.as<double>(), errors));
+                                       ex->part_t.as<double>(), errors, ties_t));
             PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
             for (int k = 0; k <= fp.degree; ++k) {
                 PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, t, fp, j, k, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr,
@@ -902,11 +919,12 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             PROF_LAUNCH("k_fit_gate", launch_fit_gate(st, t, fp, rows, nrows, models, static_cast<uint32_t>(model_count),
                                                       errors));
         PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
-                                 ex->values.ptr, models, ex->part_t.as<double>(), errors));
+                                 ex->values.ptr, models, ex->part_t.as<double>(), errors, ties_t));
         PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));
         // ---- evaluation
         const bool greeks = spec.greeks != 0;
-        const uint32_t tile = eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
+        // windowed near-tie counts run the evaluation's diagnostic form over single-strike tiles
+        const uint32_t tile = fp.tie_window != 0.0 ? 1u : eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
         bool escrow = false;   // the escrowed instantiation also serves spot-model groups
         for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
         // Several strike tiles read one set of evaluation normals from HBM instead of each
@@ -924,7 +942,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ite,
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
-                                ex->fail_e.as<unsigned long long>(), znorm));
+                                ex->fail_e.as<unsigned long long>(), znorm, ties_e));
         PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
         PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
                                    ex->summary.as<RowSummary>()));
@@ -1047,6 +1065,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             const GroupPlan &P = plans[g];
             RowOut &o = out[row0 + s];
             o.sum = summary[s];
+            o.ties_training = ties_h[s];
+            o.ties_evaluation = ties_h[nrows + s];
             const double n = static_cast<double>(P.ne);
             if (summary[s].origin_fitted && summary[s].origin_exercised) {
                 // policy.rs:672-682: deterministic origin exercise
@@ -1215,6 +1235,8 @@ extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request
         res->greeks[a] = o.greek[a];
         res->greek_standard_errors[a] = o.greek_se[a];
     }
+    res->near_tie_training_decisions = o.ties_training;
+    res->near_tie_evaluation_decisions = o.ties_evaluation;
     if (res->diagnostics) {
         for (uint32_t r = 0; r + 1 < obs; ++r) {
             const Model &m = models[r];
@@ -1698,6 +1720,8 @@ extern "C" int fohip_price_lsm_grouped(fohip_executor *ex, const fohip_grouped_r
         if (res->maximum_regression_condition_numbers) res->maximum_regression_condition_numbers[r] = o.sum.max_condition;
         if (res->early_exercise_paths) res->early_exercise_paths[r] = o.sum.early;
         if (res->expiry_paths) res->expiry_paths[r] = o.sum.expiry;
+        if (res->near_tie_training_decisions) res->near_tie_training_decisions[r] = o.ties_training;
+        if (res->near_tie_evaluation_decisions) res->near_tie_evaluation_decisions[r] = o.ties_evaluation;
         double *cols[4] = {res->delta, res->vega, res->rho, res->dividend_yield};
         double *secols[4] = {res->delta_se, res->vega_se, res->rho_se, res->dividend_yield_se};
         for (int a = 0; a < 4; ++a) {

@@ -170,6 +170,26 @@ hipError_t launch_fill(hipStream_t st, const FillList &fills) {
 
 __device__ __forceinline__ bool row_dead(const RowError *err, uint32_t s) { return err[s].key != kNoError; }
 
+// A rule application is a tie when intrinsic == continuation: the tie rule settles it, and the
+// other rule would decide it the other way. With a window w > 0 the counts take every near tie,
+// |intrinsic - continuation| <= w |continuation|, the decisions a last-bit difference in the
+// continuation (another summation order, another form of the same polynomial) could flip.
+// Counted per strike row, training and evaluation apart, and reported with the results. The
+// exact test is the decision's own equality comparison; the windowed one is a diagnostic form
+// (template argument WIN of the fused fit step and of the evaluation).
+__device__ __forceinline__ bool near_tie(double intrinsic, double continuation, double window) {
+    return fabs(intrinsic - continuation) <= window * fabs(continuation);
+}
+
+__device__ __forceinline__ bool is_tie(double intrinsic, double continuation, double window) {
+    return window == 0.0 ? intrinsic == continuation : near_tie(intrinsic, continuation, window);
+}
+
+// Rare path: near ties are counted with one atomic per wave (or lane) that has any.
+__device__ __noinline__ void add_ties(unsigned long long *counter, uint32_t c) {
+    atomicAdd(counter, static_cast<unsigned long long>(c));
+}
+
 template <typename Real>
 __device__ __forceinline__ Real intrinsic(Real spot, Real strike) {
     const Real v = spot - strike;
@@ -189,7 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams f
                                                         const uint2 *__restrict__ items, uint32_t nitems,
                                                         uint32_t columns, const double *__restrict__ paths,
                                                         double *__restrict__ values, const Model *__restrict__ models,
-                                                        double *__restrict__ partials, RowError *__restrict__ errors) {
+                                                        double *__restrict__ partials, RowError *__restrict__ errors,
+                                                        unsigned long long *__restrict__ ties) {
     __shared__ double scratch[3 * kWaves];
     uint32_t wi, local;
     if (!xcd_item_column(nitems, columns, wi, local)) return;
@@ -208,6 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams f
     const bool apply = step > 0 && m_next.kind == kRuleRegression;
     const double disc = t.disc_step[g.row_base + r];
     Moments acc = {0.0, 0.0, 0.0};
+    uint32_t tie_count = 0;
 #pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
@@ -223,8 +245,9 @@ __global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams f
                     const double cont = predict(m_next, fp.degree, s1 / K);
                     if (!isfinite(cont)) {
                         record_error(&errors[srow], error_key(step - 1, kStageApply, p));
-                    } else if (should_exercise(fp.tie, s1 - K, cont)) {
-                        v = s1 - K;
+                    } else {
+                        tie_count += is_tie(s1 - K, cont, fp.tie_window) ? 1u : 0u;
+                        if (should_exercise(fp.tie, s1 - K, cont)) v = s1 - K;
                     }
                 }
             }
@@ -239,6 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_prepare(DevTables t, FitParams f
             welford_push(acc, x);
         }
     }
+    if (tie_count != 0u) add_ties(ties + srow, tie_count);
     acc = block_chan(acc, scratch);
     if (threadIdx.x == 0) {
         double *out = partial_slot(partials, g, local, item.y);
@@ -712,13 +736,14 @@ __global__ __launch_bounds__(kBlock) void k_fit_reduce_moments(DevTables t, FitP
 __device__ __noinline__ void record_fit_errors(RowError *e, uint32_t step, uint32_t base, uint32_t n,
                                                uint32_t bad_bits, uint32_t nonfinite_bits);
 
-template <typename Store>
+template <typename Store, bool WIN>
 __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitParams fp, uint32_t step,
                                                           const uint2 *__restrict__ items, uint32_t nitems,
                                                           uint32_t sets, const Store *__restrict__ paths,
                                                           Store *__restrict__ values, const Model *__restrict__ models,
                                                           double *__restrict__ partials,
-                                                          RowError *__restrict__ errors) {
+                                                          RowError *__restrict__ errors,
+                                                          unsigned long long *__restrict__ ties) {
     __shared__ double red[kWaves][kBulkStrikes][16];
     __shared__ double transpose[kWaves][14 * 66];
     uint32_t wi, set;
@@ -778,6 +803,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
         const double pivot = apply ? mn->mean : 1.0;
         const double K = *uniform(t.strikes + srow);
         const double invK = *uniform(t.inv_strikes + srow);
+        uint32_t tie_wave = 0;   // near ties of the wave at this strike, counted with its ballots
         if (alive & (1u << i)) {
             Store *V = values + value_base + static_cast<uint64_t>(local) * n;
             // Straight-line slots: lanes past n hold zero spots, which are never in the money and
@@ -803,6 +829,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
                     const bool cand = s1 > K;
                     const bool finite = isfinite(cont);
                     const bool ex = cand && finite && should_exercise(fp.tie, s1 - K, cont);
+                    const bool tie = WIN ? near_tie(s1 - K, cont, fp.tie_window) : (s1 - K) == cont;
+                    tie_wave += static_cast<uint32_t>(__popcll(__ballot(cand && finite && tie)));
                     const Store vf = static_cast<Store>((s1 - K) * d_next);
                     if (ex) V[p] = vf;
                     v[k] = ex ? static_cast<double>(vf) : static_cast<double>(vv[k]);
@@ -846,9 +874,17 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
         }
         const double total = wave_sums_lds<14>(acc, transpose[wave]);
         if ((lane & 3) == 0 && lane < 56) red[wave][i][lane >> 2] = total;
-        if (lane == 0) red[wave][i][14] = pivot;
+        if (lane == 0) {
+            red[wave][i][14] = pivot;
+            red[wave][i][15] = static_cast<double>(tie_wave);
+        }
     }
     __syncthreads();
+    if (threadIdx.x < static_cast<uint32_t>(ns)) {
+        const int i = static_cast<int>(threadIdx.x);
+        const double c = (red[0][i][15] + red[1][i][15]) + (red[2][i][15] + red[3][i][15]);
+        if (c != 0.0) atomicAdd(ties + s_first + i, static_cast<unsigned long long>(c));
+    }
     for (int idx = threadIdx.x; idx < ns * 15; idx += kBlock) {
         const int i = idx / 15, q = idx % 15;
         const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
@@ -1032,7 +1068,8 @@ template <typename Real, bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                       uint32_t nitems, uint32_t columns, const Real *__restrict__ paths,
                                                       const Real *__restrict__ values, const Model *__restrict__ models,
-                                                      double *__restrict__ partials, RowError *__restrict__ errors) {
+                                                      double *__restrict__ partials, RowError *__restrict__ errors,
+                                                      unsigned long long *__restrict__ ties) {
     constexpr bool kBulk = FUSED;
     __shared__ double scratch[kWaves];
     uint32_t wi, local;
@@ -1053,6 +1090,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
     const double disc = stored_at_origin ? 1.0 : g.disc_final;
     const double d_last = stored_at_origin ? t.disc0[g.row_base + last_row] : 1.0;
     double sum[1] = {0.0};
+    uint32_t tie_count = 0;
 #pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
@@ -1067,8 +1105,10 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
                 const double cont = kBulk ? continuation_fast(&m, s1) : predict_ref(&m, fp.degree, s1 / K);
                 if (!isfinite(cont)) {
                     record_error(&errors[srow], error_key(g.fit_rows - 1, kStageApply, p));
-                } else if (should_exercise(fp.tie, s1 - K, cont)) {
-                    v = kBulk ? static_cast<double>(static_cast<Real>((s1 - K) * d_last)) : s1 - K;
+                } else {
+                    tie_count += is_tie(s1 - K, cont, fp.tie_window) ? 1u : 0u;
+                    if (should_exercise(fp.tie, s1 - K, cont))
+                        v = kBulk ? static_cast<double>(static_cast<Real>((s1 - K) * d_last)) : s1 - K;
                 }
             }
         }
@@ -1076,6 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
         if (!isfinite(v)) record_error(&errors[srow], error_key(g.fit_rows, kStageDiscount, p));
         sum[0] += v;
     }
+    if (tie_count != 0u) add_ties(ties + srow, tie_count);
     block_sum<1>(sum, scratch);
     if (threadIdx.x == 0 && g.origin_row) partial_slot(partials, g, local, item.y)[0] = sum[0];
 }
@@ -1142,18 +1183,20 @@ __device__ __forceinline__ double continuation_staged(const double *d, double sp
 // DEC: the tile's decision data (kDecDoubles per (row, strike), [row][strike]) and each row's rule
 // kinds (4 bits per strike) are staged in LDS after the exercise histogram, so that a decision
 // reads LDS broadcasts instead of waiting on scalar loads of rules that lie obs rules apart.
-template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED, bool DEC>
+template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED, bool DEC, bool WIN>
 __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
                                                      uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
                                                      unsigned long long *__restrict__ sim_fail, int use_lds_hist,
-                                                     const double *__restrict__ znorm) {
+                                                     const double *__restrict__ znorm,
+                                                     unsigned long long *__restrict__ ties) {
     using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ __align__(16) uint32_t hist[];   // [TS][obs] when use_lds_hist, then DEC's tables
     __shared__ double sx[257], sf[257];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
+    __shared__ uint32_t tie_lds[TS];   // near ties of this tile, flushed once at the end
     uint32_t wi, tile;
     if (!xcd_item_column(nitems, tiles, wi, tile)) return;
     const uint2 item = load_item(items, wi);
@@ -1171,6 +1214,8 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     const Model *tile_models = models + g->model_base + local0 * obs;   // [strike][row]
     const int degree = fp.degree, tie = fp.tie;
     if (!STORED) load_zig(sx, sf);
+    if (threadIdx.x < TS) tie_lds[threadIdx.x] = 0;
+    __syncthreads();
     if (use_lds_hist) {
         for (uint32_t i = threadIdx.x; i < TS * obs; i += kBlock) hist[i] = 0;
         __syncthreads();
@@ -1301,9 +1346,12 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                 const bool finite = isfinite(cont);
                 const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
                 const bool bad = reg && cand && !finite;
-                // state changes only where a lane stops: one wave-uniform branch
-                const unsigned long long stopm = __ballot(ex || bad);
+                const bool tied = reg && cand && finite && (WIN ? near_tie(intr, cont, fp.tie_window) : intr == cont);
+                // state changes only where a lane stops (or a tie is counted): one wave-uniform branch
+                const unsigned long long stopm = __ballot(ex || bad || tied);
                 if (stopm != 0ull) {
+                    const unsigned long long tm = __ballot(tied);
+                    if (tm != 0ull && lane == 0) atomicAdd(&tie_lds[i], static_cast<uint32_t>(__popcll(tm)));
                     const double pv = intr * disc;
                     pay[i] = ex ? pv : pay[i];
                     if (GREEKS) {
@@ -1370,6 +1418,9 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             }
         }
     }
+    __syncthreads();
+    if (threadIdx.x < static_cast<uint32_t>(ns) && tie_lds[threadIdx.x] != 0u)
+        atomicAdd(ties + s_first + threadIdx.x, static_cast<unsigned long long>(tie_lds[threadIdx.x]));
     // per-strike block reduction of (sum v, sum v^2[, sum dv_a, sum dv_a^2])
     const bool contributes_any = p < n && ok;
 #pragma unroll
@@ -1645,11 +1696,12 @@ hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const u
 
 hipError_t launch_fit_prepare(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                               const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                              void *values, const Model *models, double *partials, RowError *errors) {
+                              void *values, const Model *models, double *partials, RowError *errors,
+                              unsigned long long *ties) {
     if (nitems == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_prepare, dim3(xcd_blocks(nitems, max_strikes)), dim3(kBlock), 0, st, t, fp, step, items,
                        nitems, max_strikes, static_cast<const double *>(paths), static_cast<double *>(values), models,
-                       partials, errors);
+                       partials, errors, ties);
     return hipGetLastError();
 }
 
@@ -1683,17 +1735,23 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
 
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors) {
+                                void *values, const Model *models, double *partials, RowError *errors,
+                                unsigned long long *ties) {
     if (nitems == 0) return hipSuccess;
     const uint32_t sets = (max_strikes + kBulkStrikes - 1) / kBulkStrikes;
     const dim3 grid(xcd_blocks(nitems, sets));
-    if (f32)
-        hipLaunchKernelGGL(k_fit_bulk_step<float>, grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets,
-                           static_cast<const float *>(paths), static_cast<float *>(values), models, partials, errors);
-    else
-        hipLaunchKernelGGL(k_fit_bulk_step<double>, grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets,
-                           static_cast<const double *>(paths), static_cast<double *>(values), models, partials,
-                           errors);
+    const bool win = fp.tie_window != 0.0;
+#define FOHIP_BULK(REAL, WINV)                                                                                  \
+    hipLaunchKernelGGL((k_fit_bulk_step<REAL, WINV>), grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets, \
+                       static_cast<const REAL *>(paths), static_cast<REAL *>(values), models, partials, errors, ties)
+    if (f32) {
+        if (win) FOHIP_BULK(float, true);
+        else FOHIP_BULK(float, false);
+    } else {
+        if (win) FOHIP_BULK(double, true);
+        else FOHIP_BULK(double, false);
+    }
+#undef FOHIP_BULK
     return hipGetLastError();
 }
 
@@ -1707,21 +1765,22 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
 
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                            const void *values, const Model *models, double *partials, RowError *errors) {
+                            const void *values, const Model *models, double *partials, RowError *errors,
+                            unsigned long long *ties) {
     if (nitems == 0) return hipSuccess;
     const dim3 grid(xcd_blocks(nitems, max_strikes));
     if (f32)
         hipLaunchKernelGGL((k_fit_final<float, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors);
+                           errors, ties);
     else if (fused)
         hipLaunchKernelGGL((k_fit_final<double, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
-                           errors);
+                           errors, ties);
     else
         hipLaunchKernelGGL((k_fit_final<double, false>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
                            static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
-                           errors);
+                           errors, ties);
     return hipGetLastError();
 }
 
@@ -1743,11 +1802,11 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     return tile;
 }
 
-template <bool FAST, int TS, bool G, bool E, bool Z>
+template <bool FAST, int TS, bool G, bool E, bool Z, bool WIN = false>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
-                                unsigned long long *sim_fail, const double *znorm) {
+                                unsigned long long *sim_fail, const double *znorm, unsigned long long *ties) {
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
@@ -1755,14 +1814,14 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const size_t dec_bytes = static_cast<size_t>(TS) * max_obs * kDecDoubles * sizeof(double) + max_obs * sizeof(uint32_t);
     const size_t hist_padded = (hist_bytes + 15) & ~size_t(15);
     if (Z && TS <= 8 && use_lds && hist_padded + dec_bytes <= 49152) {
-        hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, (Z && TS <= 8)>), dim3(xcd_blocks(nitems, tiles)),
+        hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, (Z && TS <= 8), WIN>), dim3(xcd_blocks(nitems, tiles)),
                            dim3(kBlock), hist_padded + dec_bytes, st, t, fp, items, nitems, tiles, models, partials,
-                           counts, nosupport, errors, sim_fail, use_lds, znorm);
+                           counts, nosupport, errors, sim_fail, use_lds, znorm, ties);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, false>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
+    hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, false, WIN>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
-                       errors, sim_fail, use_lds, znorm);
+                       errors, sim_fail, use_lds, znorm, ties);
     return hipGetLastError();
 }
 
@@ -1770,10 +1829,20 @@ template <bool FAST>
 static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
-                                RowError *errors, unsigned long long *sim_fail, const double *znorm) {
+                                RowError *errors, unsigned long long *sim_fail, const double *znorm,
+                                unsigned long long *ties) {
     // stored normals serve the fused method's multi-tile launches; a single tile regenerates
 #define FOHIP_EVAL_ARGS st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, nosupport, errors, \
-                        sim_fail, znorm
+                        sim_fail, znorm, ties
+    // windowed near-tie counts: a diagnostic form, single-strike tiles regenerating their normals
+    if (fp.tie_window != 0.0) {
+        if (tile != 1 || znorm != nullptr) return hipErrorInvalidValue;
+        if (greeks)
+            return escrow ? launch_eval_t<FAST, 1, true, true, false, true>(FOHIP_EVAL_ARGS)
+                          : launch_eval_t<FAST, 1, true, false, false, true>(FOHIP_EVAL_ARGS);
+        return escrow ? launch_eval_t<FAST, 1, false, true, false, true>(FOHIP_EVAL_ARGS)
+                      : launch_eval_t<FAST, 1, false, false, false, true>(FOHIP_EVAL_ARGS);
+    }
 #define FOHIP_EVAL_CASE(TSV, GV, ZV)                                                                               \
     if (tile == TSV && greeks == GV && (znorm != nullptr) == ZV)                                                   \
         return escrow ? launch_eval_t<FAST, TSV, GV, true, ZV>(FOHIP_EVAL_ARGS)                                    \
@@ -1798,13 +1867,14 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32
 hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile, const DevTables &t,
                            const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                            uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
-                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm) {
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm,
+                           unsigned long long *ties) {
     if (nitems == 0) return hipSuccess;
     if (fast)
         return launch_eval_r<true>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                    partials, counts, nosupport, errors, sim_fail, znorm);
+                                    partials, counts, nosupport, errors, sim_fail, znorm, ties);
     return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                 partials, counts, nosupport, errors, sim_fail, nullptr);
+                                 partials, counts, nosupport, errors, sim_fail, nullptr, ties);
 }
 
 hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows, uint32_t nrows,

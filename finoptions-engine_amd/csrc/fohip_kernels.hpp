@@ -16,6 +16,8 @@ struct FitParams {
     int32_t pad;
     double rank_tol;
     double cond_limit;
+    // near-tie window of exercise decisions: |intrinsic - continuation| <= tie_window |continuation|
+    double tie_window;
 };
 
 // Word fills of up to kMaxFills device regions in one launch (the per-pricing initialisation of
@@ -43,7 +45,8 @@ hipError_t launch_simulate(hipStream_t st, bool f32, const DevTables &t, const u
 // FP64 method: prepare (apply + discount + Welford), then per degree k the moments of p_k
 hipError_t launch_fit_prepare(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                               const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                              void *values, const Model *models, double *partials, RowError *errors);
+                              void *values, const Model *models, double *partials, RowError *errors,
+                              unsigned long long *ties);
 hipError_t launch_fit_reduce_welford(hipStream_t st, const DevTables &t, const FitParams &fp,
                                      uint32_t step, const uint32_t *rows, uint32_t nrows,
                                      double *partials, Model *models, RowError *errors);
@@ -57,14 +60,15 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
 // (group, kBulkChunk-path chunk) items, then one wave per strike row reduces and fits.
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors);
+                                void *values, const Model *models, double *partials, RowError *errors,
+                                unsigned long long *ties);
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const Model *models,
-                            double *partials, RowError *errors);
+                            double *partials, RowError *errors, unsigned long long *ties);
 hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
                                     const uint32_t *rows, uint32_t nrows, double *partials,
                                     Model *models, RowError *errors);
@@ -73,7 +77,7 @@ hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, 
                            uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                            const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail,
-                           const double *znorm);
+                           const double *znorm, unsigned long long *ties);
 // Deferred rank/condition gate of every fused rule, then each row's earliest gate failure.
 hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows,
                            uint32_t nrows, Model *models, uint32_t model_count, RowError *errors);

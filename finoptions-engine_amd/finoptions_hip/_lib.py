@@ -58,6 +58,7 @@ _dp = C.POINTER(C.c_double)
 _u32p = C.POINTER(C.c_uint32)
 _i64p = C.POINTER(C.c_int64)
 _u8p = C.POINTER(C.c_uint8)
+_u64p = C.POINTER(C.c_uint64)
 
 
 class FohipCallRequest(C.Structure):
@@ -115,6 +116,8 @@ class FohipCallResult(C.Structure):
         ("diagnostics", C.POINTER(FohipNodeDiagnostic)),
         ("greeks", C.c_double * 4),
         ("greek_standard_errors", C.c_double * 4),
+        ("near_tie_training_decisions", C.c_uint64),
+        ("near_tie_evaluation_decisions", C.c_uint64),
     ]
 
 
@@ -153,6 +156,7 @@ class FohipBatchResult(C.Structure):
         ("early_exercise_paths", _u32p), ("expiry_paths", _u32p),
         ("delta", _dp), ("vega", _dp), ("rho", _dp), ("dividend_yield", _dp),
         ("delta_se", _dp), ("vega_se", _dp), ("rho_se", _dp), ("dividend_yield_se", _dp),
+        ("near_tie_training_decisions", _u64p), ("near_tie_evaluation_decisions", _u64p),
     ]
 
 
@@ -172,7 +176,8 @@ class FohipPathPlan(C.Structure):
 # Every symbol declared in include/fohip.h (checked by the CPU test suite).
 EXPORTED = (
     "fohip_abi_version", "fohip_device_count", "fohip_executor_create", "fohip_executor_destroy",
-    "fohip_executor_info", "fohip_set_tuning", "fohip_price_lsm_call", "fohip_price_lsm_grouped",
+    "fohip_executor_info", "fohip_set_tuning", "fohip_set_near_tie_window", "fohip_price_lsm_call",
+    "fohip_price_lsm_grouped",
     "fohip_grouped_diagnostics", "fohip_simulate_gbm_seeded", "fohip_simulate_gbm_seeded_device",
     "fohip_philox_blocks",
     "fohip_stream_normals", "fohip_validate_time_grid", "fohip_validate_log_discount_curve",
@@ -225,6 +230,7 @@ def lib() -> C.CDLL:
     handle.fohip_executor_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                            C.c_char_p, C.c_size_t]
     handle.fohip_set_tuning.argtypes = [C.c_void_p, C.POINTER(FohipTuning)]
+    handle.fohip_set_near_tie_window.argtypes = [C.c_void_p, C.c_double]
     handle.fohip_price_lsm_call.argtypes = [C.c_void_p, C.POINTER(FohipCallRequest),
                                             C.POINTER(FohipCallResult), E]
     handle.fohip_price_lsm_grouped.argtypes = [C.c_void_p, C.POINTER(FohipGroupedRequest),

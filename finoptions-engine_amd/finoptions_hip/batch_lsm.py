@@ -39,6 +39,7 @@ from .simulation import HipExecutor
 Float64Vector = npt.NDArray[np.float64]
 Int64Vector = npt.NDArray[np.int64]
 UInt32Vector = npt.NDArray[np.uint32]
+UInt64Vector = npt.NDArray[np.uint64]
 BoolVector = npt.NDArray[np.bool_]
 
 _MAX_U32: Final = (2**32) - 1
@@ -240,7 +241,14 @@ _COLUMNS: Final = (
 
 @dataclass(frozen=True, slots=True, eq=False)
 class LsmBatchResult:
-    """Immutable row-ordered prices, sampling error, policy support, stopping counts, Greeks."""
+    """Immutable row-ordered prices, sampling error, policy support, stopping counts, Greeks.
+
+    ``near_tie_training_decisions`` and ``near_tie_evaluation_decisions`` count, per row, the rule
+    applications of the backward pass and of the evaluation that are ties: by default exact ties,
+    intrinsic == continuation, which the tie rule settles; with a window
+    (``HipExecutor.set_near_tie_window``) every decision within it. They have no counterpart in
+    the reference's result and are filled by every HIP pricing.
+    """
 
     prices: Float64Vector
     sample_standard_deviations: Float64Vector
@@ -256,6 +264,8 @@ class LsmBatchResult:
     expiry_paths: UInt32Vector
     greeks: dict[Greek, Float64Vector] | None = None
     greek_standard_errors: dict[Greek, Float64Vector] | None = None
+    near_tie_training_decisions: UInt64Vector | None = None
+    near_tie_evaluation_decisions: UInt64Vector | None = None
 
     def __post_init__(self) -> None:
         rows = int(self.prices.size)
@@ -393,7 +403,11 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
             greek_se[g] = np.zeros(rows)
             setattr(res, g.value, ptr(greek_cols[g], C.c_double))
             setattr(res, g.value + "_se", ptr(greek_se[g], C.c_double))
-    keep.append((cols, greek_cols, greek_se))
+    ties = {name: np.zeros(rows, dtype=np.uint64)
+            for name in ("near_tie_training_decisions", "near_tie_evaluation_decisions")}
+    for name, values in ties.items():
+        setattr(res, name, ptr(values, C.c_uint64))
+    keep.append((cols, greek_cols, greek_se, ties))
     err = FohipError()
     code = lib().fohip_price_lsm_grouped(executor._handle.ptr, C.byref(req), C.byref(res),  # noqa: SLF001
                                          C.byref(err))
@@ -402,10 +416,10 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
         if dtype == "bool":
             cols[name] = cols[name].astype(np.bool_)
         cols[name].flags.writeable = False
-    for d in (greek_cols, greek_se):
+    for d in (greek_cols, greek_se, ties):
         for v in d.values():
             v.flags.writeable = False
-    result = LsmBatchResult(**cols, greeks=greek_cols or None, greek_standard_errors=greek_se or None)
+    result = LsmBatchResult(**cols, greeks=greek_cols or None, greek_standard_errors=greek_se or None, **ties)
     stopping = result.early_exercise_paths.astype(np.uint64) + result.expiry_paths.astype(np.uint64)
     if path_counts is None:
         expected = np.full(rows, method.evaluation_paths, dtype=np.uint64)

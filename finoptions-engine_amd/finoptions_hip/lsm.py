@@ -202,6 +202,10 @@ class LsmResult:
     fit_diagnostics: tuple[ExerciseFitDiagnostics, ...]
     greeks: tuple[tuple[Greek, float, float], ...] = ()
     regression_coefficients: tuple[tuple[float, ...], ...] = ()
+    # rule applications within the executor's near-tie window, backward pass and evaluation (no
+    # counterpart in the reference's result)
+    near_tie_training_decisions: int = 0
+    near_tie_evaluation_decisions: int = 0
 
     def greek(self, name: Greek) -> float:
         for g, value, _se in self.greeks:
@@ -328,6 +332,8 @@ def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typ
         fit_diagnostics=fit,
         greeks=greek_values,
         regression_coefficients=coefficients,
+        near_tie_training_decisions=int(res.near_tie_training_decisions),
+        near_tie_evaluation_decisions=int(res.near_tie_evaluation_decisions),
     )
 
 

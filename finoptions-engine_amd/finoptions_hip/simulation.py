@@ -124,6 +124,21 @@ class HipExecutor:
         if lib().fohip_set_tuning(self._handle.ptr, C.byref(t)) != 0:
             raise InputError(f"unsupported tuning eval_strike_tile={eval_strike_tile}")
 
+    def set_near_tie_window(self, window: float = 0.0) -> None:
+        """Relative window of the tie counts reported with every result.
+
+        The default 0 counts exact ties, intrinsic == continuation: the decisions the tie rule
+        settles and the other rule would decide the other way. With window > 0 a rule application
+        counts when |intrinsic - continuation| <= window * |continuation|; 2^-40 (about 9.1e-13)
+        is wider than the rounding differences between the methods' continuations, so a decision
+        outside it cannot flip between them. That mode is a diagnostic: the evaluation then runs
+        single-strike tiles and is several times slower. Prices and every other output are
+        independent of the window.
+        """
+        w = float(window)
+        if lib().fohip_set_near_tie_window(self._handle.ptr, C.c_double(w)) != 0:
+            raise InputError(f"near-tie window must be finite and in [0, 1]; observed {window!r}")
+
 
 class CpuExecutor:
     """The reference's CPU resource is not provided by this package.

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define FOHIP_ABI_VERSION 1
+#define FOHIP_ABI_VERSION 2
 
 /* ---- error codes (the numbering is shared with the CPU oracle) ---- */
 enum {
@@ -151,6 +151,15 @@ typedef struct {
 } fohip_tuning;
 int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
 
+/* Window of the tie counts reported with every result. The default 0 counts exact ties,
+ * intrinsic == continuation: the decisions the tie rule settles, which the other rule would
+ * decide the other way. A window w in (0, 1] counts every |intrinsic - continuation| <=
+ * w |continuation| instead (e.g. 2^-40, wider than the rounding differences between the methods'
+ * continuations); it is a diagnostic mode whose evaluation runs single-strike tiles and is
+ * several times slower. Results other than the counts do not depend on the window. No
+ * counterpart in the reference. */
+int fohip_set_near_tie_window(fohip_executor *executor, double window);
+
 /* Per-kernel timing with HIP events recorded on the executor's stream around every launch
  * (off by default; it adds one event pair per launch). Statistics accumulate until reset. */
 typedef struct {
@@ -227,6 +236,11 @@ typedef struct {
     fohip_node_diagnostic *diagnostics; /* [exercise_count], caller buffer */
     double greeks[4];                   /* delta, vega, rho, dividend_yield (when requested) */
     double greek_standard_errors[4];
+    /* Tied rule applications (see fohip_set_near_tie_window), training (backward pass) and
+     * evaluation: by default the exact ties, whose outcome the tie rule settles. No counterpart in
+     * the reference; reported so that decision flips at ties are accounted for. */
+    uint64_t near_tie_training_decisions;
+    uint64_t near_tie_evaluation_decisions;
 } fohip_call_result;
 
 int fohip_price_lsm_call(fohip_executor *executor, const fohip_call_request *request,
@@ -296,6 +310,8 @@ typedef struct {
     uint32_t *expiry_paths;
     double *delta, *vega, *rho, *dividend_yield;
     double *delta_se, *vega_se, *rho_se, *dividend_yield_se;
+    /* near-tie rule applications per row, training and evaluation (see fohip_call_result) */
+    uint64_t *near_tie_training_decisions, *near_tie_evaluation_decisions;
 } fohip_batch_result;
 
 int fohip_price_lsm_grouped(fohip_executor *executor, const fohip_grouped_request *request,

@@ -738,6 +738,12 @@ static inline int should_exercise(int tie, double intrinsic, double continuation
     return tie == 0 ? intrinsic > continuation : intrinsic >= continuation;
 }
 
+/* Near tie of a rule application: the decisions a last-bit change of the continuation could
+ * flip. Counting them is this project's addition; the reference has no such diagnostic. */
+static inline int near_tie(double intrinsic, double continuation, double window) {
+    return fabs(intrinsic - continuation) <= window * fabs(continuation);
+}
+
 static inline double call_intrinsic(double spot, double strike) {
     double v = spot - strike;
     return v > 0.0 ? v : 0.0;   /* f64::max(v, 0.0) for non-NaN v */
@@ -815,6 +821,7 @@ static int fit_policy(const plan_t *p, const double *log_discounts, double initi
                 if (!isfinite(continuation))
                     return set_err(err, E_NON_FINITE_PREDICTION, PH_FIT, node, path, 0, xbuf[j],
                                    0);
+                f->near_ties += (uint32_t)near_tie(intrinsic, continuation, est->tie_window);
                 if (should_exercise(est->tie_rule, intrinsic, continuation))
                     values[path] = intrinsic;
             }
@@ -854,6 +861,7 @@ typedef struct {
     double price, sd, se;
     int has_sd;
     uint32_t *counts;   /* [obs] */
+    uint64_t near_ties;
 } eval_t;
 
 /* FittedLsmCallPolicy::evaluate (policy.rs:642-736) + from_path_payoffs (:891-926). */
@@ -897,6 +905,7 @@ static int evaluate_policy(const plan_t *p, const double *log_discounts, double 
             if (!isfinite(continuation))
                 return set_err(err, E_NON_FINITE_PREDICTION, PH_EVALUATE, node, i, 0,
                                spot / est->strike, 0);
+            out->near_ties += (uint64_t)near_tie(intrinsic, continuation, est->tie_window);
             if (should_exercise(est->tie_rule, intrinsic, continuation)) {
                 double payoff = intrinsic * discount;
                 if (!isfinite(payoff))
@@ -959,6 +968,9 @@ static void fill_result(const plan_t *p, const or_node_fit *fits, const eval_t *
     }
     res->fit_count = p->obs - 1;
     if (res->fits) memcpy(res->fits, fits, sizeof(or_node_fit) * (p->obs - 1));
+    res->near_tie_training = 0;
+    for (uint32_t r = 0; r + 1 < p->obs; ++r) res->near_tie_training += fits[r].near_ties;
+    res->near_tie_evaluation = ev->near_ties;
 }
 
 int or_fit_evaluate_paths(const or_plan_input *plan, double initial_spot, const or_estimator *est,
@@ -1360,7 +1372,8 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
     }
     for (uint32_t s = 0; s < strikes && !rc; ++s) {
         or_estimator est = {in->strikes[first + s], in->degree, in->rank_tolerance,
-                            in->condition_limit, in->tie_rule, in->insufficient_support};
+                            in->condition_limit, in->tie_rule, in->insufficient_support,
+                            in->tie_window};
         rc = fit_policy(&p, gp.logd, in->initial_spots[g], &est, train, nt,
                         fits + (size_t)s * p.obs, values, xb, yb, ib, err);
         if (rc) err->row = first + s;
@@ -1378,7 +1391,8 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
     for (uint32_t s = 0; s < strikes; ++s) {
         uint32_t row = first + s;
         or_estimator est = {in->strikes[row], in->degree, in->rank_tolerance, in->condition_limit,
-                            in->tie_rule, in->insufficient_support};
+                            in->tie_rule, in->insufficient_support,
+                            in->tie_window};
         eval_t ev = {.counts = counts};
         const or_node_fit *f = fits + (size_t)s * p.obs;
         rc = evaluate_policy(&p, gp.logd, in->initial_spots[g], &est, f, evalm, ne, values, active,
@@ -1419,6 +1433,10 @@ static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *
         out->max_condition[row] = max_cond;
         out->early_paths[row] = (uint32_t)early;
         out->expiry_paths[row] = counts[p.obs - 1];
+        uint64_t train_ties = 0;
+        for (uint32_t r = 0; r + 1 < p.obs; ++r) train_ties += f[r].near_ties;
+        if (out->near_tie_training) out->near_tie_training[row] = train_ties;
+        if (out->near_tie_evaluation) out->near_tie_evaluation[row] = ev.near_ties;
     }
 done:
     free(train);
@@ -1470,7 +1488,8 @@ int or_grouped_models(const or_grouped_input *in, uint32_t g, or_node_fit *fits_
                             in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < last - first; ++s) {
         or_estimator est = {in->strikes[first + s], in->degree, in->rank_tolerance,
-                            in->condition_limit, in->tie_rule, in->insufficient_support};
+                            in->condition_limit, in->tie_rule, in->insufficient_support,
+                            in->tie_window};
         rc = fit_policy(&p, gp.logd, in->initial_spots[g], &est, train, nt, fits, values, xb, yb,
                         ib, err);
         if (rc) {
@@ -1528,7 +1547,8 @@ int or_grouped_frozen_prices(const or_grouped_input *fit_in, const or_grouped_in
                             fit_in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < strikes; ++s) {
         or_estimator est = {fit_in->strikes[first + s], fit_in->degree, fit_in->rank_tolerance,
-                            fit_in->condition_limit, fit_in->tie_rule, fit_in->insufficient_support};
+                            fit_in->condition_limit, fit_in->tie_rule, fit_in->insufficient_support,
+                            fit_in->tie_window};
         rc = fit_policy(&pf, gf.logd, fit_in->initial_spots[g], &est, train, nt,
                         fits + (size_t)s * pf.obs, values, xb, yb, ib, err);
     }
@@ -1607,7 +1627,8 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
                             in->training_domain_start + g, train, err, PH_TRAIN_SIM);
     for (uint32_t s = 0; !rc && s < strikes; ++s) {
         or_estimator est = {in->strikes[first + s], in->degree, in->rank_tolerance,
-                            in->condition_limit, in->tie_rule, in->insufficient_support};
+                            in->condition_limit, in->tie_rule, in->insufficient_support,
+                            in->tie_window};
         rc = fit_policy(&p, gp.logd, in->initial_spots[g], &est, train, nt,
                         fits + (size_t)s * p.obs, values, xb, yb, ib, err);
     }

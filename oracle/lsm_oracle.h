@@ -93,6 +93,7 @@ typedef struct {
     double rank_tolerance, condition_limit;
     int32_t tie_rule;              /* 0 strictly greater, 1 greater or equal */
     int32_t insufficient_support;  /* 0 fail, 1 continue */
+    double tie_window;             /* near-tie count: |intrinsic - continuation| <= w |continuation| */
 } or_estimator;
 
 /* kind: 0 origin decision, 1 no ITM paths, 2 insufficient support, 3 regression */
@@ -104,6 +105,7 @@ typedef struct {
     uint32_t itm_paths;
     uint32_t required_paths;
     or_model model;
+    uint32_t near_ties;            /* rule applications at this node within the near-tie window */
 } or_node_fit;
 
 typedef struct {
@@ -116,6 +118,7 @@ typedef struct {
     uint32_t *exercise_counts;     /* [obs] caller buffer */
     or_node_fit *fits;             /* [exercise_count] caller buffer */
     uint32_t fit_count;
+    uint64_t near_tie_training, near_tie_evaluation;
 } or_call_result;
 
 int or_price_lsm_call(const or_plan_input *plan, double initial_spot, const or_estimator *est,
@@ -160,6 +163,7 @@ typedef struct {
     double rank_tolerance, condition_limit;
     int32_t tie_rule, insufficient_support;
     int32_t escrowed_volatility;       /* model 2: 0 prepaid forward, 1 spot matched at origin */
+    double tie_window;                 /* near-tie window of the counts below */
 } or_grouped_input;
 
 typedef struct {
@@ -168,6 +172,7 @@ typedef struct {
     uint32_t *regression_nodes, *no_support_nodes, *insufficient_nodes, *min_itm_paths;
     double *max_condition;
     uint32_t *early_paths, *expiry_paths;
+    uint64_t *near_tie_training, *near_tie_evaluation;   /* optional (NULL skips) */
 } or_batch_output;
 
 /* Builds the transition grid of one group (pricing.rs:548-628, 630-695, 862-965). Returns the

@@ -77,6 +77,7 @@ class OrEstimator(C.Structure):
         ("condition_limit", C.c_double),
         ("tie_rule", C.c_int32),
         ("insufficient_support", C.c_int32),
+        ("tie_window", C.c_double),
     ]
 
 
@@ -89,6 +90,7 @@ class OrNodeFit(C.Structure):
         ("itm_paths", C.c_uint32),
         ("required_paths", C.c_uint32),
         ("model", OrModel),
+        ("near_ties", C.c_uint32),
     ]
 
 
@@ -104,6 +106,8 @@ class OrCallResult(C.Structure):
         ("exercise_counts", C.POINTER(C.c_uint32)),
         ("fits", C.POINTER(OrNodeFit)),
         ("fit_count", C.c_uint32),
+        ("near_tie_training", C.c_uint64),
+        ("near_tie_evaluation", C.c_uint64),
     ]
 
 
@@ -142,6 +146,7 @@ class OrGroupedInput(C.Structure):
         ("tie_rule", C.c_int32),
         ("insufficient_support", C.c_int32),
         ("escrowed_volatility", C.c_int32),
+        ("tie_window", C.c_double),
     ]
 
 
@@ -159,6 +164,8 @@ class OrBatchOutput(C.Structure):
         ("max_condition", C.POINTER(C.c_double)),
         ("early_paths", C.POINTER(C.c_uint32)),
         ("expiry_paths", C.POINTER(C.c_uint32)),
+        ("near_tie_training", C.POINTER(C.c_uint64)),
+        ("near_tie_evaluation", C.POINTER(C.c_uint64)),
     ]
 
 
@@ -380,25 +387,34 @@ def _call_result(plan: Plan, res: OrCallResult, keep) -> dict:
             "node": f.node, "kind": KIND_NAMES[f.kind], "exercise": bool(f.exercise),
             "estimated_continuation": f.estimated_continuation, "itm_paths": f.itm_paths,
             "required_paths": f.required_paths, "model": model_dict(f.model),
+            "near_ties": f.near_ties,
         })
     return {
         "price": res.price, "paths": res.paths,
         "sd": res.sd if res.has_sd else None, "se": res.se if res.has_sd else None,
         "observation_nodes": keep["nodes"].tolist(),
         "exercise_counts": keep["counts"].tolist(), "fits": fits,
+        "near_tie_training_decisions": int(res.near_tie_training),
+        "near_tie_evaluation_decisions": int(res.near_tie_evaluation),
     }
 
 
-def _estimator(strike, degree, rank_tolerance, condition_limit, tie_rule, support):
+# Default window of the tie counts (the GPU executor's default: exact ties).
+TIE_WINDOW = 0.0
+
+
+def _estimator(strike, degree, rank_tolerance, condition_limit, tie_rule, support,
+               tie_window=TIE_WINDOW):
     return OrEstimator(float(strike), int(degree), float(rank_tolerance), float(condition_limit),
-                       int(tie_rule), int(support))
+                       int(tie_rule), int(support), float(tie_window))
 
 
 def price_lsm_call(plan: Plan, initial_spot: float, strike: float, training_paths: int,
                    evaluation_paths: int, training_seed: int, training_domain: int,
                    evaluation_seed: int, evaluation_domain: int, degree: int = 4,
                    rank_tolerance: float = 1e-12, condition_limit: float = 1e8,
-                   tie_rule: int = 0, insufficient_support: int = 0) -> dict:
+                   tie_rule: int = 0, insufficient_support: int = 0,
+                   tie_window: float = TIE_WINDOW) -> dict:
     s, keep_plan = plan.struct()
     obs = plan.observations
     keep = {"nodes": np.zeros(obs, np.uint32), "counts": np.zeros(obs, np.uint32),
@@ -406,7 +422,7 @@ def price_lsm_call(plan: Plan, initial_spot: float, strike: float, training_path
     res = OrCallResult(0, 0, 0, 0, 0, obs, _p(keep["nodes"], C.c_uint32),
                        _p(keep["counts"], C.c_uint32), keep["fits"], 0)
     est = _estimator(strike, degree, rank_tolerance, condition_limit, tie_rule,
-                     insufficient_support)
+                     insufficient_support, tie_window)
     err = OrError()
     rc = lib().or_price_lsm_call(C.byref(s), C.c_double(initial_spot), C.byref(est),
                                  C.c_uint32(training_paths), C.c_uint32(evaluation_paths),
@@ -421,7 +437,7 @@ def price_lsm_call(plan: Plan, initial_spot: float, strike: float, training_path
 def fit_evaluate_paths(plan: Plan, initial_spot: float, strike: float, training: np.ndarray,
                        evaluation: np.ndarray, degree: int = 4, rank_tolerance: float = 1e-12,
                        condition_limit: float = 1e8, tie_rule: int = 0,
-                       insufficient_support: int = 0) -> dict:
+                       insufficient_support: int = 0, tie_window: float = TIE_WINDOW) -> dict:
     s, keep_plan = plan.struct()
     obs = plan.observations
     tr = np.ascontiguousarray(training, dtype=np.float64)
@@ -431,7 +447,7 @@ def fit_evaluate_paths(plan: Plan, initial_spot: float, strike: float, training:
     res = OrCallResult(0, 0, 0, 0, 0, obs, _p(keep["nodes"], C.c_uint32),
                        _p(keep["counts"], C.c_uint32), keep["fits"], 0)
     est = _estimator(strike, degree, rank_tolerance, condition_limit, tie_rule,
-                     insufficient_support)
+                     insufficient_support, tie_window)
     err = OrError()
     rc = lib().or_fit_evaluate_paths(C.byref(s), C.c_double(initial_spot), C.byref(est),
                                      _p(tr, C.c_double), C.c_uint32(tr.shape[1]),
@@ -491,6 +507,8 @@ class GroupedRequest:
     # escrowed cash-dividend model for the cash columns: None (spot cash model), 0 prepaid-forward
     # volatility, 1 spot volatility matched at the origin
     escrowed: int | None = None
+    # window of the near-tie counts
+    tie_window: float = TIE_WINDOW
 
     @property
     def dividend_model(self) -> int:
@@ -523,7 +541,7 @@ class GroupedRequest:
             cptr[0], cptr[1], int(self.training_seed), int(self.evaluation_seed), int(self.training_domain_start),
             int(self.evaluation_domain_start), int(self.degree), float(self.rank_tolerance),
             float(self.condition_limit), int(self.tie_rule), int(self.insufficient_support),
-            int(self.escrowed or 0))
+            int(self.escrowed or 0), float(self.tie_window))
         return s, keep
 
 
@@ -534,6 +552,9 @@ def price_grouped(req: GroupedRequest, threads: int = 0) -> dict[str, np.ndarray
             for name, dt in GROUPED_COLUMNS}
     ctype = {np.float64: C.c_double, np.uint32: C.c_uint32, np.bool_: C.c_uint8}
     ptrs = [_p(cols[name], ctype[dt]) for name, dt in GROUPED_COLUMNS]
+    ties = {name: np.zeros(rows, dtype=np.uint64)
+            for name in ("near_tie_training_decisions", "near_tie_evaluation_decisions")}
+    ptrs += [_p(v, C.c_uint64) for v in ties.values()]
     out = OrBatchOutput(*ptrs)
     err = OrError()
     rc = lib().or_price_grouped(C.byref(s), C.byref(out), int(threads), C.byref(err))
@@ -542,6 +563,7 @@ def price_grouped(req: GroupedRequest, threads: int = 0) -> dict[str, np.ndarray
     for name, dt in GROUPED_COLUMNS:
         if dt is np.bool_:
             cols[name] = cols[name].astype(np.bool_)
+    cols.update(ties)
     return cols
 
 

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(L.EXPORTED)
     for name in sorted(declared):
         assert hasattr(handle, name), name
-    assert handle.fohip_abi_version() == 1
+    assert handle.fohip_abi_version() == 2
 
 
 def test_library_has_gfx950_code_object():
@@ -115,3 +115,41 @@ def test_product_does_not_import_the_oracle():
     for path in list(pkg.rglob("*.py")) + list(pkg.rglob("*.cpp")) + list(pkg.rglob("*.hip")):
         text = path.read_text()
         assert "pyoracle" not in text and "lsm_oracle" not in text, path
+
+
+STRUCTS = {
+    "fohip_error": "FohipError", "fohip_tuning": "FohipTuning", "fohip_kernel_stat": "FohipKernelStat",
+    "fohip_call_request": "FohipCallRequest", "fohip_node_diagnostic": "FohipNodeDiagnostic",
+    "fohip_call_result": "FohipCallResult", "fohip_grouped_request": "FohipGroupedRequest",
+    "fohip_batch_result": "FohipBatchResult", "fohip_path_plan": "FohipPathPlan",
+}
+
+
+def test_ctypes_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror has the C struct's size and each field's offset, as gcc lays them out
+    from include/fohip.h (the field names are the header's)."""
+    import shutil
+    import subprocess
+
+    import finoptions_hip._lib as L
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc is not available")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fohip.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, *_ in getattr(L, pyname)._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True,
+                   capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    c_layout = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for cname, pyname in STRUCTS.items():
+        cls = getattr(L, pyname)
+        assert C.sizeof(cls) == c_layout[(cname, "size")], cname
+        for field, *_ in cls._fields_:
+            assert getattr(cls, field).offset == c_layout[(cname, field)], (cname, field)
