@@ -812,7 +812,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
         HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
         HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
-        HIP_TRY(ex->ties.ensure(std::max<uint32_t>(nrows, 1) * 2 * sizeof(unsigned long long)));
+        HIP_TRY(ex->ties.ensure(std::max<uint32_t>(nrows, 1) * sizeof(unsigned long long)));
         char *tb = ex->tables.as<char>();
         const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
                                     gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
@@ -832,7 +832,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         std::vector<RowSummary> summary(nrows);
         std::vector<Model> models_h(models_out ? model_count : 0);
         std::vector<uint32_t> counts_h(counts_out ? count_slots : 0);
-        std::vector<unsigned long long> ties_h(static_cast<size_t>(nrows) * 2);   // [training rows | evaluation rows]
+        // tied training decisions counted with atomics by the F64 fit passes and the last fitted row;
+        // the fused pass's are in the row summary and the evaluation's in eval_out
+        std::vector<unsigned long long> ties_h(nrows);
         const Download downloads[] = {
             {fail_t.data(), ex->fail_t.ptr, ngroups * sizeof(unsigned long long)},
             {fail_e.data(), ex->fail_e.ptr, ngroups * sizeof(unsigned long long)},
@@ -887,7 +889,6 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         add_fill(ex->ties.ptr, ties_h.size() * sizeof(unsigned long long), 0u);
         HIP_TRY(launch_fill(st, fills));
         unsigned long long *ties_t = ex->ties.as<unsigned long long>();
-        unsigned long long *ties_e = ties_t + nrows;
         Model *models = ex->models.as<Model>();
         RowError *errors = ex->errors.as<RowError>();
         FitParams fp = spec.fp;
@@ -901,7 +902,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             if (spec.fused) {   // one fused HBM pass per backward step
                 PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, ex->paths.ptr,
                                                                     ex->values.ptr, models, ex->part_t.as<double>(),
-                                                                    errors, ties_t));
+                                                                    errors));
                 PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
                                                                         models, errors));
                 continue;
@@ -942,7 +943,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ite,
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
                                 ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
-                                ex->fail_e.as<unsigned long long>(), znorm, ties_e));
+                                ex->fail_e.as<unsigned long long>(), znorm));
         PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
         PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
                                    ex->summary.as<RowSummary>()));
@@ -1065,8 +1066,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             const GroupPlan &P = plans[g];
             RowOut &o = out[row0 + s];
             o.sum = summary[s];
-            o.ties_training = ties_h[s];
-            o.ties_evaluation = ties_h[nrows + s];
+            o.ties_training = ties_h[s] + static_cast<uint64_t>(summary[s].ties_training);
+            o.ties_evaluation = static_cast<uint64_t>(eval_out[static_cast<size_t>(s) * 16 + 15]);
             const double n = static_cast<double>(P.ne);
             if (summary[s].origin_fitted && summary[s].origin_exercised) {
                 // policy.rs:672-682: deterministic origin exercise

@@ -742,8 +742,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
                                                           uint32_t sets, const Store *__restrict__ paths,
                                                           Store *__restrict__ values, const Model *__restrict__ models,
                                                           double *__restrict__ partials,
-                                                          RowError *__restrict__ errors,
-                                                          unsigned long long *__restrict__ ties) {
+                                                          RowError *__restrict__ errors) {
     __shared__ double red[kWaves][kBulkStrikes][16];
     __shared__ double transpose[kWaves][14 * 66];
     uint32_t wi, set;
@@ -880,13 +879,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
         }
     }
     __syncthreads();
-    if (threadIdx.x < static_cast<uint32_t>(ns)) {
-        const int i = static_cast<int>(threadIdx.x);
-        const double c = (red[0][i][15] + red[1][i][15]) + (red[2][i][15] + red[3][i][15]);
-        if (c != 0.0) atomicAdd(ties + s_first + i, static_cast<unsigned long long>(c));
-    }
-    for (int idx = threadIdx.x; idx < ns * 15; idx += kBlock) {
-        const int i = idx / 15, q = idx % 15;
+    // slots 0..13 power sums, 14 the pivot, 15 the tied rule applications (an integer)
+    for (int idx = threadIdx.x; idx < ns * 16; idx += kBlock) {
+        const int i = idx / 16, q = idx % 16;
         const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
         const uint32_t local = s_first + i - strike_begin;
         partials[static_cast<uint64_t>(g->part_base_t + local * g->chunks_t + item.y) * kPartialStride + q] = v;
@@ -976,25 +971,31 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
 #pragma unroll
     for (int i = 0; i < 14; ++i) acc[i] = 0.0;
     // four chunks' loads in flight per round trip; the sums still add chunks c, c + 64, ... in order
+    double ties = 0.0;   // tied applications: integers, exact in any order
     uint32_t c = lane;
     for (; c + 3 * 64 < g.chunks_b; c += 4 * 64) {
-        double in[4][14];
+        double in[4][16];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const double *slot = partial_slot(partials, g, local, c + u * 64);
 #pragma unroll
-            for (int i = 0; i < 14; ++i) in[u][i] = slot[i];
+            for (int i = 0; i < 16; ++i) in[u][i] = slot[i];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) {
 #pragma unroll
             for (int i = 0; i < 14; ++i) acc[i] += in[u][i];
+            ties += in[u][15];
+        }
     }
     for (; c < g.chunks_b; c += 64) {
         const double *in = partial_slot(partials, g, local, c);
 #pragma unroll
         for (int i = 0; i < 14; ++i) acc[i] += in[i];
+        ties += in[15];
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ties += __shfl_xor(ties, off, 64);
     const double total = wave_sums_lds<14>(acc, buf);
     if ((lane & 3) == 0 && lane < 56) buf[lane >> 2] = total;   // totals of values 0..13
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1008,6 +1009,7 @@ __global__ __launch_bounds__(64) void k_fit_bulk_reduce(DevTables t, FitParams f
     // update into a global store and every re-read into a dependent global load.
     Model m = {};
     solve_node_from_sums(m, fp, &errors[srow], step, t.strikes[srow], acc, pivot);
+    m.ties = ties;
     models[g.model_base + local * g.obs + r] = m;
 }
 
@@ -1189,14 +1191,13 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
                                                      uint32_t *__restrict__ nosupport, RowError *__restrict__ errors,
                                                      unsigned long long *__restrict__ sim_fail, int use_lds_hist,
-                                                     const double *__restrict__ znorm,
-                                                     unsigned long long *__restrict__ ties) {
+                                                     const double *__restrict__ znorm) {
     using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ __align__(16) uint32_t hist[];   // [TS][obs] when use_lds_hist, then DEC's tables
     __shared__ double sx[257], sf[257];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
-    __shared__ uint32_t tie_lds[TS];   // near ties of this tile, flushed once at the end
+    __shared__ uint32_t tie_lds[TS];   // tied decisions of this tile, written with its partials
     uint32_t wi, tile;
     if (!xcd_item_column(nitems, tiles, wi, tile)) return;
     const uint2 item = load_item(items, wi);
@@ -1418,9 +1419,6 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             }
         }
     }
-    __syncthreads();
-    if (threadIdx.x < static_cast<uint32_t>(ns) && tie_lds[threadIdx.x] != 0u)
-        atomicAdd(ties + s_first + threadIdx.x, static_cast<unsigned long long>(tie_lds[threadIdx.x]));
     // per-strike block reduction of (sum v, sum v^2[, sum dv_a, sum dv_a^2])
     const bool contributes_any = p < n && ok;
 #pragma unroll
@@ -1445,6 +1443,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                                          kPartialStride;
 #pragma unroll
             for (int a = 0; a < NACC; ++a) out[a] = vals[a];
+            out[kPartialStride - 1] = static_cast<double>(tie_lds[i]);   // block_sum's barriers published it
         }
     }
     if (use_lds_hist) {
@@ -1461,13 +1460,13 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
 __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint32_t *__restrict__ rows,
                                                         const double *__restrict__ partials, int greeks,
                                                         double *__restrict__ out) {
-    __shared__ double scratch[8 * kWaves];
+    __shared__ double scratch[9 * kWaves];
     __shared__ double mscratch[3 * kWaves];
     const uint32_t srow = rows[blockIdx.x];
     const DevGroup &g = t.groups[t.strike_group[srow]];
     const uint32_t local = srow - g.strike_begin;
     Moments acc = {0.0, 0.0, 0.0};
-    double gs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // Greek sums, then the tied decisions
     for (uint32_t c = threadIdx.x; c < g.chunks_e; c += kBlock) {
         const double *in = partials + static_cast<uint64_t>(g.part_base_e + local * g.chunks_e + c) * kPartialStride;
         const double nc = static_cast<double>(min(static_cast<uint32_t>(kEvalChunk), g.ne - c * kEvalChunk));
@@ -1477,9 +1476,10 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
         if (greeks)
 #pragma unroll
             for (int a = 0; a < 8; ++a) gs[a] += in[2 + a];
+        gs[8] += in[kPartialStride - 1];
     }
     acc = block_chan(acc, mscratch);
-    block_sum<8>(gs, scratch);
+    block_sum<9>(gs, scratch);
     if (threadIdx.x == 0) {
         double *o = out + static_cast<uint64_t>(srow) * 16;
         const double nn = static_cast<double>(g.ne);
@@ -1492,6 +1492,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
             o[3 + a] = mean;
             o[7 + a] = sqrt(var / nn);
         }
+        o[15] = gs[8];   // tied evaluation decisions (an integer)
     }
 }
 
@@ -1519,8 +1520,10 @@ __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model
     rs.nosupport_row = -1;
     rs.nosupport_count = 0;
     rs.origin_estimate = 0.0;
+    rs.ties_training = 0.0;
     for (uint32_t r = 0; r + 1 < g.obs; ++r) {
         const Model &m = models[g.model_base + local * g.obs + r];
+        rs.ties_training += m.ties;
         switch (m.kind) {
             case kRuleOrigin:
                 rs.origin_fitted = 1;
@@ -1735,15 +1738,14 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
 
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors,
-                                unsigned long long *ties) {
+                                void *values, const Model *models, double *partials, RowError *errors) {
     if (nitems == 0) return hipSuccess;
     const uint32_t sets = (max_strikes + kBulkStrikes - 1) / kBulkStrikes;
     const dim3 grid(xcd_blocks(nitems, sets));
     const bool win = fp.tie_window != 0.0;
 #define FOHIP_BULK(REAL, WINV)                                                                                  \
     hipLaunchKernelGGL((k_fit_bulk_step<REAL, WINV>), grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets, \
-                       static_cast<const REAL *>(paths), static_cast<REAL *>(values), models, partials, errors, ties)
+                       static_cast<const REAL *>(paths), static_cast<REAL *>(values), models, partials, errors)
     if (f32) {
         if (win) FOHIP_BULK(float, true);
         else FOHIP_BULK(float, false);
@@ -1806,7 +1808,7 @@ template <bool FAST, int TS, bool G, bool E, bool Z, bool WIN = false>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
-                                unsigned long long *sim_fail, const double *znorm, unsigned long long *ties) {
+                                unsigned long long *sim_fail, const double *znorm) {
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
@@ -1816,12 +1818,12 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     if (Z && TS <= 8 && use_lds && hist_padded + dec_bytes <= 49152) {
         hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, (Z && TS <= 8), WIN>), dim3(xcd_blocks(nitems, tiles)),
                            dim3(kBlock), hist_padded + dec_bytes, st, t, fp, items, nitems, tiles, models, partials,
-                           counts, nosupport, errors, sim_fail, use_lds, znorm, ties);
+                           counts, nosupport, errors, sim_fail, use_lds, znorm);
         return hipGetLastError();
     }
     hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, false, WIN>), dim3(xcd_blocks(nitems, tiles)), dim3(kBlock),
                        use_lds ? hist_bytes : 0, st, t, fp, items, nitems, tiles, models, partials, counts, nosupport,
-                       errors, sim_fail, use_lds, znorm, ties);
+                       errors, sim_fail, use_lds, znorm);
     return hipGetLastError();
 }
 
@@ -1829,11 +1831,10 @@ template <bool FAST>
 static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
-                                RowError *errors, unsigned long long *sim_fail, const double *znorm,
-                                unsigned long long *ties) {
+                                RowError *errors, unsigned long long *sim_fail, const double *znorm) {
     // stored normals serve the fused method's multi-tile launches; a single tile regenerates
 #define FOHIP_EVAL_ARGS st, t, fp, items, nitems, max_strikes, max_obs, models, partials, counts, nosupport, errors, \
-                        sim_fail, znorm, ties
+                        sim_fail, znorm
     // windowed near-tie counts: a diagnostic form, single-strike tiles regenerating their normals
     if (fp.tie_window != 0.0) {
         if (tile != 1 || znorm != nullptr) return hipErrorInvalidValue;
@@ -1867,14 +1868,13 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32
 hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile, const DevTables &t,
                            const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                            uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
-                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm,
-                           unsigned long long *ties) {
+                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm) {
     if (nitems == 0) return hipSuccess;
     if (fast)
         return launch_eval_r<true>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                    partials, counts, nosupport, errors, sim_fail, znorm, ties);
+                                    partials, counts, nosupport, errors, sim_fail, znorm);
     return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
-                                 partials, counts, nosupport, errors, sim_fail, nullptr, ties);
+                                 partials, counts, nosupport, errors, sim_fail, nullptr);
 }
 
 hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows, uint32_t nrows,

@@ -60,8 +60,7 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
 // (group, kBulkChunk-path chunk) items, then one wave per strike row reduces and fits.
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors,
-                                unsigned long long *ties);
+                                void *values, const Model *models, double *partials, RowError *errors);
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
@@ -77,7 +76,7 @@ hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, 
                            uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                            const Model *models, double *partials, uint32_t *counts,
                            uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail,
-                           const double *znorm, unsigned long long *ties);
+                           const double *znorm);
 // Deferred rank/condition gate of every fused rule, then each row's earliest gate failure.
 hipError_t launch_fit_gate(hipStream_t st, const DevTables &t, const FitParams &fp, const uint32_t *rows,
                            uint32_t nrows, Model *models, uint32_t model_count, RowError *errors);

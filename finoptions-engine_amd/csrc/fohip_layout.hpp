@@ -56,6 +56,9 @@ struct alignas(64) Model {
     int32_t gate;
     int32_t gate_index;
     double gate_observed, gate_limit;
+    // Fused method: tied applications of the next row's rule counted in the pass that fits this
+    // row (an integer; the row summary adds them up per strike).
+    double ties;
 };
 
 constexpr int32_t kGateDone = 0;
@@ -138,6 +141,7 @@ constexpr uint32_t kEvalStepBase = 0x80000u;   // evaluation failures: step fiel
 struct RowSummary {
     double max_condition;
     double origin_estimate;
+    double ties_training;   // sum of the rules' `ties` (fused method)
     uint32_t regression_nodes, no_support_nodes, insufficient_nodes, min_itm;
     uint32_t early, expiry;
     int32_t nosupport_row;
