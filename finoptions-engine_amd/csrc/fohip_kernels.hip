@@ -1163,11 +1163,12 @@ __device__ __noinline__ void record_eval_errors(RowError *e, uint32_t row, uint3
     if (payoff_bad) record_error(e, error_key(kEvalStepBase + row, kStageEvalPayoff, p));
 }
 
-// Waves per SIMD asked of an instantiation. The fused Greek evaluation over 8-strike tiles of
-// stored normals runs at 5 (96 VGPRs, a few spilled) rather than 4: the extra wave hides more of
-// the decision chain than the spills cost. Every other form keeps the compiler's own choice.
+// Waves per SIMD asked of an instantiation. The fused Greek evaluation over 6- and 8-strike tiles
+// of stored normals runs at 5 (96 VGPRs) rather than 4: the extra wave hides more of the decision
+// chain's latency. The 6-strike form, the default, fits without spills; the 8-strike form spills
+// its payoff tangents. Every other form keeps the compiler's own choice.
 template <bool FAST, int TS, bool GREEKS, bool STORED>
-constexpr int eval_waves() { return (FAST && GREEKS && STORED && TS == 8) ? 5 : 1; }
+constexpr int eval_waves() { return (FAST && GREEKS && STORED && (TS == 6 || TS == 8)) ? 5 : 1; }
 
 // Decision data of one (row, strike) as staged in LDS: K, a, b, hz0..hz4.
 constexpr int kDecDoubles = 8;
@@ -1795,11 +1796,12 @@ hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const Fi
 
 uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     const uint32_t options_plain[] = {1, 4, 16};
-    const uint32_t options_greeks[] = {1, 4, 8};
+    const uint32_t options_greeks[] = {1, 4, 6, 8};
     const uint32_t *opts = greeks ? options_greeks : options_plain;
-    const uint32_t want = requested ? requested : (max_strikes <= 1 ? 1 : (greeks ? 8 : 16));
+    const uint32_t want = requested ? requested : (max_strikes <= 1 ? 1 : (greeks ? 6 : 16));
     uint32_t tile = opts[0];
-    for (int i = 0; i < 3; ++i)
+    const int nopts = greeks ? 4 : 3;
+    for (int i = 0; i < nopts; ++i)
         if (opts[i] <= want) tile = opts[i];
     return tile;
 }
@@ -1853,11 +1855,13 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32
     FOHIP_EVAL_CASE(16, false, false)
     FOHIP_EVAL_CASE(1, true, false)
     FOHIP_EVAL_CASE(4, true, false)
+    FOHIP_EVAL_CASE(6, true, false)
     FOHIP_EVAL_CASE(8, true, false)
     if (FAST) {
         FOHIP_EVAL_CASE(4, false, true)
         FOHIP_EVAL_CASE(16, false, true)
         FOHIP_EVAL_CASE(4, true, true)
+        FOHIP_EVAL_CASE(6, true, true)
         FOHIP_EVAL_CASE(8, true, true)
     }
 #undef FOHIP_EVAL_CASE

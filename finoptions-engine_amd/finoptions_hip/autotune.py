@@ -17,7 +17,7 @@ from typing import Callable
 from .simulation import HipExecutor
 
 PLAIN_TILES = (1, 4, 16)
-GREEK_TILES = (1, 4, 8)
+GREEK_TILES = (1, 4, 6, 8)
 
 
 @dataclass

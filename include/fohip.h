@@ -146,7 +146,7 @@ int fohip_abi_version(void);
 /* Launch parameters an autotuner may change. Results are bitwise invariant to them: reduction
  * trees are fixed by 1024-path chunks, not by these values. Zero fields keep defaults. */
 typedef struct {
-    uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16) */
+    uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16; Greeks 1, 4, 6, 8) */
     uint32_t sim_waves_per_eu;      /* occupancy hint for simulation launches (0 = default) */
 } fohip_tuning;
 int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
