@@ -1261,6 +1261,12 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     uint32_t active = ok ? live : 0u;
     double pay[TS];
     Tan dpay[TS][GREEKS ? 4 : 1];
+    // DEC: each lane keeps its stopping row per strike, 8 bits each (0xFF: no stop; the launch
+    // guarantees obs <= 255), and the exercise histogram is added once at the end rather than
+    // with a ballot and an atomic at every stop
+    uint32_t stop_rows[(TS + 3) / 4];
+#pragma unroll
+    for (int h = 0; h < (TS + 3) / 4; ++h) stop_rows[h] = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < TS; ++i) {
         pay[i] = 0.0;
@@ -1366,13 +1372,19 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                     const bool payoff_bad = ex && !isfinite(pv);
                     if (__ballot(bad || payoff_bad) != 0ull)
                         record_eval_errors(&errors[s_first + i], row, p, bad, payoff_bad);
-                    const unsigned long long exm = __ballot(ex);
-                    if (exm != 0ull && lane == 0) {
-                        const uint32_t c = static_cast<uint32_t>(__popcll(exm));
-                        if (use_lds_hist)
-                            atomicAdd(&hist[i * obs + row], c);
-                        else
-                            atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
+                    if constexpr (DEC) {
+                        const int sh = 8 * (i & 3);
+                        const uint32_t w = stop_rows[i >> 2];
+                        stop_rows[i >> 2] = ex ? ((w & ~(0xFFu << sh)) | (row << sh)) : w;
+                    } else {
+                        const unsigned long long exm = __ballot(ex);
+                        if (exm != 0ull && lane == 0) {
+                            const uint32_t c = static_cast<uint32_t>(__popcll(exm));
+                            if (use_lds_hist)
+                                atomicAdd(&hist[i * obs + row], c);
+                            else
+                                atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
+                        }
                     }
                 }
                 if (kind == kRuleNoSupport) {   // wave-uniform
@@ -1403,13 +1415,23 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             pay[i] = hold ? pv : pay[i];
             if (hold && !isfinite(pv))
                 record_error(&errors[s_first + i], error_key(kEvalStepBase + last, kStageEvalPayoff, p));
-            const unsigned long long hm = __ballot(hold);
-            if (hm != 0ull && lane == 0) {
-                const uint32_t c = static_cast<uint32_t>(__popcll(hm));
-                if (use_lds_hist)
-                    atomicAdd(&hist[i * obs + last], c);
-                else
-                    atomicAdd(&counts[count_base + (local0 + i) * obs + last], c);
+            if constexpr (DEC) {
+                const uint32_t sr = hold ? last : ((stop_rows[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+                if (sr != 0xFFu) {
+                    if (use_lds_hist)
+                        atomicAdd(&hist[i * obs + sr], 1u);
+                    else
+                        atomicAdd(&counts[count_base + (local0 + i) * obs + sr], 1u);
+                }
+            } else {
+                const unsigned long long hm = __ballot(hold);
+                if (hm != 0ull && lane == 0) {
+                    const uint32_t c = static_cast<uint32_t>(__popcll(hm));
+                    if (use_lds_hist)
+                        atomicAdd(&hist[i * obs + last], c);
+                    else
+                        atomicAdd(&counts[count_base + (local0 + i) * obs + last], c);
+                }
             }
             if (GREEKS) {
                 const bool itm = hold && spot > K;
@@ -1814,10 +1836,11 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
     const uint32_t tiles = (max_strikes + TS - 1) / TS;
     const size_t hist_bytes = static_cast<size_t>(TS) * max_obs * sizeof(uint32_t);
     const int use_lds = hist_bytes <= 32768 ? 1 : 0;
-    // staged decision tables for the multi-tile fused forms when they fit beside the histogram
+    // staged decision tables for the multi-tile fused forms when they fit beside the histogram;
+    // that form keeps 8-bit stopping rows, so it also needs obs <= 255 (implied by the LDS bound)
     const size_t dec_bytes = static_cast<size_t>(TS) * max_obs * kDecDoubles * sizeof(double) + max_obs * sizeof(uint32_t);
     const size_t hist_padded = (hist_bytes + 15) & ~size_t(15);
-    if (Z && TS <= 8 && use_lds && hist_padded + dec_bytes <= 49152) {
+    if (Z && TS <= 8 && use_lds && max_obs <= 255 && hist_padded + dec_bytes <= 49152) {
         hipLaunchKernelGGL((k_evaluate<FAST, TS, G, E, Z, (Z && TS <= 8), WIN>), dim3(xcd_blocks(nitems, tiles)),
                            dim3(kBlock), hist_padded + dec_bytes, st, t, fp, items, nitems, tiles, models, partials,
                            counts, nosupport, errors, sim_fail, use_lds, znorm);
