@@ -1355,9 +1355,29 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                 const bool ex = reg && cand && finite && should_exercise(tie, intr, cont);
                 const bool bad = reg && cand && !finite;
                 const bool tied = reg && cand && finite && (WIN ? near_tie(intr, cont, fp.tie_window) : intr == cont);
-                // state changes only where a lane stops (or a tie is counted): one wave-uniform branch
-                const unsigned long long stopm = __ballot(ex || bad || tied);
-                if (stopm != 0ull) {
+                if constexpr (DEC) {
+                    // state changes only where a lane stops or ties: a per-lane branch, which the
+                    // wave skips when none does (its histogram entry is the lane's stopping row)
+                    if (ex || bad || tied) {
+                        if (tied) atomicAdd(&tie_lds[i], 1u);
+                        const double pv = intr * disc;
+                        if (ex) {
+                            pay[i] = pv;
+                            if (GREEKS) {
+                                dpay[i][0] = od0;
+                                dpay[i][1] = od1;
+                                dpay[i][2] = static_cast<Tan>(od2 - t_row * pv);
+                                dpay[i][3] = od3;
+                            }
+                            const int sh = 8 * (i & 3);
+                            stop_rows[i >> 2] = (stop_rows[i >> 2] & ~(0xFFu << sh)) | (row << sh);
+                        }
+                        if (ex || bad) active &= ~(1u << i);
+                        const bool payoff_bad = ex && !isfinite(pv);
+                        if (bad || payoff_bad) record_eval_errors(&errors[s_first + i], row, p, bad, payoff_bad);
+                    }
+                } else if (__ballot(ex || bad || tied) != 0ull) {
+                    // state changes only where a lane stops (or a tie is counted): one wave-uniform branch
                     const unsigned long long tm = __ballot(tied);
                     if (tm != 0ull && lane == 0) atomicAdd(&tie_lds[i], static_cast<uint32_t>(__popcll(tm)));
                     const double pv = intr * disc;
@@ -1372,19 +1392,13 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                     const bool payoff_bad = ex && !isfinite(pv);
                     if (__ballot(bad || payoff_bad) != 0ull)
                         record_eval_errors(&errors[s_first + i], row, p, bad, payoff_bad);
-                    if constexpr (DEC) {
-                        const int sh = 8 * (i & 3);
-                        const uint32_t w = stop_rows[i >> 2];
-                        stop_rows[i >> 2] = ex ? ((w & ~(0xFFu << sh)) | (row << sh)) : w;
-                    } else {
-                        const unsigned long long exm = __ballot(ex);
-                        if (exm != 0ull && lane == 0) {
-                            const uint32_t c = static_cast<uint32_t>(__popcll(exm));
-                            if (use_lds_hist)
-                                atomicAdd(&hist[i * obs + row], c);
-                            else
-                                atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
-                        }
+                    const unsigned long long exm = __ballot(ex);
+                    if (exm != 0ull && lane == 0) {
+                        const uint32_t c = static_cast<uint32_t>(__popcll(exm));
+                        if (use_lds_hist)
+                            atomicAdd(&hist[i * obs + row], c);
+                        else
+                            atomicAdd(&counts[count_base + (local0 + i) * obs + row], c);
                     }
                 }
                 if (kind == kRuleNoSupport) {   // wave-uniform
