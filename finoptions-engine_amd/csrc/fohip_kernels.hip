@@ -776,14 +776,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
     for (int i = 0; i < ns; ++i)
         alive |= (errors[s_first + i].key == kNoError ? 1u : 0u) << i;
     alive = __builtin_amdgcn_readfirstlane(alive);
-    // cashflows of the next strike are loaded while the current one is processed
+    // cashflows of the next strike are loaded while the current one is processed; a chunk that
+    // lies wholly inside the population (every chunk but a ragged last one) loads without masks
     Store vnext[kBulkPathsPerThread];
+    const bool full_chunk = step > 0 && (item.y + 1) * kBulkChunk <= n;   // wave-uniform
     auto load_values = [&](int i, Store (&dst)[kBulkPathsPerThread]) {
         const Store *V = values + value_base + static_cast<uint64_t>(s_first + i - strike_begin) * n;
+        if (full_chunk) {
 #pragma unroll
-        for (int k = 0; k < kBulkPathsPerThread; ++k) {
-            const uint32_t p = base + k * kBlock;
-            dst[k] = (step > 0 && p < n) ? V[p] : Store(0);
+            for (int k = 0; k < kBulkPathsPerThread; ++k) dst[k] = V[base + k * kBlock];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                const uint32_t p = base + k * kBlock;
+                dst[k] = (step > 0 && p < n) ? V[p] : Store(0);
+            }
         }
     };
     load_values(0, vnext);
@@ -828,8 +835,11 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
                     const bool cand = s1 > K;
                     const bool finite = isfinite(cont);
                     const bool ex = cand && finite && should_exercise(fp.tie, s1 - K, cont);
-                    const bool tie = WIN ? near_tie(s1 - K, cont, fp.tie_window) : (s1 - K) == cont;
-                    tie_wave += static_cast<uint32_t>(__popcll(__ballot(cand && finite && tie)));
+                    // an exact tie implies a finite continuation, so its count is the AND of two
+                    // compare masks
+                    tie_wave += static_cast<uint32_t>(
+                        WIN ? __popcll(__ballot(cand && finite && near_tie(s1 - K, cont, fp.tie_window)))
+                            : __popcll(__ballot((s1 - K) == cont) & __ballot(cand)));
                     const Store vf = static_cast<Store>((s1 - K) * d_next);
                     if (ex) V[p] = vf;
                     v[k] = ex ? static_cast<double>(vf) : static_cast<double>(vv[k]);
