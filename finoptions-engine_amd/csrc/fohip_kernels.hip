@@ -855,7 +855,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
             // failures are rare: one wave-uniform test for all slots
             bool any = false;
 #pragma unroll
-            for (int k = 0; k < kBulkPathsPerThread; ++k) any = any || bad[k] || !isfinite(v[k]);
+            for (int k = 0; k < kBulkPathsPerThread; ++k) any = any | bad[k] | !isfinite(v[k]);   // no short circuit: no exec-mask branches
             if (__ballot(any) != 0ull) {
                 uint32_t bad_bits = 0, nonfinite_bits = 0;
 #pragma unroll
