@@ -827,11 +827,20 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
                     bad[k] = false;
                 }
             } else if (apply) {
+                // continuation_fast's operations with its two leading addends held in vector
+                // registers for the strike: a VOP3 FMA reads at most one scalar operand, so they
+                // would otherwise be copied once per path slot
+                double rule_b = mn->b, rule_h3 = mn->hz[3];
+                asm volatile("" : "+v"(rule_b), "+v"(rule_h3));
 #pragma unroll
                 for (int k = 0; k < kBulkPathsPerThread; ++k) {
                     const uint32_t p = base + k * kBlock;
                     const double s1 = s1v[k];
-                    const double cont = continuation_fast(mn, s1);
+                    const double z = fma(s1, mn->a, rule_b);
+                    double c = fma(mn->hz[4], z, rule_h3);
+                    c = fma_uniform_addend(c, z, mn->hz[2]);
+                    c = fma_uniform_addend(c, z, mn->hz[1]);
+                    const double cont = fma_uniform_addend(c, z, mn->hz[0]);
                     const bool cand = s1 > K;
                     const bool finite = isfinite(cont);
                     const bool ex = cand && finite && should_exercise(fp.tie, s1 - K, cont);
