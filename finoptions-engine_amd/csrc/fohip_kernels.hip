@@ -880,8 +880,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
                 const double s0 = s0v[k];
                 const bool itm = s0 > K;
                 const double u = fma(s0, invK, -pivot);
-                const double y = itm ? v[k] * inv_d : 0.0;
                 double pw = itm ? 1.0 : 0.0;
+                // the response masked by a multiply (exact: v is a finite payoff >= 0 wherever the
+                // row is still alive, and a non-finite v has already failed the row)
+                const double y = (v[k] * inv_d) * pw;
 #pragma unroll
                 for (int j = 0; j <= 8; ++j) {
                     acc[j] += pw;
