@@ -63,6 +63,51 @@ bool canonical_zero_or_positive(double v) {
     return std::isfinite(v) && v >= 0.0 && !(v == 0.0 && std::signbit(v));
 }
 
+// A binary64 value as the reference's messages print it (Rust's `{:?}`): the shortest digits that
+// round-trip, a decimal point on integral values ("1.0"), scientific form outside [1e-4, 1e16).
+struct F64Text {
+    char s[40];
+};
+F64Text rust_f64(double v) {
+    F64Text out{};
+    if (!std::isfinite(v)) {
+        std::snprintf(out.s, sizeof out.s, "%s", std::isnan(v) ? "NaN" : (v > 0 ? "inf" : "-inf"));
+        return out;
+    }
+    if (v == 0.0) {
+        std::snprintf(out.s, sizeof out.s, "%s", std::signbit(v) ? "-0.0" : "0.0");
+        return out;
+    }
+    char digits[40];
+    int prec = 0;
+    for (; prec < 17; ++prec) {   // shortest round-trip mantissa
+        std::snprintf(digits, sizeof digits, "%.*e", prec, v);
+        if (std::strtod(digits, nullptr) == v) break;
+    }
+    const char *e = std::strchr(digits, 'e');
+    const int exp10 = std::atoi(e + 1);
+    std::string mant(digits, e);   // "-d.ddd"
+    const bool neg = mant[0] == '-';
+    std::string ds;
+    for (char c : mant)
+        if (c >= '0' && c <= '9') ds.push_back(c);
+    while (ds.size() > 1 && ds.back() == '0') ds.pop_back();
+    std::string body;
+    if (exp10 < -4 || exp10 >= 16) {
+        body = ds.substr(0, 1);
+        if (ds.size() > 1) body += "." + ds.substr(1);
+        body += "e" + std::to_string(exp10);
+    } else if (exp10 < 0) {
+        body = "0." + std::string(static_cast<size_t>(-exp10 - 1), '0') + ds;
+    } else {
+        const size_t point = static_cast<size_t>(exp10) + 1;
+        if (ds.size() <= point) body = ds + std::string(point - ds.size(), '0') + ".0";
+        else body = ds.substr(0, point) + "." + ds.substr(point);
+    }
+    std::snprintf(out.s, sizeof out.s, "%s%s", neg ? "-" : "", body.c_str());
+    return out;
+}
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t _e = (expr);                                                                \
@@ -1503,7 +1548,8 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
     if (rq->mesh_kind == FOHIP_MESH_UNIFORM && rq->base_intervals == 0)
         return fail(err, FOHIP_E_EXERCISE_MESH, 0, "uniform exercise mesh needs at least one base interval");
     if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES && !has_cash_schedule)
-        return fail(err, FOHIP_E_EXERCISE_MESH, 0, "CashDividendExerciseMesh requires cash-dividend inputs");
+        return fail(err, FOHIP_E_EXERCISE_MESH, 0,
+                    "cash-dividend-date exercise mesh requires a deterministic cash-dividend model");
     if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
         if (!rq->exercise_offsets) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "explicit exercise offsets are missing");
         int rc = fohip_validate_ragged_offsets(rq->exercise_offsets, groups + 1, rq->exercise_events, err);
@@ -1526,14 +1572,22 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
             ex_t.assign(rq->exercise_times + (rq->exercise_offsets[g] - rq->exercise_offsets[0]),
                         rq->exercise_times + (rq->exercise_offsets[g + 1] - rq->exercise_offsets[0]));
         if (rq->mesh_kind == FOHIP_MESH_CASH_DIVIDEND_DATES) {
-            // validate_cash_dividend_exercise_mesh_row (pricing.rs:397-419)
-            if (rate < 0.0)
-                return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu has negative rate %.17g for the cash-dividend mesh",
-                            (size_t)rq->group_offsets[g], rate);
-            for (double t : cash_t)
-                if (t == T)
-                    return fail(err, FOHIP_E_EXERCISE_MESH, 0, "row %zu has terminal cash for the cash-dividend mesh",
-                                (size_t)rq->group_offsets[g]);
+            // validate_cash_dividend_exercise_mesh_row (batch/pricing.rs:816-838)
+            if (rate < 0.0) {
+                set_fields(err, rq->group_offsets[g], -1, -1, -1, rate, 0.0);
+                return fail(err, FOHIP_E_EXERCISE_MESH, 0,
+                            "cash-dividend-date exercise mesh requires a nonnegative continuously compounded rate "
+                            "at batch row %zu; observed %s",
+                            (size_t)rq->group_offsets[g], rust_f64(rate).s);
+            }
+            for (size_t e = 0; e < cash_t.size(); ++e)
+                if (cash_t[e] == T) {
+                    set_fields(err, rq->group_offsets[g], -1, -1, static_cast<int64_t>(e), T, 0.0);
+                    return fail(err, FOHIP_E_EXERCISE_MESH, 0,
+                                "cash-dividend-date exercise mesh does not admit terminal cash at batch row %zu, "
+                                "event %zu; event and maturity are %s",
+                                (size_t)rq->group_offsets[g], e, rust_f64(T).s);
+                }
         }
         if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {
             for (size_t e = 0; e < ex_t.size(); ++e) {
@@ -1541,11 +1595,12 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
                     return fail(err, FOHIP_E_EXERCISE_MESH, 0, "explicit exercise times must be positive and strictly increasing");
                 if (ex_t[e] >= T)
                     return fail(err, FOHIP_E_EXERCISE_MESH, 0,
-                                "row %zu explicit exercise event %zu at %.17g is at or after maturity %.17g",
-                                (size_t)rq->group_offsets[g], e, ex_t[e], T);
+                                "explicit exercise time at batch row %zu, event %zu, time %s must be strictly before "
+                                "maturity %s",
+                                (size_t)rq->group_offsets[g], e, rust_f64(ex_t[e]).s, rust_f64(T).s);
             }
         }
-        // exact_union_times / exact_explicit_union_times (pricing.rs:548-695)
+        // exact_union_times / exact_explicit_union_times (batch/pricing.rs:967-1047, 1049-1114)
         std::vector<double> &times = p.times;
         times.clear();
         if (rq->mesh_kind == FOHIP_MESH_EXPLICIT) {

@@ -1554,7 +1554,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
     }
 }
 
-// Per strike row: the 12-column summary of try_from_parts (batch/pricing.rs:711-812) computed
+// Per strike row: the 12-column summary of try_from_parts (batch/pricing.rs:1130-1230) computed
 // from the fitted rules and stopping counts, plus the first evaluation row whose declared
 // missing-support rule met an active in-the-money path (policy.rs:684-703).
 __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model *__restrict__ models,

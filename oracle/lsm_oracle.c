@@ -1059,7 +1059,8 @@ static int cmp_total(double a, double b) {
     return a < b ? -1 : (a > b ? 1 : 0);
 }
 
-/* exact_union_times (pricing.rs:548-628) */
+/* exact_union_times (crates/finoptions-lsm/src/batch/pricing.rs:967-1047) with next_base_node
+ * (:1116-1122) */
 static uint32_t union_times(double maturity, uint32_t base, const double *cash, uint32_t ncash,
                             double *times) {
     uint32_t count = 0, event = 0;
@@ -1094,7 +1095,7 @@ static uint32_t union_times(double maturity, uint32_t base, const double *cash, 
     return count;
 }
 
-/* exact_explicit_union_times (pricing.rs:630-695) */
+/* exact_explicit_union_times (crates/finoptions-lsm/src/batch/pricing.rs:1049-1114) */
 static uint32_t explicit_union_times(double maturity, const double *ex, uint32_t nex,
                                      const double *cash, uint32_t ncash, double *times) {
     uint32_t count = 0, e = 0, c = 0;
@@ -1340,7 +1341,25 @@ static int group_simulate(const group_plan_t *gp, const plan_t *p, double S0, ui
     return rc;
 }
 
-/* price_group_into (pricing.rs:505-589) + try_from_parts (:711-812). */
+/* price_group_into (pricing.rs:505-589) + try_from_parts (:1130-1230). */
+static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *out,
+                       or_error *err);
+
+/* One group on the calling thread alone (simulate_group_population runs the serial simulator,
+ * pricing.rs:681-701). */
+static int price_group_serial(const or_grouped_input *in, uint32_t g, or_batch_output *out,
+                              or_error *err) {
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    omp_set_num_threads(1);
+#endif
+    int rc = price_group(in, g, out, err);
+#ifdef _OPENMP
+    omp_set_num_threads(saved);
+#endif
+    return rc;
+}
+
 static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *out,
                        or_error *err) {
     group_plan_t gp;
@@ -1453,19 +1472,45 @@ done:
     return rc;
 }
 
+/* GroupedBatchLsmCallInputs::price (pricing.rs:422-503) and BatchLsmCallInputs::price
+ * (pricing.rs:191-254): serial over groups for threads <= 1, else group-parallel (a row batch is
+ * a batch of one-strike groups) with every group priced serially, as the reference's private Rayon
+ * pool does. The error reported is that of the lowest failing group, which is what the reference's
+ * serial replay of the first failure returns. */
 int or_price_grouped(const or_grouped_input *in, or_batch_output *out, int threads,
                      or_error *err) {
     zig_build();
-#ifdef _OPENMP
-    if (threads > 0) omp_set_num_threads(threads);
-#else
-    (void)threads;
-#endif
-    for (uint32_t g = 0; g < in->groups; ++g) {
-        int rc = price_group(in, g, out, err);
-        if (rc) return rc;
+    if (threads <= 1 || in->groups < 2) {
+        for (uint32_t g = 0; g < in->groups; ++g) {
+            int rc = price_group_serial(in, g, out, err);
+            if (rc) return rc;
+        }
+        return 0;
     }
-    return 0;
+    or_error *errs = calloc(in->groups, sizeof(or_error));
+    int *codes = calloc(in->groups, sizeof(int));
+    if (!errs || !codes) {
+        free(errs);
+        free(codes);
+        return set_err(err, E_INTERNAL, PH_NONE, -1, -1, 0, 0, 0);
+    }
+#ifdef _OPENMP
+    omp_set_max_active_levels(1);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int64_t g = 0; g < (int64_t)in->groups; ++g)
+        codes[g] = price_group(in, (uint32_t)g, out, &errs[g]);
+    int rc = 0;
+    for (uint32_t g = 0; g < in->groups; ++g) {
+        if (codes[g]) {
+            rc = codes[g];
+            if (err) *err = errs[g];
+            break;
+        }
+    }
+    free(errs);
+    free(codes);
+    return rc;
 }
 
 int or_grouped_models(const or_grouped_input *in, uint32_t g, or_node_fit *fits_out,
@@ -1605,6 +1650,153 @@ out:
     return rc;
 }
 
+/* The tangent-linear evaluation pass shared by the grouped and the single-call Greeks: every
+ * evaluation path of (seed, domain) is regenerated with its tangents, each strike's stopping row
+ * is decided by its frozen fitted rules, and the discounted payoff tangents are averaged.
+ * `spot_vol` is the supplied volatility (the escrowed spot-matched chain needs it),
+ * `escrowed_volatility` the escrowed convention. */
+static int pathwise_sums(const group_plan_t *gp, const plan_t *p, double S0, double spot_vol,
+                         int escrowed_volatility, const double *strikes, uint32_t nstrikes,
+                         const or_node_fit *fits, int tie_rule, uint64_t seed, uint32_t domain,
+                         uint32_t ne, double *delta, double *vega, double *rho, double *dq,
+                         or_error *err) {
+    int rc = 0;
+    const double sigma = gp->in.volatility;
+    const uint32_t rows = p->obs - 1;
+    double *sums = calloc((size_t)nstrikes * 4, sizeof(double));
+    double *spot_obs = malloc(sizeof(double) * p->obs * 5);
+    /* Escrowed chain factors. The tangents t follow the simulated state F (d/dF0, d/dsigma_F,
+     * drift d/dr, d/dq). The observed stock is F + A_node(r) with F0 = S0 - A0(r) and, spot
+     * matched, sigma_F = sigma S0 / F0. So dS/dS0 = t0 + t1 dsigma_F/dS0,
+     * dS/dsigma = t1 dsigma_F/dsigma, dS/dr = t2 + t0 dF0/dr + t1 dsigma_F/dr + dA_node/dr.
+     * The spot models use the identity factors. */
+    double c_s = 0.0, c_v = 1.0, c_r0 = 0.0, c_rs = 0.0;
+    double *dprepaid = calloc(gp->nodes, sizeof(double));
+    if (gp->escrowed) {
+        /* d/dr of the prepaid recursion: d exp(logD[n+1] - logD[n]) / dr = -(t[n+1] - t[n]) x
+         * the factor; the origin value differentiates sum_k D_k exp(-r t_k). */
+        for (int64_t n = (int64_t)gp->nodes - 2; n >= 0; --n) {
+            double e = exp(gp->logd[n + 1] - gp->logd[n]);
+            double h = gp->times[n + 1] - gp->times[n];
+            dprepaid[n] = dprepaid[n + 1] * e - gp->prepaid[n + 1] * e * h;
+        }
+        double d0 = 0.0;
+        for (uint32_t d = 0; d < gp->ndiv; ++d) {
+            uint32_t node = gp->div_nodes[d];
+            d0 += -gp->times[node] * gp->div_amounts[d] * exp(gp->logd[node]);
+        }
+        dprepaid[0] = d0;
+        dprepaid[gp->nodes - 1] = 0.0;   /* expiry observes no prepaid cash */
+        const double A0 = gp->prepaid[0], F0 = gp->start;
+        c_r0 = -d0;
+        if (escrowed_volatility == 1) {
+            c_s = -spot_vol * A0 / (F0 * F0);
+            c_v = S0 / F0;
+            c_rs = spot_vol * S0 * d0 / (F0 * F0);
+        }
+    }
+    for (uint32_t path = 0; path < ne && !rc; ++path) {
+        stream_t s;
+        stream_init(&s, seed, domain, path, p->intervals);
+        double S = gp->start, t[4] = {1.0, 0.0, 0.0, 0.0};
+        uint32_t observation = 0;
+        if (p->obs_nodes[0] == 0) {
+            spot_obs[0] = S0;
+            spot_obs[1] = 1.0;
+            for (int k = 1; k < 4; ++k) spot_obs[1 + k] = 0.0;
+            observation = 1;
+        }
+        for (uint32_t iv = 0; iv < p->intervals; ++iv) {
+            double z;
+            if (sample_normal(&s, &z)) {
+                rc = set_err(err, E_NORMAL_REJECTION_LIMIT, PH_EVAL_SIM, -1, path, iv, 0, 0);
+                break;
+            }
+            double dt = gp->times[iv + 1] - gp->times[iv];
+            double next = advance(p, iv, S, z);
+            if (S == 0.0) {
+                for (int k = 0; k < 4; ++k) t[k] = 0.0;
+            } else {
+                double growth = next / S;
+                double sq = sqrt(dt);
+                t[0] = t[0] * growth;
+                t[1] = t[1] * growth + next * (-sigma * dt + sq * z);
+                t[2] = t[2] * growth + next * dt;
+                t[3] = t[3] * growth - next * dt;
+            }
+            S = next;
+            uint32_t node = iv + 1;
+            double cash = p->dividend[node];
+            int has_cash = !isnan(cash);
+            int terminal_cash = has_cash && node == p->obs_nodes[p->obs - 1];
+            if (terminal_cash) {
+                double before = S;
+                S = apply_dividend(p->policy, cash, S);
+                if (p->policy == 0 && !(before > cash))
+                    for (int k = 0; k < 4; ++k) t[k] = 0.0;
+            }
+            if (observation < p->obs && p->obs_nodes[observation] == node) {
+                const int expiry = node == gp->nodes - 1;
+                const double add = gp->escrowed && !expiry ? gp->prepaid[node] : 0.0;
+                double *o = spot_obs + observation * 5;
+                o[0] = S + add;
+                o[1] = t[0] + c_s * t[1];
+                o[2] = c_v * t[1];
+                o[3] = t[2] + c_r0 * t[0] + c_rs * t[1] + dprepaid[node];
+                o[4] = t[3];
+                observation += 1;
+            }
+            if (has_cash && !terminal_cash) {
+                double before = S;
+                S = apply_dividend(p->policy, cash, S);
+                if (p->policy == 0 && !(before > cash))
+                    for (int k = 0; k < 4; ++k) t[k] = 0.0;
+            }
+        }
+        if (rc) break;
+        for (uint32_t sidx = 0; sidx < nstrikes; ++sidx) {
+            const or_node_fit *f = fits + (size_t)sidx * p->obs;
+            double K = strikes[sidx];
+            int64_t stop = -1;
+            if (rows > 0 && f[0].kind == 0 && f[0].exercise) continue; /* deterministic */
+            for (uint32_t r = 0; r < rows && stop < 0; ++r) {
+                if (f[r].kind != 3) continue;
+                double spot = spot_obs[r * 5];
+                if (spot <= K) continue;
+                double cont = or_predict(&f[r].model, spot / K);
+                if (should_exercise(tie_rule, spot - K, cont)) stop = r;
+            }
+            uint32_t r = stop < 0 ? rows : (uint32_t)stop;
+            double spot = spot_obs[r * 5];
+            if (!(spot > K)) continue;
+            double tn = gp->times[p->obs_nodes[r]];
+            double disc = exp(gp->logd[p->obs_nodes[r]]);
+            double pay = (spot - K) * disc;
+            sums[sidx * 4 + 0] += spot_obs[r * 5 + 1] * disc;
+            sums[sidx * 4 + 1] += spot_obs[r * 5 + 2] * disc;
+            sums[sidx * 4 + 2] += spot_obs[r * 5 + 3] * disc - tn * pay;
+            sums[sidx * 4 + 3] += spot_obs[r * 5 + 4] * disc;
+        }
+    }
+    for (uint32_t sidx = 0; sidx < nstrikes && !rc; ++sidx) {
+        const or_node_fit *f = fits + (size_t)sidx * p->obs;
+        if (rows > 0 && f[0].kind == 0 && f[0].exercise) {
+            double K = strikes[sidx];
+            delta[sidx] = S0 > K ? 1.0 : 0.0;
+            vega[sidx] = rho[sidx] = dq[sidx] = 0.0;
+            continue;
+        }
+        delta[sidx] = sums[sidx * 4 + 0] / ne;
+        vega[sidx] = sums[sidx * 4 + 1] / ne;
+        rho[sidx] = sums[sidx * 4 + 2] / ne;
+        dq[sidx] = sums[sidx * 4 + 3] / ne;
+    }
+    free(sums);
+    free(spot_obs);
+    free(dprepaid);
+    return rc;
+}
+
 int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *delta, double *vega,
                                double *rho, double *dq, or_error *err) {
     zig_build();
@@ -1637,149 +1829,60 @@ int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t g, double *d
     free(xb);
     free(yb);
     free(ib);
-    if (rc) goto out;
-    {
-        const double S0 = in->initial_spots[g], sigma = gp.in.volatility, rate = in->rates[g];
-        const uint32_t rows = p.obs - 1;
-        const uint32_t ed = in->evaluation_domain_start + g;
-        double *sums = calloc((size_t)strikes * 4, sizeof(double));
-        double *spot_obs = malloc(sizeof(double) * p.obs * 5);
-        /* Escrowed chain factors. The tangents t follow the simulated state F (d/dF0, d/dsigma_F,
-         * drift d/dr, d/dq). The observed stock is F + A_node(r) with F0 = S0 - A0(r) and, spot
-         * matched, sigma_F = sigma S0 / F0. So dS/dS0 = t0 + t1 dsigma_F/dS0,
-         * dS/dsigma = t1 dsigma_F/dsigma, dS/dr = t2 + t0 dF0/dr + t1 dsigma_F/dr + dA_node/dr.
-         * The spot models use the identity factors. */
-        double c_s = 0.0, c_v = 1.0, c_r0 = 0.0, c_rs = 0.0;
-        double *dprepaid = calloc(gp.nodes, sizeof(double));
-        if (gp.escrowed) {
-            /* d/dr of the prepaid recursion: d exp(logD[n+1] - logD[n]) / dr = -(t[n+1] - t[n]) x
-             * the factor; the origin value differentiates sum_k D_k exp(-r t_k). */
-            for (int64_t n = (int64_t)gp.nodes - 2; n >= 0; --n) {
-                double e = exp(gp.logd[n + 1] - gp.logd[n]);
-                double h = gp.times[n + 1] - gp.times[n];
-                dprepaid[n] = dprepaid[n + 1] * e - gp.prepaid[n + 1] * e * h;
-            }
-            double d0 = 0.0;
-            for (uint32_t d = 0; d < gp.ndiv; ++d) {
-                uint32_t node = gp.div_nodes[d];
-                d0 += -gp.times[node] * gp.div_amounts[d] * exp(gp.logd[node]);
-            }
-            dprepaid[0] = d0;
-            dprepaid[gp.nodes - 1] = 0.0;   /* expiry observes no prepaid cash */
-            const double A0 = gp.prepaid[0], F0 = gp.start;
-            c_r0 = -d0;
-            if (in->escrowed_volatility == 1) {
-                const double sv = in->volatilities[g];
-                c_s = -sv * A0 / (F0 * F0);
-                c_v = S0 / F0;
-                c_rs = sv * S0 * d0 / (F0 * F0);
-            }
-        }
-        for (uint32_t path = 0; path < ne && !rc; ++path) {
-            stream_t s;
-            stream_init(&s, in->evaluation_seed, ed, path, p.intervals);
-            double S = gp.start, t[4] = {1.0, 0.0, 0.0, 0.0};
-            uint32_t observation = 0;
-            if (p.obs_nodes[0] == 0) {
-                spot_obs[0] = S0;
-                spot_obs[1] = 1.0;
-                for (int k = 1; k < 4; ++k) spot_obs[1 + k] = 0.0;
-                observation = 1;
-            }
-            for (uint32_t iv = 0; iv < p.intervals; ++iv) {
-                double z;
-                if (sample_normal(&s, &z)) {
-                    rc = set_err(err, E_NORMAL_REJECTION_LIMIT, PH_EVAL_SIM, -1, path, iv, 0, 0);
-                    break;
-                }
-                double dt = gp.times[iv + 1] - gp.times[iv];
-                double next = advance(&p, iv, S, z);
-                if (S == 0.0) {
-                    for (int k = 0; k < 4; ++k) t[k] = 0.0;
-                } else {
-                    double growth = next / S;
-                    double sq = sqrt(dt);
-                    t[0] = t[0] * growth;
-                    t[1] = t[1] * growth + next * (-sigma * dt + sq * z);
-                    t[2] = t[2] * growth + next * dt;
-                    t[3] = t[3] * growth - next * dt;
-                }
-                S = next;
-                uint32_t node = iv + 1;
-                double cash = p.dividend[node];
-                int has_cash = !isnan(cash);
-                int terminal_cash = has_cash && node == p.obs_nodes[p.obs - 1];
-                if (terminal_cash) {
-                    double before = S;
-                    S = apply_dividend(p.policy, cash, S);
-                    if (p.policy == 0 && !(before > cash))
-                        for (int k = 0; k < 4; ++k) t[k] = 0.0;
-                }
-                if (observation < p.obs && p.obs_nodes[observation] == node) {
-                    const int expiry = node == gp.nodes - 1;
-                    const double add = gp.escrowed && !expiry ? gp.prepaid[node] : 0.0;
-                    double *o = spot_obs + observation * 5;
-                    o[0] = S + add;
-                    o[1] = t[0] + c_s * t[1];
-                    o[2] = c_v * t[1];
-                    o[3] = t[2] + c_r0 * t[0] + c_rs * t[1] + dprepaid[node];
-                    o[4] = t[3];
-                    observation += 1;
-                }
-                if (has_cash && !terminal_cash) {
-                    double before = S;
-                    S = apply_dividend(p.policy, cash, S);
-                    if (p.policy == 0 && !(before > cash))
-                        for (int k = 0; k < 4; ++k) t[k] = 0.0;
-                }
-            }
-            if (rc) break;
-            for (uint32_t sidx = 0; sidx < strikes; ++sidx) {
-                const or_node_fit *f = fits + (size_t)sidx * p.obs;
-                double K = in->strikes[first + sidx];
-                int64_t stop = -1;
-                if (rows > 0 && f[0].kind == 0 && f[0].exercise) continue; /* deterministic */
-                for (uint32_t r = 0; r < rows && stop < 0; ++r) {
-                    if (f[r].kind != 3) continue;
-                    double spot = spot_obs[r * 5];
-                    if (spot <= K) continue;
-                    double cont = or_predict(&f[r].model, spot / K);
-                    if (should_exercise(in->tie_rule, spot - K, cont)) stop = r;
-                }
-                uint32_t r = stop < 0 ? rows : (uint32_t)stop;
-                double spot = spot_obs[r * 5];
-                if (!(spot > K)) continue;
-                double tn = gp.times[p.obs_nodes[r]];
-                double disc = exp(gp.logd[p.obs_nodes[r]]);
-                double pay = (spot - K) * disc;
-                sums[sidx * 4 + 0] += spot_obs[r * 5 + 1] * disc;
-                sums[sidx * 4 + 1] += spot_obs[r * 5 + 2] * disc;
-                sums[sidx * 4 + 2] += spot_obs[r * 5 + 3] * disc - tn * pay;
-                sums[sidx * 4 + 3] += spot_obs[r * 5 + 4] * disc;
-            }
-        }
-        (void)rate;
-        for (uint32_t sidx = 0; sidx < strikes; ++sidx) {
-            const or_node_fit *f = fits + (size_t)sidx * p.obs;
-            if (rows > 0 && f[0].kind == 0 && f[0].exercise) {
-                double K = in->strikes[first + sidx];
-                delta[sidx] = S0 > K ? 1.0 : 0.0;
-                vega[sidx] = rho[sidx] = dq[sidx] = 0.0;
-                continue;
-            }
-            delta[sidx] = sums[sidx * 4 + 0] / ne;
-            vega[sidx] = sums[sidx * 4 + 1] / ne;
-            rho[sidx] = sums[sidx * 4 + 2] / ne;
-            dq[sidx] = sums[sidx * 4 + 3] / ne;
-        }
-        free(sums);
-        free(spot_obs);
-        free(dprepaid);
-    }
-out:
+    if (!rc)
+        rc = pathwise_sums(&gp, &p, in->initial_spots[g], in->volatilities[g],
+                           in->escrowed_volatility, in->strikes + first, strikes, fits,
+                           in->tie_rule, in->evaluation_seed, in->evaluation_domain_start + g, ne,
+                           delta, vega, rho, dq, err);
     free(fits);
     plan_free(&p);
     group_plan_free(&gp);
+    return rc;
+}
+
+/* Frozen-policy pathwise sensitivities of one scheduled call (the price_seeded request of
+ * policy.rs:306-336, fitted as or_price_lsm_call fits it). The rate and yield directions are
+ * parallel shifts of the supplied curves: logD(t) - h t and Q(t) + h t. out = delta, vega, rho,
+ * dividend yield. */
+int or_call_pathwise_greeks(const or_plan_input *plan, double initial_spot, const or_estimator *est,
+                            uint32_t training_paths, uint32_t evaluation_paths,
+                            uint64_t training_seed, uint32_t training_domain,
+                            uint64_t evaluation_seed, uint32_t evaluation_domain, double *out,
+                            or_error *err) {
+    if (training_domain == evaluation_domain)
+        return set_err(err, E_RANDOM_DOMAIN_COLLISION, PH_NONE, -1, -1, training_domain, 0, 0);
+    zig_build();
+    plan_t p;
+    plan_build(plan, &p);
+    double *train = malloc(sizeof(double) * p.obs * (size_t)training_paths);
+    double *values = malloc(sizeof(double) * training_paths);
+    double *xb = malloc(sizeof(double) * training_paths), *yb = malloc(sizeof(double) * training_paths);
+    uint32_t *ib = malloc(sizeof(uint32_t) * training_paths);
+    or_node_fit *fits = calloc(p.obs, sizeof(or_node_fit));
+    int rc = simulate_population(&p, initial_spot, training_paths, training_seed, training_domain,
+                                 NULL, train, err, PH_TRAIN_SIM);
+    if (!rc)
+        rc = fit_policy(&p, plan->log_discounts, initial_spot, est, train, training_paths, fits,
+                        values, xb, yb, ib, err);
+    free(train);
+    free(values);
+    free(xb);
+    free(yb);
+    free(ib);
+    if (!rc) {
+        group_plan_t gp;
+        memset(&gp, 0, sizeof gp);
+        gp.times = (double *)plan->times;
+        gp.logd = (double *)plan->log_discounts;
+        gp.nodes = plan->nodes;
+        gp.in = *plan;
+        gp.start = initial_spot;
+        rc = pathwise_sums(&gp, &p, initial_spot, plan->volatility, 0, &est->strike, 1, fits,
+                           est->tie_rule, evaluation_seed, evaluation_domain, evaluation_paths,
+                           &out[0], &out[1], &out[2], &out[3], err);
+    }
+    free(fits);
+    plan_free(&p);
     return rc;
 }
 

@@ -175,12 +175,14 @@ typedef struct {
     uint64_t *near_tie_training, *near_tie_evaluation;   /* optional (NULL skips) */
 } or_batch_output;
 
-/* Builds the transition grid of one group (pricing.rs:548-628, 630-695, 862-965). Returns the
+/* Builds the transition grid of one group (pricing.rs:967-1114, 862-965). Returns the
  * node count, or -1 when `capacity` is too small. Exercise nodes go to exercise_nodes. */
 int or_group_grid(const or_grouped_input *in, uint32_t group, double *times, uint32_t capacity,
                   uint32_t *exercise_nodes, uint32_t *exercise_count, uint32_t *div_nodes,
                   double *div_amounts, uint32_t *div_count);
 
+/* threads <= 1: groups in order on the calling thread; otherwise group-parallel over `threads`
+ * threads with every group serial (the reference's Rayon model); results do not depend on it. */
 int or_price_grouped(const or_grouped_input *in, or_batch_output *out, int threads,
                      or_error *err);
 
@@ -189,6 +191,14 @@ int or_price_grouped(const or_grouped_input *in, or_batch_output *out, int threa
  * training population exactly as or_price_grouped does. */
 int or_grouped_pathwise_greeks(const or_grouped_input *in, uint32_t group, double *delta,
                                double *vega, double *rho, double *dq, or_error *err);
+
+/* The same sensitivities for one scheduled call (or_price_lsm_call's request); the rate and
+ * yield directions shift the supplied curves in parallel. out[4] = delta, vega, rho, yield. */
+int or_call_pathwise_greeks(const or_plan_input *plan, double initial_spot, const or_estimator *est,
+                            uint32_t training_paths, uint32_t evaluation_paths,
+                            uint64_t training_seed, uint32_t training_domain,
+                            uint64_t evaluation_seed, uint32_t evaluation_domain, double *out,
+                            or_error *err);
 
 /* Test utility: group g of eval_in priced under the policy fitted on group g of fit_in;
  * stop_mode 0 decide, 1 decide and record stops[strike * ne + path], 2 hold recorded rows. */

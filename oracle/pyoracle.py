@@ -434,6 +434,27 @@ def price_lsm_call(plan: Plan, initial_spot: float, strike: float, training_path
     return _call_result(plan, res, keep)
 
 
+def call_pathwise_greeks(plan: Plan, initial_spot: float, strike: float, training_paths: int,
+                         evaluation_paths: int, training_seed: int, training_domain: int,
+                         evaluation_seed: int, evaluation_domain: int, degree: int = 4,
+                         rank_tolerance: float = 1e-12, condition_limit: float = 1e8,
+                         tie_rule: int = 0, insufficient_support: int = 0) -> dict:
+    """Frozen-policy pathwise delta, vega, rho and dividend-yield sensitivity of one call."""
+    s, keep_plan = plan.struct()
+    est = _estimator(strike, degree, rank_tolerance, condition_limit, tie_rule,
+                     insufficient_support)
+    out = np.zeros(4)
+    err = OrError()
+    rc = lib().or_call_pathwise_greeks(C.byref(s), C.c_double(initial_spot), C.byref(est),
+                                       C.c_uint32(training_paths), C.c_uint32(evaluation_paths),
+                                       C.c_uint64(training_seed), C.c_uint32(training_domain),
+                                       C.c_uint64(evaluation_seed), C.c_uint32(evaluation_domain),
+                                       _p(out, C.c_double), C.byref(err))
+    if rc:
+        raise OracleError(err)
+    return dict(zip(("delta", "vega", "rho", "dividend_yield"), out.tolist()))
+
+
 def fit_evaluate_paths(plan: Plan, initial_spot: float, strike: float, training: np.ndarray,
                        evaluation: np.ndarray, degree: int = 4, rank_tolerance: float = 1e-12,
                        condition_limit: float = 1e8, tie_rule: int = 0,
@@ -546,6 +567,8 @@ class GroupedRequest:
 
 
 def price_grouped(req: GroupedRequest, threads: int = 0) -> dict[str, np.ndarray]:
+    """The 12 row columns (and tie counts). ``threads`` <= 1 prices groups in order on one thread;
+    more threads price groups in parallel, each group serially (the reference's threading)."""
     s, keep = req.struct()
     rows = len(req.strikes)
     cols = {name: np.zeros(rows, dtype=(np.uint8 if dt is np.bool_ else dt))

@@ -423,3 +423,68 @@ def test_frozen_policy_evaluation_reproduces_the_priced_request(oracle):
     frozen = oracle.grouped_frozen_prices(req, req, 0)["prices"]
     priced = oracle.price_grouped(req)["prices"]
     np.testing.assert_allclose(frozen, priced, rtol=1e-13)
+
+
+def test_single_call_greeks_equal_the_grouped_greeks_on_a_flat_curve(oracle):
+    """The single-call Greek entry (a supplied curve, shifted in parallel) and the grouped entry
+    (a flat rate) share one tangent pass: a call whose curve is exactly -r t_i and whose yields are
+    q t_i gives the grouped strike's Greeks bit for bit, for every strike of the group."""
+    base = _greek_request(oracle)
+    grouped = oracle.grouped_pathwise_greeks(base, 0)
+    grid = oracle.group_grid(base, 0)
+    times = grid["times"].tolist()
+    plan = oracle.Plan(times=times, log_discounts=[0.0] + [-0.03 * t for t in times[1:]],
+                       cum_yields=[0.0] + [0.02 * t for t in times[1:]], volatility=0.25,
+                       exercise_nodes=grid["exercise_nodes"].tolist())
+    for k, strike in enumerate(base.strikes):
+        call = oracle.call_pathwise_greeks(plan, 100.0, float(strike), 4096, 4096, 5, 1, 5, 2,
+                                           insufficient_support=1)
+        for name in ("delta", "vega", "rho", "dividend_yield"):
+            assert call[name] == grouped[name][k], (strike, name)
+
+
+def test_single_call_rho_shifts_a_nonflat_curve_in_parallel(oracle):
+    """Under the headline single-call curve (log discounts 0, -0.01, -0.025 at t = 0, 0.5, 1) the
+    rate direction is logD(t) - h t: the European slice (no exercise node) has the closed-form
+    pathwise rho E[dS_T/dr] e^{logD(T)} - T price, with dS_T/dr = T S_T on every path (no
+    dividend), i.e. rho = T (E[S_T e^{logD}] - price). A tangent that shifted only the last
+    interval's rate, or dropped the discount term, fails it."""
+    times = [0.0, 0.5, 1.0]
+    plan = oracle.Plan(times=times, log_discounts=[0.0, -0.01, -0.025], cum_yields=[0.0] * 3,
+                       volatility=0.2, exercise_nodes=[], expiry_node=2)
+    g = oracle.call_pathwise_greeks(plan, 100.0, 100.0, 1024, 65536, 7, 1, 7, 2)
+    paths = oracle.simulate_seeded(plan, 100.0, 65536, 7, 2)[-1]
+    disc = math.exp(-0.025)
+    itm = paths > 100.0
+    price = float(np.mean(np.where(itm, paths - 100.0, 0.0))) * disc
+    expected = 1.0 * (float(np.mean(np.where(itm, paths, 0.0))) * disc - price)
+    assert g["rho"] == pytest.approx(expected, rel=1e-12)
+    assert g["delta"] == pytest.approx(float(np.mean(np.where(itm, paths / 100.0, 0.0))) * disc,
+                                       rel=1e-12)
+
+
+def test_group_parallel_pricing_equals_serial_pricing(oracle):
+    """The reference's group-parallel batch (pricing.rs:452-503) is bitwise its serial batch; the
+    oracle's parallel form reproduces every column and reports the lowest failing group."""
+    cols = dict(group_offsets=np.array([0, 2, 3, 6, 7], dtype=np.uint32),
+                initial_spots=np.array([100.0, 80.0, 120.0, 95.0]),
+                strikes=np.array([90.0, 110.0, 85.0, 100.0, 120.0, 140.0, 90.0]),
+                maturities=np.array([1.0, 0.5, 2.0, 1.5]), rates=np.array([0.03, 0.0, 0.05, 0.02]),
+                volatilities=np.array([0.25, 0.4, 0.3, 0.2]),
+                dividend_yields=np.array([0.0, 0.01, 0.02, 0.0]))
+    req = oracle.GroupedRequest(training_paths=2048, evaluation_paths=2048, training_seed=3,
+                                evaluation_seed=4, training_domain_start=10,
+                                evaluation_domain_start=100, base_intervals=6,
+                                insufficient_support=1, **cols)
+    serial = oracle.price_grouped(req, threads=1)
+    parallel = oracle.price_grouped(req, threads=4)
+    for name, value in serial.items():
+        assert np.array_equal(np.asarray(value), np.asarray(parallel[name])), name
+    import dataclasses
+
+    failing = dataclasses.replace(req, insufficient_support=0, condition_limit=50.0)
+    with pytest.raises(oracle.OracleError) as a:
+        oracle.price_grouped(failing, threads=1)
+    with pytest.raises(oracle.OracleError) as b:
+        oracle.price_grouped(failing, threads=4)
+    assert (a.value.code, a.value.row, a.value.node) == (b.value.code, b.value.row, b.value.node)
