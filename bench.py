@@ -1,24 +1,27 @@
 #!/usr/bin/env python3
-"""Headline benchmark: grouped-strike LSM path-steps/s including delta/vega, on one MI355X.
+"""Headline benchmark: ragged-batch LSM path-steps/s including delta/vega, on one MI355X.
 
-Workload (one step = one complete pricing request): one simulated state, 64 strikes
-linspace(80, 120), S0 100, r 0.03, q 0.02, sigma 0.25, T 1, 50 equal intervals (49 exercise
-nodes + expiry), 2^20 training + 2^20 evaluation paths, streams (11, 1) / (11, 2); the fused
-method (one HBM pass per exercise node, binary64 storage and decisions), pathwise delta + vega.
-The binary32-storage variant (FP32 bulk) is timed beside it with its accuracy.
-Path-steps = intervals x (training + evaluation paths), the reference's transition unit.
+Workload (one step = one complete pricing request): the ragged batch of 10,000 Bermudan calls,
+numpy default_rng(2024): exercise counts U{4..252} on an equal grid over T, 0-4 cash dividends at
+exercise dates with amounts U[0.1, 2.0], S0 lognormal(ln 100, 0.3), K/S0 U[0.7, 1.3], sigma
+U[0.1, 0.6], T U[0.1, 3.0], r U[0, 0.05], paths per contract log-uniform [16 384, 262 144]
+(training = evaluation), Arrow-style int64 offsets, streams (2024, 1 + row) / (2024, 2 + rows +
+row); the fused method (one HBM pass per exercise node, binary64 storage and decisions) with
+pathwise delta + vega under the frozen policy, all contracts in one request.
+Path-steps = intervals x (training + evaluation paths) per contract, the reference's unit.
 
-Also reported: the plain FP64 method's price time and its price + bump-and-reprice delta/vega
-time (1 + 4 pricings), the accuracy of every fast price and Greek in FP64 standard errors, the
-dominant kernel's roofline fraction (HIP events on the executor's stream) and the CPU checker's
-throughput on a bounded sample of the same workload.
+Also reported: the plain FP64 method's price and price + bump-and-reprice delta/vega time on the
+same batch (1 + 4 pricings), every fast price and Greek against the FP64 method in standard errors,
+the binary32-storage method's time and accuracy, the fit step's roofline (HIP events on the
+executor's stream against the algorithmic bytes 2 s N_train per node), the grouped-strike state
+(64 strikes x 50 intervals, 2^20 + 2^20 paths) and the 4,096 x 256 grouped frame as extra keys,
+and the CPU checker, contract-parallel like the reference, on a bounded sample of the batch.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -31,77 +34,205 @@ sys.path.insert(0, str(ROOT / "finoptions-engine_amd"))
 
 METRIC = "LSM path-steps/sec incl. delta/vega vs plain FP64, 1 GPU, |bias|<0.1 SE"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE at this workload, scaled by
-# a copy-kernel calibration (profiles/c2_fused_hbm_traffic.md); average over the 49 launches
-MEASURED_TRAFFIC = {"k_fit_bulk_step": {"read": 543.2e6, "written": 101.1e6,
-                                        "source": "profiles/c2_fused_hbm_traffic.md"}}
-INTERVALS = 50
-STRIKES = 64
+C4_CONTRACTS = 10_000
+C2_STRIKES, C2_INTERVALS, C2_PATHS = 64, 50, 1 << 20
+C5_GROUPS, C5_STRIKES, C5_PATHS = 4096, 256, 32_768
+BUMP = 1e-4
+# HBM bytes per fit-step launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
+# copy-kernel calibration (factor 2.0 for FETCH_SIZE, 1.0 for WRITE_SIZE); averages over a
+# pricing's launches. None where no counter run of the current kernel exists.
+MEASURED_TRAFFIC: dict[str, dict] = {}
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_request(fh, paths: int, *, spot: float = 100.0, vol: float = 0.25):
+# ------------------------------------------------------------------------------ workloads
+
+def c4_workload():
+    from finoptions_hip.workloads import ragged_batch, ragged_path_steps
+
+    w = ragged_batch(C4_CONTRACTS, seed=2024)
+    return w, ragged_path_steps(w)
+
+
+def price_c4(fh, executor, w, greeks=(), **bump):
+    from finoptions_hip.workloads import ragged_batch_request
+
+    return fh.price_lsm_call_batch(executor=executor, greeks=greeks,
+                                   **ragged_batch_request(w, **bump))
+
+
+def price_c2(fh, executor, greeks=(), *, spot=100.0, vol=0.25, paths=C2_PATHS):
     groups = fh.LsmStrikeGroups(
-        group_offsets=np.array([0, STRIKES], dtype=np.uint32),
-        initial_spots=np.array([spot]), strikes=np.linspace(80.0, 120.0, STRIKES),
+        group_offsets=np.array([0, C2_STRIKES], dtype=np.uint32),
+        initial_spots=np.array([spot]), strikes=np.linspace(80.0, 120.0, C2_STRIKES),
         maturities=np.array([1.0]), rates=np.array([0.03]), volatilities=np.array([vol]),
         dividends=fh.ContinuousDividendYieldBatch(annualized_decimals=np.array([0.02])))
-    mesh = fh.UniformExerciseMesh(base_intervals=INTERVALS,
+    mesh = fh.UniformExerciseMesh(base_intervals=C2_INTERVALS,
                                   origin_exercise=fh.OriginExercise.EXCLUDED)
     method = fh.Lsm(training_paths=paths, evaluation_paths=paths,
                     training_stream=fh.RandomStream(seed=11, domain=1),
                     evaluation_stream=fh.RandomStream(seed=11, domain=2),
                     insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
-    return groups, mesh, method
-
-
-def price(fh, executor, paths, greeks=(), **kw):
-    groups, mesh, method = build_request(fh, paths, **kw)
     return fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=executor,
                                            method=method, greeks=greeks)
 
 
-def algorithmic_bytes(kernel: str, paths: int, real: int) -> int | None:
-    """HBM bytes a launch must move at this workload (paths resident, no re-reads)."""
-    obs = INTERVALS
-    if kernel == "k_simulate":
-        return obs * paths * real                            # write every observation once
-    if kernel == "k_fit_bulk_step":
-        # two spot rows + every strike's value column read once; value writes happen only for
-        # paths that exercise and are data-dependent, so they are left out (a lower bound)
-        return 2 * paths * real + STRIKES * paths * real
-    if kernel == "k_fit_prepare":
-        return 2 * paths * real + 2 * STRIKES * paths * real  # two spot rows, values read+write
-    if kernel == "k_fit_moments":
-        return paths * real + STRIKES * paths * real         # one spot row, values read
-    if kernel == "k_fit_final":
-        return paths * real + STRIKES * paths * real
-    return None
+def c5_pricer(fh):
+    from finoptions_hip.workloads import large_grouped_frame
+
+    w = large_grouped_frame(groups=C5_GROUPS, strikes_per_group=C5_STRIKES)
+    groups = fh.LsmStrikeGroups(
+        group_offsets=w["group_offsets"], initial_spots=w["initial_spots"], strikes=w["strikes"],
+        maturities=w["maturities"], rates=w["rates"], volatilities=w["volatilities"],
+        dividends=fh.CashDividendBatch(
+            schedules=fh.RaggedCashDividends(offsets=w["cash_offsets"], times=w["cash_times"],
+                                             amounts=w["cash_amounts"]),
+            policy=fh.DividendPolicy.LIQUIDATOR,
+            maturity_boundary=fh.MaturityBoundary.STRICTLY_BEFORE))
+    mesh = fh.UniformExerciseMesh(base_intervals=w["intervals"],
+                                  origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = fh.Lsm(training_paths=C5_PATHS, evaluation_paths=C5_PATHS,
+                    training_stream=fh.RandomStream(seed=5, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=5, domain=1 + C5_GROUPS),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    steps = float(C5_GROUPS) * w["intervals"] * 2 * C5_PATHS
+
+    def run(executor, greeks):
+        return fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=executor,
+                                               method=method, greeks=greeks)
+    return run, steps
 
 
-def cpu_baseline(sample_paths: int) -> dict:
+def timed(label, fn, steps, warmup):
+    for i in range(warmup):
+        fn()
+        log(f"{label} warmup {i + 1}/{warmup} done")
+    times, res = [], None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        res = fn()   # synchronous: returns after the device drains and the results are copied
+        times.append(time.perf_counter() - t0)
+        log(f"{label} step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms")
+    return 1e3 * float(np.mean(times)), res
+
+
+def bias_in_se(value, reference, se):
+    se = np.asarray(se, dtype=np.float64)
+    diff = np.abs(np.asarray(value) - np.asarray(reference))
+    # a row whose every payoff is zero has SE 0 and must then agree exactly
+    return np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(diff == 0, 0.0, np.inf))
+
+
+# ------------------------------------------------------------------------------ roofline
+
+def fit_step_roofline(stats: dict, w: dict, real: int = 8) -> dict:
+    """The fit step against HBM: algorithmic bytes 2 s N_train per active group and node (the
+    node's share of the training matrix stored once and read once, 2 s M_obs N_train per group);
+    achieved = those bytes over all launches / the launches' total time."""
+    name = "k_fit_bulk_step"
+    launches, total_ms = stats[name]
+    fit_rows = np.diff(w["exercise_offsets"]).astype(np.float64)   # positive-time exercise rows
+    alg_total = float(np.sum(2.0 * real * w["paths"].astype(np.float64) * fit_rows))
+    achieved = alg_total / (total_ms / 1e3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(name)
+    entry = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1),
+             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+             "traffic": traffic["bytes_per_launch"] if traffic else None,
+             "launches": launches, "avg_launch_us": round(1e3 * total_ms / launches, 2),
+             "algorithmic_bytes_per_launch": round(alg_total / launches),
+             "algorithmic_bytes_definition": "2 x 8 B x N_train per active contract and node"}
+    if traffic:
+        entry["hbm_utilisation_gbs"] = round(
+            traffic["bytes_per_launch"] / (1e-3 * total_ms / launches) / 1e9, 1)
+        entry["traffic_source"] = traffic["source"]
+    return entry
+
+
+# ------------------------------------------------------------------------------ CPU baseline
+
+def _oracle_block(pyoracle, w, count: int):
+    """Contracts [0, count) of the ragged batch as one oracle row batch; group g keeps the domains
+    of row g in the whole batch (training 1 + g, evaluation 2 + rows + g)."""
+    rows = len(w["initial_spots"])
+    eo, co = w["exercise_offsets"][:count + 1], w["cash_offsets"][:count + 1]
+    return pyoracle.GroupedRequest(
+        group_offsets=np.arange(count + 1, dtype=np.uint32),
+        initial_spots=w["initial_spots"][:count], strikes=w["strikes"][:count],
+        maturities=w["maturities"][:count], rates=w["rates"][:count],
+        volatilities=w["volatilities"][:count], cash_offsets=co,
+        cash_times=w["cash_times"][:co[-1]], cash_amounts=w["cash_amounts"][:co[-1]],
+        cash_policy=0, training_paths=int(w["paths"][:count].max()),
+        evaluation_paths=int(w["paths"][:count].max()),
+        training_path_counts=w["paths"][:count], evaluation_path_counts=w["paths"][:count],
+        training_seed=2024, evaluation_seed=2024, training_domain_start=1,
+        evaluation_domain_start=2 + rows, mesh_kind=2, exercise_offsets=eo,
+        exercise_times=w["exercise_times"][:eo[-1]], origin_exercise=0, insufficient_support=1)
+
+
+def cpu_baseline(w, steps_per_contract, fast_res, plain_res, threads: int) -> dict:
+    """The oracle, contract-parallel like the reference's row batch (pricing.rs:220-254: every
+    contract serial, contracts spread over `threads` threads), timed on a bounded sample of the
+    ragged batch (its first contracts, which are i.i.d. draws of the same generator), and on one
+    thread over a smaller block; and, as the checker, the fast and plain GPU prices of the sampled
+    contracts against the oracle's."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
 
-    req = pyoracle.GroupedRequest(
-        group_offsets=np.array([0, STRIKES], dtype=np.uint32), initial_spots=np.array([100.0]),
-        strikes=np.linspace(80.0, 120.0, STRIKES), maturities=np.array([1.0]),
-        rates=np.array([0.03]), volatilities=np.array([0.25]), dividend_yields=np.array([0.02]),
-        training_paths=sample_paths, evaluation_paths=sample_paths, training_seed=11,
-        evaluation_seed=11, training_domain_start=1, evaluation_domain_start=2, mesh_kind=0,
-        base_intervals=INTERVALS, origin_exercise=0, insufficient_support=1)
-    start = time.perf_counter()
-    pyoracle.price_grouped(req, threads=1)
-    seconds = time.perf_counter() - start
-    steps = INTERVALS * 2 * sample_paths
-    return {"value": steps / seconds, "unit": "path-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"same workload at {sample_paths} + {sample_paths} paths (FP64 price only, "
-                       f"single thread, as the reference prices one group serially); "
-                       f"{seconds:.2f} s"),
-            "cpu": _cpu_model()}
+    cum = np.cumsum(steps_per_contract)
+    count = int(np.searchsorted(cum, 1.0e9 * threads / 16)) + 1
+    count_serial = int(np.searchsorted(cum, 1.0e8)) + 1
+    t0 = time.perf_counter()
+    out = pyoracle.price_grouped(_oracle_block(pyoracle, w, count), threads=threads)
+    par_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pyoracle.price_grouped(_oracle_block(pyoracle, w, count_serial), threads=1)
+    ser_s = time.perf_counter() - t0
+    # the sanity case of docs/adr/0006-independent-policy-lsm.md:58-59 (M4 Pro: 101.31 ms at one
+    # thread, 30.11 ms at four): 32 equal steps, logD = -0.06 t, cash 7.0 at node 16 (liquidator),
+    # exercise node 16, sigma 0.30, S0 = K = 100, 65 536 + 65 536 paths, streams (10000, 1) /
+    # (20000, 2) (crates/finoptions-lsm/benches/lsm_call.rs:22-120); the oracle's single call
+    # simulates path-parallel and fits serially, as price_seeded does
+    times = [k / 32 for k in range(33)]
+    plan = pyoracle.Plan(times=times, log_discounts=[0.0] + [-0.06 * k / 32 for k in range(1, 33)],
+                         cum_yields=[0.0] * 33, volatility=0.30, dividend_nodes=[16],
+                         dividend_amounts=[7.0], exercise_nodes=[16], expiry_node=32)
+    sanity = {}
+    for th in (1, 4):
+        pyoracle.set_threads(th)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pyoracle.price_lsm_call(plan, 100.0, 100.0, 65_536, 65_536, 10_000, 1, 20_000, 2)
+            best = min(best, time.perf_counter() - t0)
+        sanity[f"ms_{th}_thread{'s' if th > 1 else ''}"] = round(1e3 * best, 2)
+    sanity.update({"published_m4_pro_ms_1_thread": 101.31, "published_m4_pro_ms_4_threads": 30.11,
+                   "workload": "32 steps, 65536 + 65536 paths, one exercise date, degree 4"})
+    ref_p, ref_se = out["prices"], out["standard_errors"]
+    fast_bias = bias_in_se(fast_res.prices[:count], ref_p, ref_se)
+    plain_rel = float(np.max(np.abs(plain_res.prices[:count] - ref_p)
+                             / np.maximum(np.abs(ref_p), 1e-300)))
+    steps, steps_s = float(cum[count - 1]), float(cum[count_serial - 1])
+    return {
+        "value": steps / par_s, "unit": "path-steps/s", "cores": threads, "kind": "port",
+        "sample": (f"contracts 0..{count - 1} of the ragged batch ({steps:.3e} path-steps), priced "
+                   f"by the oracle contract-parallel over {threads} threads (each contract serial, "
+                   f"as the reference's row batch), FP64 price only; {par_s:.1f} s"),
+        "one_thread": {"value": steps_s / ser_s, "cores": 1,
+                       "sample": f"contracts 0..{count_serial - 1} ({steps_s:.3e} path-steps), "
+                                 f"{ser_s:.1f} s"},
+        "sanity_32_step_case": sanity,
+        "cpu": _cpu_model(),
+        "checker_on_sample": {
+            "contracts": count,
+            "fast_max_price_bias_over_se": float(np.max(fast_bias)),
+            "plain_fp64_max_price_rel_err": plain_rel,
+            "plain_fp64_equal_stopping_counts": bool(
+                np.array_equal(plain_res.expiry_paths[:count], out["expiry_paths"])),
+        },
+    }
 
 
 def _cpu_model() -> str:
@@ -112,6 +243,15 @@ def _cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def _c2_fixture() -> dict | None:
+    path = ROOT / "tests" / "golden" / "c2_full_oracle.json"
+    if not path.exists():
+        return None
+    raw = json.loads(path.read_text())
+    return {k: (np.array([float.fromhex(x) for x in v]) if isinstance(v, list) and isinstance(v[0], str)
+                else (np.array(v) if isinstance(v, list) else v)) for k, v in raw.items()}
 
 
 def copy_calibration() -> int:
@@ -128,15 +268,19 @@ def copy_calibration() -> int:
     return 0
 
 
+# ------------------------------------------------------------------------------ main
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--paths", type=int, default=1 << 20)
-    ap.add_argument("--cpu-sample-paths", type=int, default=1 << 17)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="skip the grouped-strike state, the grouped frame and the FP32-bulk method")
+    ap.add_argument("--only-c4", action="store_true",
+                    help="time the headline pricing only (for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the reference copy for the rocprofv3 FETCH_SIZE / WRITE_SIZE scale "
                          "factors: three elementwise copies of 256 MiB (64 Mi float32)")
@@ -151,113 +295,134 @@ def main() -> int:
 
     import finoptions_hip as fh
 
-    paths = args.paths
     fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
-    bulk32 = fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
-    plain = fh.HipExecutor(device=0, precision=fh.HipPrecision.F64)
     greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
-    log(f"device: {fast.device_name}; paths {paths}+{paths}, {STRIKES} strikes, "
-        f"{INTERVALS} intervals")
+    w, steps_per = c4_workload()
+    path_steps = float(np.sum(steps_per))
+    log(f"device: {fast.device_name}; ragged batch of {C4_CONTRACTS} contracts, "
+        f"{path_steps:.4e} path-steps per pricing")
 
-    def timed(executor, label, steps, warmup):
-        for i in range(warmup):
-            price(fh, executor, paths, greeks)
-            log(f"{label} warmup {i + 1}/{warmup} done")
-        times, res = [], None
-        for i in range(steps):
-            t0 = time.perf_counter()
-            res = price(fh, executor, paths, greeks)   # synchronous: returns after the device drains
-            times.append(time.perf_counter() - t0)
-            log(f"{label} step {i + 1}/{steps}: {times[-1] * 1e3:.2f} ms")
-        return 1e3 * float(np.mean(times)), res
-
-    # ---- fast path: fused FP64 method + pathwise delta/vega
-    ms_fast, fast_res = timed(fast, "fast", args.steps, args.warmup)
-    path_steps = INTERVALS * 2 * paths
+    # ---- headline: fused method + pathwise delta/vega on the whole ragged batch
+    ms_fast, fast_res = timed("c4 fast", lambda: price_c4(fh, fast, w, greeks), args.steps,
+                              args.warmup)
     value = path_steps / (ms_fast / 1e3)
+    if args.only_c4:
+        print(json.dumps({"metric": METRIC, "value": value, "ms_per_step": ms_fast}), flush=True)
+        return 0
 
-    # ---- dominant kernel, timed live with HIP events on the executor's stream
+    # ---- per-kernel times of one headline pricing, HIP events on the executor's stream
     fast.profile(True)
-    price(fh, fast, paths, greeks)
+    price_c4(fh, fast, w, greeks)
     stats = fast.kernel_stats()
     fast.profile(False)
-    dominant = max(stats.items(), key=lambda kv: kv[1][1])
-    name, (launches, total_ms) = dominant
-    avg_ms = total_ms / launches
-    kernel_table = {k: {"launches": n, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / n, 2)}
+    kernel_total = sum(v[1] for v in stats.values())
+    kernel_table = {k: {"launches": n, "total_ms": round(ms, 3), "avg_us": round(1e3 * ms / n, 2),
+                        "share": round(ms / kernel_total, 3)}
                     for k, (n, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1])}
-    log("kernel times (one fast step): " + json.dumps(kernel_table))
-    hbm_kernels = [(k, v) for k, v in stats.items() if algorithmic_bytes(k, paths, 8)]
-    hbm_name, (hbm_n, hbm_ms) = max(hbm_kernels, key=lambda kv: kv[1][1])
-    hbm_avg_ms = hbm_ms / hbm_n
-    alg = algorithmic_bytes(hbm_name, paths, 8)
-    achieved = alg / (hbm_avg_ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "kernel": hbm_name, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (MEASURED_TRAFFIC[hbm_name]["read"] + MEASURED_TRAFFIC[hbm_name]["written"]
-                            if hbm_name in MEASURED_TRAFFIC and paths == 1 << 20 else None),
-                "traffic_source": MEASURED_TRAFFIC.get(hbm_name, {}).get("source"),
-                "avg_launch_us": round(hbm_avg_ms * 1e3, 2),
-                "algorithmic_bytes_per_launch": alg,
-                "dominant_kernel": name, "dominant_avg_launch_us": round(avg_ms * 1e3, 2),
-                "dominant_share": round(total_ms / sum(v[1] for v in stats.values()), 3)}
+    log("kernel times (one headline pricing): " + json.dumps(kernel_table))
+    dominant = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    roofline = fit_step_roofline(stats, w)
+    roofline["dominant_kernel"] = dominant
+    roofline["dominant_share"] = kernel_table[dominant]["share"]
+    generation = {}
+    for k in ("k_simulate", "k_evaluate"):
+        if k in stats:
+            # one simulated path-step per path and interval of the population the kernel covers
+            generation[k] = {"path_steps_per_s": round((path_steps / 2) / (stats[k][1] / 1e3)),
+                             "total_ms": round(stats[k][1], 2)}
+    roofline["generation_kernels"] = generation
 
-    # ---- FP32-bulk storage variant (binary32 spots and cashflows), for comparison
-    ms_bulk32, bulk32_res = timed(bulk32, "f32-bulk", max(1, min(args.steps, 3)), 1)
-
-    # ---- plain FP64: price and price + bump-and-reprice delta/vega (1 + 4 pricings)
-    plain_info = None
-    accuracy = None
+    # ---- plain FP64: price + bump-and-reprice delta/vega (1 + 4 pricings) on the same batch
+    plain_info = accuracy = None
+    plain_res = None
     if not args.skip_plain:
+        plain = fh.HipExecutor(device=0, precision=fh.HipPrecision.F64)
+        price_c4(fh, plain, w)   # allocation warmup for this executor
         t0 = time.perf_counter()
-        base = price(fh, plain, paths)
+        plain_res = price_c4(fh, plain, w)
         t_price = time.perf_counter() - t0
-        h_s, h_v = 1e-4 * 100.0, 1e-4 * 0.25
         t0 = time.perf_counter()
-        up_s = price(fh, plain, paths, spot=100.0 + h_s)
-        dn_s = price(fh, plain, paths, spot=100.0 - h_s)
-        up_v = price(fh, plain, paths, vol=0.25 + h_v)
-        dn_v = price(fh, plain, paths, vol=0.25 - h_v)
+        up_s = price_c4(fh, plain, w, spot_scale=1 + BUMP)
+        dn_s = price_c4(fh, plain, w, spot_scale=1 - BUMP)
+        up_v = price_c4(fh, plain, w, vol_shift=BUMP)
+        dn_v = price_c4(fh, plain, w, vol_shift=-BUMP)
         t_bumps = time.perf_counter() - t0
         ms_plain = 1e3 * (t_price + t_bumps)
-        # the bumped pricings are the baseline's cost; with the policy re-fitted at every bump
-        # their differences are not compared with the frozen-policy pathwise estimator
-        del up_s, dn_s, up_v, dn_v
-        log(f"plain FP64: price {t_price * 1e3:.1f} ms, price+4 bumps {ms_plain:.1f} ms")
-        ref = price(fh, plain, paths, greeks)   # FP64 pathwise Greeks: accuracy reference
-
-        def biases(res):
-            bp = np.abs(res.prices - base.prices) / base.standard_errors
-            bd = (np.abs(res.greeks[fh.Greek.DELTA] - ref.greeks[fh.Greek.DELTA])
-                  / ref.greek_standard_errors[fh.Greek.DELTA])
-            bv = (np.abs(res.greeks[fh.Greek.VEGA] - ref.greeks[fh.Greek.VEGA])
-                  / ref.greek_standard_errors[fh.Greek.VEGA])
-            return bp, bd, bv
-
-        bias_price, bias_delta, bias_vega = biases(fast_res)
-        b32 = biases(bulk32_res)
+        log(f"plain FP64: price {t_price:.2f} s, price + 4 bumps {ms_plain / 1e3:.2f} s")
+        ref = price_c4(fh, plain, w, greeks)   # FP64 pathwise Greeks: the accuracy reference
+        bp = bias_in_se(fast_res.prices, plain_res.prices, plain_res.standard_errors)
+        bd = bias_in_se(fast_res.greeks[fh.Greek.DELTA], ref.greeks[fh.Greek.DELTA],
+                        ref.greek_standard_errors[fh.Greek.DELTA])
+        bv = bias_in_se(fast_res.greeks[fh.Greek.VEGA], ref.greeks[fh.Greek.VEGA],
+                        ref.greek_standard_errors[fh.Greek.VEGA])
         accuracy = {
-            "max_price_bias_over_se": float(bias_price.max()),
-            "max_delta_bias_over_se": float(bias_delta.max()),
-            "max_vega_bias_over_se": float(bias_vega.max()),
-            "all_within_0.1_se": bool(max(bias_price.max(), bias_delta.max(),
-                                          bias_vega.max()) < 0.1),
-            "f32_bulk_max_bias_over_se": {"price": float(b32[0].max()),
-                                          "delta": float(b32[1].max()),
-                                          "vega": float(b32[2].max())},
-            "f32_bulk_strikes_over_0.1_se": int(np.sum(np.maximum.reduce(b32) > 0.1)),
-            "reference": ("plain FP64 GPU method: prices match the CPU checker to 1e-10 "
-                          "relative, pathwise Greeks to 1e-9"),
+            "reference": "plain FP64 GPU method on all 10000 contracts (it matches the oracle to "
+                         "1e-10 relative, see cpu_baseline.checker_on_sample)",
+            "max_price_bias_over_se": float(bp.max()),
+            "max_delta_bias_over_se": float(bd.max()),
+            "max_vega_bias_over_se": float(bv.max()),
+            "all_within_0.1_se": bool(max(bp.max(), bd.max(), bv.max()) < 0.1),
         }
-        plain_info = {"ms_price": round(t_price * 1e3, 2),
-                      "ms_price_bump_greeks": round(ms_plain, 2),
+        plain_info = {"ms_price": round(t_price * 1e3, 1),
+                      "ms_price_bump_greeks": round(ms_plain, 1),
                       "fast_speedup_vs_plain_price_bump_greeks": round(ms_plain / ms_fast, 2),
                       "plain_path_steps_per_s_price_bump_greeks": path_steps * 5 / (ms_plain / 1e3)}
+        del up_s, dn_s, up_v, dn_v
+
+    extras = {}
+    if not args.skip_extras:
+        # ---- binary32 spots and cashflows (FP64 Gram/solve) on the same batch
+        bulk = fh.HipExecutor(device=0, precision=fh.HipPrecision.F32_BULK)
+        ms_b, bulk_res = timed("c4 f32-bulk", lambda: price_c4(fh, bulk, w, greeks), 2, 1)
+        f32 = {"ms_per_step": round(ms_b, 1), "path_steps_per_s": path_steps / (ms_b / 1e3)}
+        if plain_res is not None:
+            b = max(float(bias_in_se(bulk_res.prices, plain_res.prices,
+                                     plain_res.standard_errors).max()),
+                    float(bias_in_se(bulk_res.greeks[fh.Greek.DELTA], ref.greeks[fh.Greek.DELTA],
+                                     ref.greek_standard_errors[fh.Greek.DELTA]).max()))
+            f32["max_bias_over_se"] = b
+            f32["meets_0.1_se_bar"] = bool(b < 0.1)
+        extras["c4_f32_bulk"] = f32
+        del bulk
+        # ---- grouped-strike state at full size
+        c2_steps = C2_INTERVALS * 2 * C2_PATHS
+        ms_c2, c2_res = timed("c2 fast", lambda: price_c2(fh, fast, greeks), 10, 2)
+        c2 = {"workload": "1 state x 64 strikes linspace(80,120), S0 100, r 0.03, q 0.02, sigma "
+                          "0.25, T 1, 50 intervals, 2^20 + 2^20 paths, seed 11 domains 1/2",
+              "ms_per_step": round(ms_c2, 3), "path_steps_per_s": c2_steps / (ms_c2 / 1e3),
+              "contract_equivalent_path_steps_per_s": C2_STRIKES * c2_steps / (ms_c2 / 1e3)}
+        if not args.skip_plain:
+            price_c2(fh, plain)
+            t0 = time.perf_counter()
+            base2 = price_c2(fh, plain)
+            for kw in ({"spot": 100 * (1 + BUMP)}, {"spot": 100 * (1 - BUMP)},
+                       {"vol": 0.25 * (1 + BUMP)}, {"vol": 0.25 * (1 - BUMP)}):
+                price_c2(fh, plain, **kw)
+            ms_p2 = 1e3 * (time.perf_counter() - t0)
+            c2["plain_ms_price_bump_greeks"] = round(ms_p2, 1)
+            c2["fast_speedup_vs_plain_price_bump_greeks"] = round(ms_p2 / ms_c2, 2)
+            fx = _c2_fixture()
+            if fx is not None:
+                c2["fast_max_price_bias_over_se_vs_oracle"] = float(
+                    bias_in_se(c2_res.prices, fx["prices"], fx["standard_errors"]).max())
+                c2["plain_max_price_rel_err_vs_oracle"] = float(
+                    np.max(np.abs(base2.prices - fx["prices"]) / fx["prices"]))
+                c2["oracle_reference"] = "tests/golden/c2_full_oracle.json"
+        extras["c2_grouped_strikes"] = c2
+        # ---- the 4096 x 256 grouped frame
+        run5, steps5 = c5_pricer(fh)
+        ms_c5, _ = timed("c5 fast", lambda: run5(fast, greeks), 2, 1)
+        extras["c5_grouped_frame"] = {
+            "workload": "4096 states x 256 strikes, 12 monthly nodes, quarterly cash, 32768 + "
+                        "32768 paths per state",
+            "ms_per_step": round(ms_c5, 1), "path_steps_per_s": steps5 / (ms_c5 / 1e3),
+            "strike_rows_per_s": C5_GROUPS * C5_STRIKES / (ms_c5 / 1e3)}
 
     cpu = None
-    if not args.skip_cpu:
-        log("timing the CPU checker on a bounded sample ...")
-        cpu = cpu_baseline(args.cpu_sample_paths)
+    if not args.skip_cpu and plain_res is not None:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        log(f"timing the CPU checker on a bounded sample over {threads} threads ...")
+        cpu = cpu_baseline(w, steps_per, fast_res, plain_res, threads)
         cpu["fast_speedup_vs_cpu"] = round(value / cpu["value"], 1)
 
     line = {
@@ -274,24 +439,24 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": (f"grouped-strike LSM call: 1 state x {STRIKES} strikes linspace(80,120), "
-                         f"S0 100, r 0.03, q 0.02, sigma 0.25, T 1, {INTERVALS} intervals, "
-                         f"{paths}+{paths} paths, Philox seed 11 domains 1/2"),
+            "workload": (f"ragged batch of {C4_CONTRACTS} Bermudan calls, seed 2024: exercise "
+                         "counts U{4..252}, 0-4 cash dividends U[0.1,2.0] at exercise dates, S0 "
+                         "lognormal(ln100,0.3), K/S0 U[0.7,1.3], sigma U[0.1,0.6], T U[0.1,3], r "
+                         "U[0,0.05], paths log-uniform [16384,262144] (train = eval), Arrow "
+                         "offsets, one request"),
             "method": ("fused: one HBM pass per exercise node (binary64 spots and cashflows, "
                        "pivoted power sums, moment-form Stieltjes solve), binary64 exercise "
                        "decisions, binary32 Greek tangents"),
-            "insufficient_support": "continue",
-            "greeks": "pathwise delta+vega, frozen policy, evaluation paths rebuilt from normals drawn once per pricing",
+            "greeks": "pathwise delta+vega, frozen policy, evaluation paths regenerated from Philox",
             "path_steps_per_step": path_steps,
             "parallelism": "single GPU",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_fp64": plain_info,
-        "f32_bulk": {"ms_per_step": round(ms_bulk32, 3),
-                     "path_steps_per_s": path_steps / (ms_bulk32 / 1e3)},
         "accuracy": accuracy,
         "kernels": kernel_table,
+        **extras,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -26,6 +26,13 @@
 
 using namespace fohip;
 
+// The fused fit step's unmasked-load test at the largest path counts the engine accepts: the last
+// (ragged) chunk of 2^32 - 1 paths is chunk 2^21 - 1, whose end 2^32 lies past the population.
+static_assert(bulk_chunk_is_full(0, kBulkChunk) && !bulk_chunk_is_full(0, kBulkChunk - 1), "chunk bound");
+static_assert(!bulk_chunk_is_full((1u << 21) - 1, 0xFFFFFFFFu), "chunk bound: 32-bit wrap at 2^32 - 1 paths");
+static_assert(bulk_chunk_is_full((1u << 21) - 2, 0xFFFFFFFFu), "chunk bound: next-to-last chunk is full");
+static_assert(bulk_chunk_is_full((1u << 20) - 1, 1u << 31), "chunk bound: exact multiple");
+
 // ================================================================================ errors
 
 namespace {
@@ -86,7 +93,7 @@ F64Text rust_f64(double v) {
     }
     const char *e = std::strchr(digits, 'e');
     const int exp10 = std::atoi(e + 1);
-    std::string mant(digits, e);   // "-d.ddd"
+    std::string mant(static_cast<const char *>(digits), e);   // "-d.ddd"
     const bool neg = mant[0] == '-';
     std::string ds;
     for (char c : mant)
@@ -335,8 +342,8 @@ struct fohip_executor {
     double near_tie_window = 0.0;   // 0: exact ties; w > 0: |intrinsic - continuation| <= w |continuation|
     size_t batch_budget = 0;   // bytes for paths + values per batch
     std::mutex mu;
-    DevBuf paths, values, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t, eval_out,
-        summary, replay, probe_a, probe_b, probe_c, znorm, ties;
+    DevBuf paths, values, stops, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t,
+        eval_out, summary, replay, probe_a, probe_b, probe_c, znorm, ties;
     PinnedBuf staging;   // per-batch tables and items on the way in, results on the way out
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
@@ -463,7 +470,7 @@ extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     if (!ex) return FOHIP_OK;
     (void)hipSetDevice(ex->device);
     (void)hipStreamSynchronize(ex->stream);
-    for (DevBuf *b : {&ex->paths, &ex->values, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
+    for (DevBuf *b : {&ex->paths, &ex->values, &ex->stops, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
                       &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t,
                       &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm,
                       &ex->ties})
@@ -512,7 +519,8 @@ extern "C" int fohip_reset_profile(fohip_executor *ex) {
 
 extern "C" int fohip_set_tuning(fohip_executor *ex, const fohip_tuning *t) {
     if (!ex || !t) return FOHIP_E_INVALID_ARGUMENT;
-    if (t->eval_strike_tile > 16) return FOHIP_E_INVALID_ARGUMENT;
+    if (t->eval_strike_tile > 16 || t->fit_cashflows > FOHIP_CASHFLOWS_STOPPING_ROWS) return FOHIP_E_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ex->mu);
     ex->tuning = *t;
     return FOHIP_OK;
 }
@@ -749,20 +757,66 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
     const size_t real = f32 ? sizeof(float) : sizeof(double);
     out.assign(strikes.size(), RowOut());
     // memory-bounded batches of consecutive groups
+    // Cashflow encoding of the fused backward induction, one choice per request: stopping rows
+    // (1 byte per strike and path) when every group has at most 255 observation rows and the
+    // tuning asks for them. The default keeps value columns: the fit step is bound by its
+    // instruction issue, not by the column bytes, and the stopping rows' gathers cost more than
+    // the bytes they save (measured on the 64-strike, 256-strike and single-strike shapes).
+    bool use_stops = false;
+    uint32_t most_strikes = 0;
+    size_t most_obs = 0;
+    for (const GroupPlan &p : all) {
+        most_strikes = std::max<uint32_t>(most_strikes, p.strike_end - p.strike_begin);
+        most_obs = std::max<size_t>(most_obs, p.obs_nodes.size());
+    }
+    if (spec.fused) {
+        const uint32_t mode = ex->tuning.fit_cashflows;
+        use_stops = most_obs <= 255 && mode == FOHIP_CASHFLOWS_STOPPING_ROWS;
+    }
+    // The evaluation's strike tile, one per request (results do not depend on it); windowed
+    // near-tie counts run the evaluation's diagnostic form over single-strike tiles.
+    const bool greeks = spec.greeks != 0;
+    const uint32_t tile = ex->near_tie_window != 0.0 ? 1u
+                                                     : eval_tile_for(most_strikes, greeks, ex->tuning.eval_strike_tile);
+    // Batches of consecutive groups bounded by memory (paths, cashflows and stored normals within
+    // the batch budget) and by the launch grids: a dispatch holds at most 2^32 - 1 work-items, so
+    // no kernel of a batch may have more than kMaxBlocks blocks of 256 threads. A grid is (work
+    // items of the batch, rounded up to 8 for the XCD mapping) x (strike columns of the batch).
+    constexpr uint64_t kMaxBlocks = 0xFFFFFFFFull / 256;   // 256-thread blocks
+    auto grid_blocks = [&](uint64_t items_t, uint64_t items_e, uint64_t items_b, uint64_t strikes) {
+        const uint64_t fit = ((items_t + 7) / 8) * 8 * strikes;
+        const uint64_t eval = ((items_e + 7) / 8) * 8 * ((strikes + tile - 1) / tile);
+        const uint64_t bulk = ((items_b + 7) / 8) * 8 * ((strikes + kBulkStrikes - 1) / kBulkStrikes);
+        return std::max({fit, eval, bulk, items_e});
+    };
     std::vector<std::pair<size_t, size_t>> batches;
     {
         size_t begin = 0, bytes = 0;
+        uint64_t it_t = 0, it_e = 0, it_b = 0, cols = 0;
         for (size_t g = 0; g < all.size(); ++g) {
             const GroupPlan &p = all[g];
-            size_t need = (p.obs_nodes.size() + (p.strike_end - p.strike_begin)) * p.nt * real;
-            if (spec.fused && p.strike_end - p.strike_begin > 1)   // stored evaluation normals
+            const uint64_t ns = p.strike_end - p.strike_begin;
+            size_t need = p.obs_nodes.size() * p.nt * real + ns * p.nt * (use_stops ? 1 : real);
+            if (spec.fused && ns > 1)   // stored evaluation normals
                 need += (p.times.size() - 1) * p.ne * sizeof(double);
-            if (g > begin && bytes + need > ex->batch_budget) {
+            const uint64_t ct = (p.nt + kChunk - 1) / kChunk, ce = (p.ne + kEvalChunk - 1) / kEvalChunk,
+                           cb = (p.nt + kBulkChunk - 1) / kBulkChunk;
+            if (grid_blocks(ct, ce, cb, ns) > kMaxBlocks)
+                return fail(err, FOHIP_E_UNSUPPORTED, 0,
+                            "group %zu (%llu strikes, %llu + %llu paths) exceeds one launch grid", g,
+                            (unsigned long long)ns, (unsigned long long)p.nt, (unsigned long long)p.ne);
+            if (g > begin && (bytes + need > ex->batch_budget ||
+                              grid_blocks(it_t + ct, it_e + ce, it_b + cb, std::max(cols, ns)) > kMaxBlocks)) {
                 batches.push_back({begin, g});
                 begin = g;
                 bytes = 0;
+                it_t = it_e = it_b = cols = 0;
             }
             bytes += need;
+            it_t += ct;
+            it_e += ce;
+            it_b += cb;
+            cols = std::max(cols, ns);
         }
         batches.push_back({begin, all.size()});
     }
@@ -846,7 +900,10 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         }
         HIP_TRY(ex->tables.ensure(tables_bytes));
         HIP_TRY(ex->paths.ensure(std::max<uint64_t>(path_elems, 1) * real));
-        HIP_TRY(ex->values.ensure(std::max<uint64_t>(value_elems, 1) * real));
+        if (use_stops)
+            HIP_TRY(ex->stops.ensure((std::max<uint64_t>(value_elems, 1) + 3) & ~uint64_t(3)));
+        else
+            HIP_TRY(ex->values.ensure(std::max<uint64_t>(value_elems, 1) * real));
         HIP_TRY(ex->models.ensure(std::max<uint64_t>(model_count, 1) * sizeof(Model)));
         HIP_TRY(ex->part_t.ensure(std::max<uint64_t>(part_t, 1) * kPartialStride * sizeof(double)));
         HIP_TRY(ex->part_e.ensure(std::max<uint64_t>(part_e, 1) * kPartialStride * sizeof(double)));
@@ -932,6 +989,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         add_fill(ex->fail_t.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
         add_fill(ex->fail_e.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
         add_fill(ex->ties.ptr, ties_h.size() * sizeof(unsigned long long), 0u);
+        // every path starts stopped at expiry: stopping byte 0
+        if (use_stops) add_fill(ex->stops.ptr, (value_elems + 3) & ~uint64_t(3), 0u);
+        uint8_t *stops = use_stops ? ex->stops.as<uint8_t>() : nullptr;
         HIP_TRY(launch_fill(st, fills));
         unsigned long long *ties_t = ex->ties.as<unsigned long long>();
         Model *models = ex->models.as<Model>();
@@ -946,7 +1006,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             const uint32_t ni = step_items[j], nr = step_rows[j];
             if (spec.fused) {   // one fused HBM pass per backward step
                 PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, ex->paths.ptr,
-                                                                    ex->values.ptr, models, ex->part_t.as<double>(),
+                                                                    ex->values.ptr, stops, models, ex->part_t.as<double>(),
                                                                     errors));
                 PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
                                                                         models, errors));
@@ -965,12 +1025,9 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             PROF_LAUNCH("k_fit_gate", launch_fit_gate(st, t, fp, rows, nrows, models, static_cast<uint32_t>(model_count),
                                                       errors));
         PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
-                                 ex->values.ptr, models, ex->part_t.as<double>(), errors, ties_t));
+                                 ex->values.ptr, stops, models, ex->part_t.as<double>(), errors, ties_t));
         PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));
         // ---- evaluation
-        const bool greeks = spec.greeks != 0;
-        // windowed near-tie counts run the evaluation's diagnostic form over single-strike tiles
-        const uint32_t tile = fp.tie_window != 0.0 ? 1u : eval_tile_for(max_strikes, greeks, ex->tuning.eval_strike_tile);
         bool escrow = false;   // the escrowed instantiation also serves spot-model groups
         for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
         // Several strike tiles read one set of evaluation normals from HBM instead of each

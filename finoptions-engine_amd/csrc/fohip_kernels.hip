@@ -779,7 +779,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
     // cashflows of the next strike are loaded while the current one is processed; a chunk that
     // lies wholly inside the population (every chunk but a ragged last one) loads without masks
     Store vnext[kBulkPathsPerThread];
-    const bool full_chunk = step > 0 && (item.y + 1) * kBulkChunk <= n;   // wave-uniform
+    const bool full_chunk = step > 0 && bulk_chunk_is_full(item.y, n);   // wave-uniform
     auto load_values = [&](int i, Store (&dst)[kBulkPathsPerThread]) {
         const Store *V = values + value_base + static_cast<uint64_t>(s_first + i - strike_begin) * n;
         if (full_chunk) {
@@ -901,6 +901,169 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
     }
     __syncthreads();
     // slots 0..13 power sums, 14 the pivot, 15 the tied rule applications (an integer)
+    for (int idx = threadIdx.x; idx < ns * 16; idx += kBlock) {
+        const int i = idx / 16, q = idx % 16;
+        const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
+        const uint32_t local = s_first + i - strike_begin;
+        partials[static_cast<uint64_t>(g->part_base_t + local * g->chunks_t + item.y) * kPartialStride + q] = v;
+    }
+}
+
+// The fused backward step with stopping-row cashflows. A path's cashflow for a strike is a function
+// of its stopping row alone, v_o = max(S[stop] - K, 0) exp(logD(stop)), with the spot S[stop] of
+// the group's own training matrix; so each (strike, path) keeps one byte, e = obs - 1 - stop
+// (0: expiry, the state every path starts in; a path exercising in step j stops at row
+// r + 1 = obs - 1 - j, i.e. e = j). The response of an in-the-money path at row r is that cashflow
+// rebuilt from a gather of S[stop] (all strikes of a group share its spot matrix, and the strike
+// sets of one path chunk run back to back on one XCD, so the gathers mostly hit its L2) over
+// exp(logD(r)). The arithmetic is the value-column kernel's operation for operation, so the two
+// encodings give bitwise the same power sums; the stopping bytes cost 1 B of HBM per strike and
+// path and node instead of the binary64 column's 8 (plus 8 B per exercise rewrite).
+template <typename Store, bool WIN>
+__global__ __launch_bounds__(kBlock, 4) void k_fit_stop_step(DevTables t, FitParams fp, uint32_t step,
+                                                          const uint2 *__restrict__ items, uint32_t nitems,
+                                                          uint32_t sets, const Store *__restrict__ paths,
+                                                          uint8_t *__restrict__ stops, const Model *__restrict__ models,
+                                                          double *__restrict__ partials,
+                                                          RowError *__restrict__ errors) {
+    __shared__ double red[kWaves][kBulkStrikes][16];
+    __shared__ double transpose[kWaves][14 * 66];
+    __shared__ double disc_rows[256];   // exp(logD) of the group's observation rows (obs <= 255)
+    uint32_t wi, set;
+    if (!xcd_item_column(nitems, sets, wi, set)) return;
+    const uint2 item = load_item(items, wi);
+    const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
+    const uint32_t strike_begin = g->strike_begin, strike_end = g->strike_end;
+    const uint32_t s_first = strike_begin + set * kBulkStrikes;
+    if (s_first >= strike_end || step >= g->fit_rows) return;
+    const int ns = static_cast<int>(min(static_cast<uint32_t>(kBulkStrikes), strike_end - s_first));
+    const uint32_t obs = g->obs, r = obs - 2 - step, n = g->nt;
+    const Store *S_group = paths + g->path_base;
+    const Store *S_r = S_group + static_cast<uint64_t>(r) * n;
+    const Store *S_next = S_r + n;
+    const uint32_t base = item.y * kBulkChunk + threadIdx.x;
+    const uint32_t row_base = g->row_base;
+    for (uint32_t i = threadIdx.x; i < obs; i += kBlock) disc_rows[i] = t.disc0[row_base + i];
+    Store s0v[kBulkPathsPerThread], s1v[kBulkPathsPerThread];
+#pragma unroll
+    for (int k = 0; k < kBulkPathsPerThread; ++k) {
+        const uint32_t p = base + k * kBlock;
+        s0v[k] = p < n ? S_r[p] : Store(0);
+        s1v[k] = p < n ? S_next[p] : Store(0);
+    }
+    const double d_next = *uniform(t.disc0 + row_base + r + 1);
+    const double inv_d = 1.0 / *uniform(t.disc0 + row_base + r);
+    const uint32_t model_row = g->model_base + r + 1;
+    const uint64_t stop_base = g->value_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t alive = 0;
+    for (int i = 0; i < ns; ++i)
+        alive |= (errors[s_first + i].key == kNoError ? 1u : 0u) << i;
+    alive = __builtin_amdgcn_readfirstlane(alive);
+    // stopping bytes of the next strike are loaded while the current one is processed (lanes past
+    // n read nothing and keep e = 0, whose row they never gather)
+    uint32_t enext[kBulkPathsPerThread];
+    auto load_stops = [&](int i, uint32_t (&dst)[kBulkPathsPerThread]) {
+        const uint8_t *E = stops + stop_base + static_cast<uint64_t>(s_first + i - strike_begin) * n;
+#pragma unroll
+        for (int k = 0; k < kBulkPathsPerThread; ++k) {
+            const uint32_t p = base + k * kBlock;
+            dst[k] = (step > 0 && p < n) ? static_cast<uint32_t>(E[p]) : 0u;
+        }
+    };
+    load_stops(0, enext);
+    __syncthreads();   // disc_rows
+    for (int i = 0; i < kBulkStrikes; ++i) {
+        if (i >= ns) break;
+        uint32_t ev[kBulkPathsPerThread];
+#pragma unroll
+        for (int k = 0; k < kBulkPathsPerThread; ++k) ev[k] = enext[k];
+        if (i + 1 < ns) load_stops(i + 1, enext);
+        const uint32_t srow = s_first + i, local = srow - strike_begin;
+        double acc[14];
+#pragma unroll
+        for (int q = 0; q < 14; ++q) acc[q] = 0.0;
+        const FOHIP_UNIFORM Model *mn = uniform(models + model_row + local * obs);
+        const bool apply = step > 0 && mn->kind == kRuleRegression;   // wave-uniform
+        const double pivot = apply ? mn->mean : 1.0;
+        const double K = *uniform(t.strikes + srow);
+        const double invK = *uniform(t.inv_strikes + srow);
+        uint32_t tie_wave = 0;
+        if (alive & (1u << i)) {
+            uint8_t *E = stops + stop_base + static_cast<uint64_t>(local) * n;
+            // the current cashflow of every path in the money at row r: its stopping row's spot,
+            // gathered before the decision so that the loads overlap the continuation
+            Store sg[kBulkPathsPerThread];
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                const uint32_t p = base + k * kBlock;
+                const bool need = static_cast<double>(s0v[k]) > K;   // false for lanes past n
+                sg[k] = need ? S_group[static_cast<uint64_t>(obs - 1 - ev[k]) * n + p] : Store(0);
+            }
+            double v[kBulkPathsPerThread];
+            bool bad[kBulkPathsPerThread];
+            double rule_b = apply ? mn->b : 0.0, rule_h3 = apply ? mn->hz[3] : 0.0;
+            asm volatile("" : "+v"(rule_b), "+v"(rule_h3));
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                const uint32_t p = base + k * kBlock;
+                const double s1 = s1v[k];
+                bool ex = false, cand = false, finite = true;
+                if (apply) {
+                    const double z = fma(s1, mn->a, rule_b);
+                    double c = fma(mn->hz[4], z, rule_h3);
+                    c = fma_uniform_addend(c, z, mn->hz[2]);
+                    c = fma_uniform_addend(c, z, mn->hz[1]);
+                    const double cont = fma_uniform_addend(c, z, mn->hz[0]);
+                    cand = s1 > K;
+                    finite = isfinite(cont);
+                    ex = cand && finite && should_exercise(fp.tie, s1 - K, cont);
+                    tie_wave += static_cast<uint32_t>(
+                        WIN ? __popcll(__ballot(cand && finite && near_tie(s1 - K, cont, fp.tie_window)))
+                            : __popcll(__ballot((s1 - K) == cont) & __ballot(cand)));
+                    if (ex) E[p] = static_cast<uint8_t>(step);
+                }
+                const uint32_t stop = obs - 1 - ev[k];
+                const double held = static_cast<double>(sg[k]);
+                const double vf = (held > K ? held - K : 0.0) * disc_rows[stop];
+                v[k] = ex ? (s1 - K) * d_next : vf;
+                bad[k] = cand && !finite;
+            }
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) any = any | bad[k] | !isfinite(v[k]);
+            if (__ballot(any) != 0ull) {
+                uint32_t bad_bits = 0, nonfinite_bits = 0;
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    bad_bits |= (bad[k] ? 1u : 0u) << k;
+                    nonfinite_bits |= (!isfinite(v[k]) ? 1u : 0u) << k;
+                }
+                record_fit_errors(&errors[srow], step, base, n, bad_bits, nonfinite_bits);
+            }
+#pragma unroll
+            for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                const double s0 = s0v[k];
+                const bool itm = s0 > K;
+                const double u = fma(s0, invK, -pivot);
+                double pw = itm ? 1.0 : 0.0;
+                const double y = (v[k] * inv_d) * pw;
+#pragma unroll
+                for (int j = 0; j <= 8; ++j) {
+                    acc[j] += pw;
+                    if (j <= 4) acc[9 + j] = fma(y, pw, acc[9 + j]);
+                    pw *= u;
+                }
+            }
+        }
+        const double total = wave_sums_lds<14>(acc, transpose[wave]);
+        if ((lane & 3) == 0 && lane < 56) red[wave][i][lane >> 2] = total;
+        if (lane == 0) {
+            red[wave][i][14] = pivot;
+            red[wave][i][15] = static_cast<double>(tie_wave);
+        }
+    }
+    __syncthreads();
     for (int idx = threadIdx.x; idx < ns * 16; idx += kBlock) {
         const int i = idx / 16, q = idx % 16;
         const double v = q == 14 ? red[0][i][14] : (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
@@ -1087,10 +1250,12 @@ __device__ __forceinline__ void solve_node_from_sums(Model &m, const FitParams &
 // cashflows for the valuation-origin decision (policy.rs:267-281). Stored values are either
 // discounted to the last fitted node (FP64 method, the reference's representation and arithmetic)
 // or values already discounted to the origin (fused methods, evaluation-form decisions).
-template <typename Real, bool FUSED>
+// STOPS: the fused cashflows are stopping bytes (see k_fit_stop_step), rebuilt here per path.
+template <typename Real, bool FUSED, bool STOPS = false>
 __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                       uint32_t nitems, uint32_t columns, const Real *__restrict__ paths,
-                                                      const Real *__restrict__ values, const Model *__restrict__ models,
+                                                      const Real *__restrict__ values, const uint8_t *__restrict__ stops,
+                                                      const Model *__restrict__ models,
                                                       double *__restrict__ partials, RowError *__restrict__ errors,
                                                       unsigned long long *__restrict__ ties) {
     constexpr bool kBulk = FUSED;
@@ -1123,7 +1288,13 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
         if (g.fit_rows == 0) {
             v = s1 > K ? s1 - K : 0.0;
         } else {
-            v = V[p];
+            if constexpr (STOPS) {
+                const uint32_t stop = g.obs - 1 - static_cast<uint32_t>(stops[g.value_base + static_cast<uint64_t>(local) * n + p]);
+                const double held = paths[g.path_base + static_cast<uint64_t>(stop) * n + p];
+                v = (held > K ? held - K : 0.0) * t.disc0[g.row_base + stop];
+            } else {
+                v = V[p];
+            }
             if (apply && s1 > K) {
                 const double cont = kBulk ? continuation_fast(&m, s1) : predict_ref(&m, fp.degree, s1 / K);
                 if (!isfinite(cont)) {
@@ -1131,7 +1302,8 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
                 } else {
                     tie_count += is_tie(s1 - K, cont, fp.tie_window) ? 1u : 0u;
                     if (should_exercise(fp.tie, s1 - K, cont))
-                        v = kBulk ? static_cast<double>(static_cast<Real>((s1 - K) * d_last)) : s1 - K;
+                        v = STOPS ? (s1 - K) * d_last
+                                  : (kBulk ? static_cast<double>(static_cast<Real>((s1 - K) * d_last)) : s1 - K);
                 }
             }
         }
@@ -1796,7 +1968,8 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
 
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors) {
+                                void *values, uint8_t *stops, const Model *models, double *partials,
+                                RowError *errors) {
     if (nitems == 0) return hipSuccess;
     const uint32_t sets = (max_strikes + kBulkStrikes - 1) / kBulkStrikes;
     const dim3 grid(xcd_blocks(nitems, sets));
@@ -1804,7 +1977,18 @@ hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, co
 #define FOHIP_BULK(REAL, WINV)                                                                                  \
     hipLaunchKernelGGL((k_fit_bulk_step<REAL, WINV>), grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets, \
                        static_cast<const REAL *>(paths), static_cast<REAL *>(values), models, partials, errors)
-    if (f32) {
+#define FOHIP_STOP(REAL, WINV)                                                                                  \
+    hipLaunchKernelGGL((k_fit_stop_step<REAL, WINV>), grid, dim3(kBlock), 0, st, t, fp, step, items, nitems, sets, \
+                       static_cast<const REAL *>(paths), stops, models, partials, errors)
+    if (stops) {
+        if (f32) {
+            if (win) FOHIP_STOP(float, true);
+            else FOHIP_STOP(float, false);
+        } else {
+            if (win) FOHIP_STOP(double, true);
+            else FOHIP_STOP(double, false);
+        }
+    } else if (f32) {
         if (win) FOHIP_BULK(float, true);
         else FOHIP_BULK(float, false);
     } else {
@@ -1812,6 +1996,7 @@ hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, co
         else FOHIP_BULK(double, false);
     }
 #undef FOHIP_BULK
+#undef FOHIP_STOP
     return hipGetLastError();
 }
 
@@ -1825,22 +2010,24 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
 
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                            const void *values, const Model *models, double *partials, RowError *errors,
-                            unsigned long long *ties) {
+                            const void *values, const uint8_t *stops, const Model *models, double *partials,
+                            RowError *errors, unsigned long long *ties) {
     if (nitems == 0) return hipSuccess;
     const dim3 grid(xcd_blocks(nitems, max_strikes));
-    if (f32)
-        hipLaunchKernelGGL((k_fit_final<float, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
-                           static_cast<const float *>(paths), static_cast<const float *>(values), models, partials,
-                           errors, ties);
-    else if (fused)
-        hipLaunchKernelGGL((k_fit_final<double, true>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
-                           static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
-                           errors, ties);
-    else
-        hipLaunchKernelGGL((k_fit_final<double, false>), grid, dim3(kBlock), 0, st, t, fp, items, nitems, max_strikes,
-                           static_cast<const double *>(paths), static_cast<const double *>(values), models, partials,
-                           errors, ties);
+#define FOHIP_FINAL(REAL, FUSEDV, STOPSV)                                                                          \
+    hipLaunchKernelGGL((k_fit_final<REAL, FUSEDV, STOPSV>), grid, dim3(kBlock), 0, st, t, fp, items, nitems,         \
+                       max_strikes, static_cast<const REAL *>(paths), static_cast<const REAL *>(values), stops, models, \
+                       partials, errors, ties)
+    if (f32) {
+        if (stops) FOHIP_FINAL(float, true, true);
+        else FOHIP_FINAL(float, true, false);
+    } else if (fused) {
+        if (stops) FOHIP_FINAL(double, true, true);
+        else FOHIP_FINAL(double, true, false);
+    } else {
+        FOHIP_FINAL(double, false, false);
+    }
+#undef FOHIP_FINAL
     return hipGetLastError();
 }
 

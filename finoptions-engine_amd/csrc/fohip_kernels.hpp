@@ -58,15 +58,18 @@ hipError_t launch_fit_reduce_moments(hipStream_t st, const DevTables &t, const F
                                      double *partials, Model *models, RowError *errors);
 // Fused method: one HBM pass per backward step (apply, Welford + Gram via pivoted power sums) over
 // (group, kBulkChunk-path chunk) items, then one wave per strike row reduces and fits.
+// `stops` non-null selects the stopping-row encoding of the cashflows (one byte per strike and path,
+// obs <= 255) instead of the value columns `values`; both give bitwise the same sums.
 hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp, uint32_t step,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, const void *paths,
-                                void *values, const Model *models, double *partials, RowError *errors);
+                                void *values, uint8_t *stops, const Model *models, double *partials,
+                                RowError *errors);
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
-                            const void *paths, const void *values, const Model *models,
+                            const void *paths, const void *values, const uint8_t *stops, const Model *models,
                             double *partials, RowError *errors, unsigned long long *ties);
 hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
                                     const uint32_t *rows, uint32_t nrows, double *partials,

@@ -21,6 +21,15 @@ constexpr int kBulkStrikes = 8;         // fused fit pass: strikes per block
 constexpr int kPartialStride = 16;      // doubles per chunk partial
 constexpr int kMaxDegree = 4;
 
+// A fused-fit chunk lies wholly inside a population of n paths (its loads need no bound). The
+// product is formed in 64 bits: chunk + 1 reaches 2^21 for n near 2^32, where 32 bits wrap.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+constexpr inline bool bulk_chunk_is_full(uint32_t chunk, uint32_t n) {
+    return (static_cast<uint64_t>(chunk) + 1u) * static_cast<uint64_t>(kBulkChunk) <= n;
+}
+
 // Rule kinds of one (strike, observation row).
 // kRuleNone is zero so a cleared model buffer starts with no rule.
 enum RuleKind : int32_t {

@@ -51,7 +51,7 @@ class FohipKernelStat(C.Structure):
 
 
 class FohipTuning(C.Structure):
-    _fields_ = [("eval_strike_tile", C.c_uint32), ("sim_waves_per_eu", C.c_uint32)]
+    _fields_ = [("eval_strike_tile", C.c_uint32), ("fit_cashflows", C.c_uint32)]
 
 
 _dp = C.POINTER(C.c_double)

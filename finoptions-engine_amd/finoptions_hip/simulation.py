@@ -88,6 +88,7 @@ class HipExecutor:
     device: int
     precision: HipPrecision = HipPrecision.F64
     _handle: _Handle = field(init=False, repr=False, compare=False)
+    _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto"))
 
     def __post_init__(self) -> None:
         if type(self.device) is not int or self.device < 0:
@@ -118,11 +119,27 @@ class HipExecutor:
         return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms))
                 for i in range(n.value)}
 
-    def tune(self, *, eval_strike_tile: int = 0) -> None:
-        """Set launch parameters; results are bitwise invariant to them."""
-        t = FohipTuning(int(eval_strike_tile), 0)
+    def tune(self, *, eval_strike_tile: int = 0, fit_cashflows: str = "auto") -> None:
+        """Set launch parameters; results are bitwise invariant to them.
+
+        ``eval_strike_tile``: strikes per evaluation launch column (0: by shape). ``fit_cashflows``
+        (fused methods): "value_columns" keeps each strike's cashflow per path in HBM,
+        "stopping_rows" keeps one byte per strike and path and rebuilds the cashflow from the
+        group's spot matrix (8x less cashflow memory; slower on the shapes measured), "auto"
+        keeps value columns.
+        """
+        codes = {"auto": 0, "value_columns": 1, "stopping_rows": 2}
+        if fit_cashflows not in codes:
+            raise InputError(f"fit_cashflows must be one of {sorted(codes)}; observed {fit_cashflows!r}")
+        t = FohipTuning(int(eval_strike_tile), codes[fit_cashflows])
         if lib().fohip_set_tuning(self._handle.ptr, C.byref(t)) != 0:
             raise InputError(f"unsupported tuning eval_strike_tile={eval_strike_tile}")
+        object.__setattr__(self, "_tuning", (int(eval_strike_tile), fit_cashflows))
+
+    @property
+    def tuning(self) -> tuple[int, str]:
+        """The launch parameters last set with :meth:`tune` (tile, cashflow encoding)."""
+        return self._tuning
 
     def set_near_tie_window(self, window: float = 0.0) -> None:
         """Relative window of the tie counts reported with every result.

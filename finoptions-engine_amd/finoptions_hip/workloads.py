@@ -84,3 +84,53 @@ def large_grouped_frame(groups: int = 4096, strikes_per_group: int = 256, *, see
         "cash_amounts": amounts.reshape(-1),
         "intervals": intervals,
     }
+
+
+def ragged_path_steps(w: dict) -> np.ndarray:
+    """Simulated path-steps of every contract of a ragged batch: intervals x (training +
+    evaluation paths), the exercise times plus expiry being the grid (the cash dates lie on
+    exercise times)."""
+    intervals = np.diff(w["exercise_offsets"]) + 1
+    return intervals.astype(np.float64) * 2.0 * w["paths"].astype(np.float64)
+
+
+def ragged_batch_request(w: dict, *, first: int = 0, count: int | None = None,
+                         spot_scale: float = 1.0, vol_shift: float = 0.0, seed: int = 2024,
+                         support=None) -> dict:
+    """Keyword arguments of ``price_lsm_call_batch`` for contracts [first, first + count) of a
+    ragged batch, with the stream domains those rows have in the whole batch (training
+    1 + row, evaluation 2 + rows + row), liquidator cash dividends strictly before maturity, the
+    explicit exercise mesh and per-contract path counts. ``spot_scale`` and ``vol_shift`` bump
+    every contract's spot (multiplicatively) and volatility (additively, relative to itself)."""
+    from . import (CashDividendBatch, DividendPolicy, ExplicitExerciseMesh,
+                   InsufficientExerciseSupportPolicy, Lsm, MaturityBoundary, OriginExercise,
+                   RaggedCashDividends, RaggedExerciseTimes, RaggedPathCounts, RandomStream)
+
+    rows = len(w["initial_spots"])
+    count = rows - first if count is None else count
+    sl = slice(first, first + count)
+    eo = w["exercise_offsets"][first:first + count + 1]
+    co = w["cash_offsets"][first:first + count + 1]
+    paths = np.ascontiguousarray(w["paths"][sl])
+    method = Lsm(training_paths=int(paths.max()), evaluation_paths=int(paths.max()),
+                 training_stream=RandomStream(seed=seed, domain=1 + first),
+                 evaluation_stream=RandomStream(seed=seed, domain=2 + rows + first),
+                 insufficient_exercise_support=support or InsufficientExerciseSupportPolicy.CONTINUE)
+    vols = np.ascontiguousarray(w["volatilities"][sl])
+    return dict(
+        initial_spots=np.ascontiguousarray(w["initial_spots"][sl] * spot_scale),
+        strikes=np.ascontiguousarray(w["strikes"][sl]),
+        maturities=np.ascontiguousarray(w["maturities"][sl]),
+        rates=np.ascontiguousarray(w["rates"][sl]),
+        volatilities=np.ascontiguousarray(vols + vol_shift * vols) if vol_shift else vols,
+        dividends=CashDividendBatch(
+            schedules=RaggedCashDividends(offsets=np.ascontiguousarray(co),
+                                          times=np.ascontiguousarray(w["cash_times"][co[0]:co[-1]]),
+                                          amounts=np.ascontiguousarray(w["cash_amounts"][co[0]:co[-1]])),
+            policy=DividendPolicy.LIQUIDATOR, maturity_boundary=MaturityBoundary.STRICTLY_BEFORE),
+        mesh=ExplicitExerciseMesh(
+            schedules=RaggedExerciseTimes(offsets=np.ascontiguousarray(eo),
+                                          times=np.ascontiguousarray(w["exercise_times"][eo[0]:eo[-1]])),
+            origin_exercise=OriginExercise.EXCLUDED),
+        method=method,
+        path_counts=RaggedPathCounts(training=paths, evaluation=paths))

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define FOHIP_ABI_VERSION 2
+#define FOHIP_ABI_VERSION 3
 
 /* ---- error codes (the numbering is shared with the CPU oracle) ---- */
 enum {
@@ -144,10 +144,16 @@ int fohip_device_count(int32_t *count);
 int fohip_abi_version(void);
 
 /* Launch parameters an autotuner may change. Results are bitwise invariant to them: reduction
- * trees are fixed by 1024-path chunks, not by these values. Zero fields keep defaults. */
+ * trees are fixed by the path chunks, not by these values. Zero fields keep defaults. */
+enum { FOHIP_CASHFLOWS_AUTO = 0, FOHIP_CASHFLOWS_VALUE_COLUMNS = 1, FOHIP_CASHFLOWS_STOPPING_ROWS = 2 };
 typedef struct {
     uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16; Greeks 1, 4, 6, 8) */
-    uint32_t sim_waves_per_eu;      /* occupancy hint for simulation launches (0 = default) */
+    /* Fused methods: how the backward induction keeps each (strike, path) cashflow. Value
+     * columns hold it (8 B, binary32 for FP32 bulk); stopping rows hold one byte, the path's
+     * stopping row, and rebuild the cashflow from the group's spot matrix (observation rows
+     * <= 255): 8x less cashflow memory and traffic, bitwise the same FUSED_F64 results, but
+     * slower on the shapes measured. Auto keeps value columns. */
+    uint32_t fit_cashflows;
 } fohip_tuning;
 int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
 

@@ -1345,6 +1345,16 @@ static int group_simulate(const group_plan_t *gp, const plan_t *p, double S0, ui
 static int price_group(const or_grouped_input *in, uint32_t g, or_batch_output *out,
                        or_error *err);
 
+/* Threads of the path-parallel simulation of a single call (the reference's CpuExecutor
+ * threads, runtime/src/lib.rs:364-424). */
+void or_set_threads(int threads) {
+#ifdef _OPENMP
+    omp_set_num_threads(threads > 0 ? threads : 1);
+#else
+    (void)threads;
+#endif
+}
+
 /* One group on the calling thread alone (simulate_group_population runs the serial simulator,
  * pricing.rs:681-701). */
 static int price_group_serial(const or_grouped_input *in, uint32_t g, or_batch_output *out,

@@ -181,6 +181,9 @@ int or_group_grid(const or_grouped_input *in, uint32_t group, double *times, uin
                   uint32_t *exercise_nodes, uint32_t *exercise_count, uint32_t *div_nodes,
                   double *div_amounts, uint32_t *div_count);
 
+/* threads of the single call's path-parallel simulation */
+void or_set_threads(int threads);
+
 /* threads <= 1: groups in order on the calling thread; otherwise group-parallel over `threads`
  * threads with every group serial (the reference's Rayon model); results do not depend on it. */
 int or_price_grouped(const or_grouped_input *in, or_batch_output *out, int threads,

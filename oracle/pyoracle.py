@@ -696,5 +696,10 @@ def escrowed_prepaid_cash(req: GroupedRequest, group: int) -> np.ndarray:
     return out[:n]
 
 
+def set_threads(threads: int) -> None:
+    """Threads of a single call's path-parallel simulation."""
+    lib().or_set_threads(int(threads))
+
+
 def threads_available() -> int:
     return len(os.sched_getaffinity(0))

@@ -162,3 +162,56 @@ def test_full_grouped_frame_4096_states_by_256_strikes(hip_fused, oracle):
     bias = np.abs(res.prices[g * per:(g + 1) * per] - ref["prices"]) / ref["standard_errors"]
     print(f"group {g} vs oracle: max |bias|/SE {bias.max():.2e}")
     assert bias.max() < 0.01
+
+
+def _fixture_columns(name: str) -> dict:
+    import json
+    from pathlib import Path
+
+    raw = json.loads((Path(__file__).parent / "golden" / name).read_text())
+    out = {}
+    for k, v in raw.items():
+        if isinstance(v, list):
+            out[k] = np.array([float.fromhex(x) for x in v]) if isinstance(v[0], str) else np.array(v)
+        else:
+            out[k] = v
+    return out
+
+
+def test_c2_full_size_matches_the_oracle_fixture(hip_f64, hip_fused):
+    """The grouped-strike state at its full size (64 strikes x 50 intervals, 2^20 + 2^20 paths)
+    against the oracle's columns committed by tests/golden/make_c2_oracle_fixture.py. The FP64
+    method must agree to 1e-10 relative on every strike whose stopping counts agree exactly, and a
+    strike whose counts differ (a decision flipped at a near tie by a last-bit difference of the
+    device's exp) is reported with its shift in standard errors; the fused method stays within a
+    hundredth of a standard error everywhere."""
+    import finoptions_hip as fh
+
+    ref = _fixture_columns("c2_full_oracle.json")
+    paths = int(ref["paths"])
+    cols_kw = dict(group_offsets=np.array([0, 64], dtype=np.uint32), initial_spots=np.array([100.0]),
+                   strikes=np.linspace(80.0, 120.0, 64), maturities=np.array([1.0]),
+                   rates=np.array([0.03]), volatilities=np.array([0.25]))
+    groups = fh.LsmStrikeGroups(**cols_kw, dividends=fh.ContinuousDividendYieldBatch(
+        annualized_decimals=np.array([0.02])))
+    mesh = fh.UniformExerciseMesh(base_intervals=50, origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = fh.Lsm(training_paths=paths, evaluation_paths=paths,
+                    training_stream=fh.RandomStream(seed=11, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=11, domain=2),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    for hip, label in ((hip_f64, "F64"), (hip_fused, "fused")):
+        res = fh.price_lsm_call_grouped_batch(groups=groups, mesh=mesh, executor=hip, method=method)
+        same = ((res.early_exercise_paths == ref["early_exercise_paths"])
+                & (res.expiry_paths == ref["expiry_paths"]))
+        rel = np.abs(res.prices - ref["prices"]) / np.abs(ref["prices"])
+        rel_se = np.abs(res.standard_errors - ref["standard_errors"]) / ref["standard_errors"]
+        bias = np.abs(res.prices - ref["prices"]) / ref["standard_errors"]
+        print(f"C2 full size {label}: strikes with equal stopping counts {int(same.sum())}/64; "
+              f"max price rel err there {rel[same].max() if same.any() else 0:.2e}; "
+              f"max |bias|/SE {bias.max():.2e}; flipped strikes {np.flatnonzero(~same).tolist()}")
+        assert bias.max() < 0.01
+        assert np.array_equal(res.regression_nodes, ref["regression_nodes"])
+        if label == "F64":
+            assert rel[same].max() < 1e-10
+            assert rel_se[same].max() < 1e-10
+            assert int((~same).sum()) <= 2
