@@ -47,14 +47,17 @@ _MIN_EVALUATION_PATHS: Final = 2
 
 
 class EscrowedVolatilityConvention(str, Enum):
-    """Meaning of the volatility vector for the escrowed prepaid-forward approximation."""
+    """Which volatility the escrowed model's volatility column holds: the prepaid forward's own
+    (PREPAID_FORWARD) or a spot volatility converted at the origin by S0 / F0
+    (SPOT_MATCHED_AT_ORIGIN)."""
 
     PREPAID_FORWARD = "prepaid_forward"
     SPOT_MATCHED_AT_ORIGIN = "spot_matched_at_origin"
 
 
 class OriginExercise(str, Enum):
-    """Whether valuation origin is an explicit common exercise decision."""
+    """Whether t = 0 is an exercise opportunity, decided once for all paths from the training
+    cashflows (INCLUDED) or not (EXCLUDED)."""
 
     EXCLUDED = "excluded"
     INCLUDED = "included"
@@ -62,7 +65,8 @@ class OriginExercise(str, Enum):
 
 @dataclass(frozen=True, slots=True)
 class UniformExerciseMesh:
-    """Uniform Bermudan base mesh augmented by every exact cash-event time."""
+    """Exercise grid T i / n for i = 0..n (n = ``base_intervals``), with every cash-dividend date
+    inserted as its own node; every interior node is an exercise node."""
 
     base_intervals: int
     origin_exercise: OriginExercise
@@ -77,12 +81,15 @@ class UniformExerciseMesh:
 
 @dataclass(frozen=True, slots=True)
 class CashDividendExerciseMesh:
-    """Exercise only at left limits of strictly pre-expiry deterministic cash events."""
+    """Exercise grid made of the cash-dividend dates alone (the state just before each payment),
+    for cash dividends strictly before maturity and nonnegative rates."""
 
 
 @dataclass(frozen=True, slots=True)
 class RaggedExerciseTimes:
-    """Borrowable row-aligned Arrow-LargeList-style exact exercise times."""
+    """Per-row exercise times in Arrow list layout: int64 ``offsets`` (rows + 1 entries, possibly
+    starting above 0) into one float64 ``times`` column; each row positive and strictly
+    increasing. The arrays are borrowed, not copied."""
 
     offsets: Int64Vector
     times: Float64Vector
@@ -113,7 +120,8 @@ class RaggedExerciseTimes:
 
 @dataclass(frozen=True, slots=True)
 class ExplicitExerciseMesh:
-    """Exact row-aligned pre-expiry Bermudan opportunities."""
+    """Exercise grid given explicitly per row (each time strictly before that row's maturity),
+    united with its cash-dividend dates."""
 
     schedules: RaggedExerciseTimes
     origin_exercise: OriginExercise
@@ -127,7 +135,7 @@ class ExplicitExerciseMesh:
 
 @dataclass(frozen=True, slots=True)
 class ContinuousDividendYieldBatch:
-    """One constant nonnegative annualized proportional yield per option row."""
+    """Continuous proportional dividend yield q >= 0 per group: Q(t) = q t on its grid."""
 
     annualized_decimals: Float64Vector
 
@@ -137,7 +145,8 @@ class ContinuousDividendYieldBatch:
 
 @dataclass(frozen=True, slots=True)
 class CashDividendBatch:
-    """Exact cash schedules with explicit jump and terminal-boundary conventions."""
+    """Cash dividends per group (Arrow layout), the policy for a dividend larger than the spot,
+    and whether a payment exactly at maturity is admitted."""
 
     schedules: RaggedCashDividends
     policy: DividendPolicy
@@ -154,7 +163,8 @@ class CashDividendBatch:
 
 @dataclass(frozen=True, slots=True)
 class EscrowedCashDividendBatch:
-    """Risky prepaid forward plus the deterministic value of later cash dividends.
+    """Escrowed cash dividends: the stock is a GBM prepaid forward F plus the discounted value of
+    the dividends still to be paid.
 
     An escrowed-dividend approximation, distinct from the exact spot cash-jump model of
     :class:`CashDividendBatch`: the simulated state is the prepaid forward F (no dividends), started
@@ -208,7 +218,9 @@ ExerciseMesh = Union[UniformExerciseMesh, CashDividendExerciseMesh, ExplicitExer
 
 @dataclass(frozen=True, slots=True)
 class LsmStrikeGroups:
-    """Compact exact simulation states with one or more row-ordered strikes per state."""
+    """Market states (spot, maturity, rate, volatility, dividends) with their strike rows:
+    uint32 ``group_offsets`` (groups + 1, from 0 to the strike count) partition ``strikes`` into
+    the rows of each state, which share one simulated path population."""
 
     group_offsets: UInt32Vector
     initial_spots: Float64Vector
@@ -440,12 +452,11 @@ def price_lsm_call_grouped_batch(
     greeks: tuple[Greek, ...] = (),
     path_counts: RaggedPathCounts | None = None,
 ) -> LsmBatchResult:
-    """Price strike groups while simulating each declared market state exactly once.
+    """Price every strike row of every group; each group's state is simulated once and shared.
 
-    Every strike fits its own stopping policy on the shared group training population and is valued
-    out of sample on the shared, disjoint evaluation population (common random numbers across
-    strikes). Group ``g`` uses domains ``training_stream.domain + g`` and
-    ``evaluation_stream.domain + g``.
+    A strike row has its own stopping policy, fitted on its group's training paths and valued on
+    the group's evaluation paths, so the strikes of a group see common random numbers. Group g
+    draws from domains ``training_stream.domain + g`` and ``evaluation_stream.domain + g``.
 
     Raises:
         InputError: If a runtime type, array layout, compact boundary, financial value, mesh, or

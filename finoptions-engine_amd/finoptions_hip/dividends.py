@@ -35,7 +35,8 @@ def validate_vector(value: object, *, field: str, dtype: str) -> None:
 
 
 class MaturityBoundary(str, Enum):
-    """Whether an event exactly at option maturity belongs to a model input."""
+    """Whether a cash dividend dated exactly at maturity is admitted (AT_OR_BEFORE; paid before the
+    expiry payoff) or rejected (STRICTLY_BEFORE)."""
 
     STRICTLY_BEFORE = "strictly_before"
     AT_OR_BEFORE = "at_or_before"
@@ -43,7 +44,9 @@ class MaturityBoundary(str, Enum):
 
 @dataclass(frozen=True, slots=True)
 class RaggedCashDividends:
-    """Borrowable Arrow-LargeList-style offsets, times, and exact cash amounts."""
+    """Per-row cash dividends in Arrow list layout: int64 ``offsets`` (rows + 1 entries, possibly
+    starting above 0) into float64 ``times`` (positive, nondecreasing within a row) and
+    ``amounts`` (finite, >= 0). The arrays are borrowed, not copied."""
 
     offsets: Int64Vector
     times: Float64Vector

@@ -1,8 +1,9 @@
-"""Independent-policy least-squares Monte Carlo for explicitly scheduled calls, on HIP devices.
+"""Longstaff-Schwartz pricing of one scheduled Bermudan call on a HIP device.
 
-Mirrors python/finoptions/lsm.py of the reference: ``Lsm``, the diagnostic classes, ``LsmResult``
-and ``price_lsm_call`` keep their names, fields, defaults and checks. The executor is an explicit
-``HipExecutor``; ``greeks`` additionally requests pathwise sensitivities under the frozen policy.
+The public names, fields, defaults and error messages are those of the reference's
+python/finoptions/lsm.py (``Lsm``, the per-node diagnostics, ``LsmResult``, ``price_lsm_call``), so
+requests port unchanged; the executor is an explicit ``HipExecutor`` and ``greeks`` requests
+pathwise sensitivities under the frozen policy. Request checks are the rule tables below.
 """
 
 from __future__ import annotations
@@ -33,6 +34,7 @@ from .model import (
     EventSchedule,
     LogDiscountCurve,
 )
+from . import _rules as R
 from .simulation import HipExecutor, RandomStream
 
 _MAX_PATHS: Final = (2**32) - 1
@@ -40,14 +42,16 @@ _MAX_POLYNOMIAL_DEGREE: Final = 4
 
 
 class ExerciseTieRule(str, Enum):
-    """Explicit decision when intrinsic and fitted continuation compare equal."""
+    """How an exercise decision treats intrinsic == continuation: STRICTLY_GREATER holds the
+    option, GREATER_OR_EQUAL exercises it."""
 
     STRICTLY_GREATER = "strictly_greater"
     GREATER_OR_EQUAL = "greater_or_equal"
 
 
 class InsufficientExerciseSupportPolicy(str, Enum):
-    """Declared stopping rule when in-the-money rows cannot identify the polynomial basis."""
+    """What a node with fewer in-the-money training paths than basis columns does: FAIL raises,
+    CONTINUE records the node and never exercises there."""
 
     FAIL = "fail"
     CONTINUE = "continue"
@@ -75,22 +79,45 @@ def greek_mask(greeks: tuple[Greek, ...]) -> int:
     return mask
 
 
-def _validate_random_streams(training: RandomStream, evaluation: RandomStream) -> None:
-    if type(training) is not RandomStream:
-        raise InputError("training_stream must be RandomStream")
-    if type(evaluation) is not RandomStream:
-        raise InputError("evaluation_stream must be RandomStream")
-    if training.domain == evaluation.domain:
-        msg = (
-            "training_stream and evaluation_stream must use different domains; "
-            f"both use {training.domain}"
-        )
-        raise InputError(msg)
+def _method_rows(m: "Lsm") -> list:
+    """The checks of an Lsm method in reporting order: path counts, the two streams and their
+    distinct domains, the support policy, degree, rank tolerance, condition limit (bounded by
+    1 / rank tolerance) and tie rule."""
+    path_count = R.int_in(1, _MAX_PATHS)
+    streams = (m.training_stream, m.evaluation_stream)
+    rows = [("training_paths", m.training_paths, path_count),
+            ("evaluation_paths", m.evaluation_paths, path_count),
+            ("training_stream", m.training_stream,
+             (R.exact_type(RandomStream, "{field} must be RandomStream"),)),
+            ("evaluation_stream", m.evaluation_stream,
+             (R.exact_type(RandomStream, "{field} must be RandomStream"),)),
+            ("streams", streams,
+             (R.Lazy(lambda st: st[0].domain != st[1].domain,
+                     lambda st: "training_stream and evaluation_stream must use different "
+                                f"domains; both use {st[0].domain}"),)),
+            ("insufficient_exercise_support", m.insufficient_exercise_support,
+             R.instance_of(InsufficientExerciseSupportPolicy)),
+            ("polynomial_degree", m.polynomial_degree, R.int_in(0, _MAX_POLYNOMIAL_DEGREE)),
+            ("rank_tolerance", m.rank_tolerance,
+             (R.Check(lambda v: type(v) is float and isfinite(v) and 0.0 < v < 1.0,
+                      "{field} must be a finite plain float strictly between 0 and 1; "
+                      "observed {value!r}"),))]
+    # rows are checked in order, so the condition limit's bound 1 / rank_tolerance is formed only
+    # once the tolerance has passed its own row
+    rows += [("condition_limit", m.condition_limit,
+              (R.Lazy(lambda v: type(v) is float and isfinite(v) and 1.0 <= v <= 1.0 / m.rank_tolerance,
+                      lambda v: "condition_limit must be a finite plain float in "
+                                f"[1, {1.0 / m.rank_tolerance}]; observed {v!r}"),)),
+             ("tie_rule", m.tie_rule, R.instance_of(ExerciseTieRule))]
+    return rows
 
 
 @dataclass(frozen=True, slots=True)
 class Lsm:
-    """Complete simulation/regression identity for independent-policy LSM (reference defaults)."""
+    """The estimator: path counts and streams of the training and evaluation populations, the
+    support policy, the polynomial basis degree with its rank tolerance and condition limit, and
+    the tie rule. The two streams must use different domains so that the evaluation paths are
+    independent of the paths the policy was fitted on."""
 
     training_paths: int
     evaluation_paths: int
@@ -103,48 +130,13 @@ class Lsm:
     tie_rule: ExerciseTieRule = ExerciseTieRule.STRICTLY_GREATER
 
     def __post_init__(self) -> None:
-        for name, value in (("training_paths", self.training_paths),
-                            ("evaluation_paths", self.evaluation_paths)):
-            if type(value) is not int or not 1 <= value <= _MAX_PATHS:
-                msg = f"{name} must be a plain int in [1, {_MAX_PATHS}]; observed {value!r}"
-                raise InputError(msg)
-        _validate_random_streams(self.training_stream, self.evaluation_stream)
-        if type(self.insufficient_exercise_support) is not InsufficientExerciseSupportPolicy:
-            msg = (
-                "insufficient_exercise_support must be InsufficientExerciseSupportPolicy; "
-                f"observed {type(self.insufficient_exercise_support).__name__}"
-            )
-            raise InputError(msg)
-        if (type(self.polynomial_degree) is not int
-                or not 0 <= self.polynomial_degree <= _MAX_POLYNOMIAL_DEGREE):
-            msg = (
-                "polynomial_degree must be a plain int in "
-                f"[0, {_MAX_POLYNOMIAL_DEGREE}]; observed {self.polynomial_degree!r}"
-            )
-            raise InputError(msg)
-        if (type(self.rank_tolerance) is not float or not isfinite(self.rank_tolerance)
-                or not 0.0 < self.rank_tolerance < 1.0):
-            msg = (
-                "rank_tolerance must be a finite plain float strictly between 0 and 1; "
-                f"observed {self.rank_tolerance!r}"
-            )
-            raise InputError(msg)
-        maximum_condition = 1.0 / self.rank_tolerance
-        if (type(self.condition_limit) is not float or not isfinite(self.condition_limit)
-                or not 1.0 <= self.condition_limit <= maximum_condition):
-            msg = (
-                "condition_limit must be a finite plain float in "
-                f"[1, {maximum_condition}]; observed {self.condition_limit!r}"
-            )
-            raise InputError(msg)
-        if type(self.tie_rule) is not ExerciseTieRule:
-            msg = f"tie_rule must be ExerciseTieRule; observed {type(self.tie_rule).__name__}"
-            raise InputError(msg)
+        R.enforce(_method_rows(self))
 
 
 @dataclass(frozen=True, slots=True)
 class OriginDecisionDiagnostics:
-    """Common valuation-origin policy decision fitted from training continuation."""
+    """The valuation origin's decision: exercise when the intrinsic value beats the mean
+    discounted training cashflow ``estimated_continuation``."""
 
     node: int
     exercise: bool
@@ -153,14 +145,15 @@ class OriginDecisionDiagnostics:
 
 @dataclass(frozen=True, slots=True)
 class NoInTheMoneyPathsDiagnostics:
-    """Training had no in-the-money state at this exercise node."""
+    """An exercise node at which no training path was in the money (no rule was fitted)."""
 
     node: int
 
 
 @dataclass(frozen=True, slots=True)
 class InsufficientExerciseSupportDiagnostics:
-    """Declared continuation at a node whose training population cannot identify the basis."""
+    """An exercise node whose in-the-money training paths were fewer than the basis columns, under
+    the CONTINUE policy: no exercise there."""
 
     node: int
     in_the_money_paths: int
@@ -169,7 +162,8 @@ class InsufficientExerciseSupportDiagnostics:
 
 @dataclass(frozen=True, slots=True)
 class RegressionFitDiagnostics:
-    """Accepted continuation regression population and numerical diagnostics."""
+    """A fitted continuation regression: its in-the-money population, design shape and numerical
+    rank, condition number, and the moneyness standardization (mean, scale)."""
 
     node: int
     in_the_money_paths: int
@@ -191,7 +185,10 @@ ExerciseFitDiagnostics = Union[
 
 @dataclass(frozen=True, slots=True)
 class LsmResult:
-    """Independent mean payoff, sampling error, stopping counts, and fit diagnostics."""
+    """One pricing: the mean discounted payoff over the evaluation paths with its sample standard
+    deviation and standard error (None for a single path), the observation nodes with the
+    number of paths stopping at each, one diagnostic per exercise node, and the requested
+    Greeks as (greek, value, standard error)."""
 
     price: float
     paths: int
@@ -239,19 +236,18 @@ def _diagnostic(d: FohipNodeDiagnostic) -> ExerciseFitDiagnostics:
     raise LsmError(f"native LSM result returned unsupported diagnostic kind {d.kind!r}")
 
 
-def _validate_specs(*, initial_spot, strike, schedule, curve, dividend_yield_curve, volatility,
-                    dividend_policy, executor, method) -> None:
-    for name, value in (("initial_spot", initial_spot), ("strike", strike)):
-        if type(value) is not float:
-            raise InputError(f"{name} must be a plain float; observed {type(value).__name__}")
-    checks = (("schedule", schedule, EventSchedule), ("curve", curve, LogDiscountCurve),
-              ("dividend_yield_curve", dividend_yield_curve, CumulativeDividendYieldCurve),
-              ("volatility", volatility, ConstantVolatility),
-              ("dividend_policy", dividend_policy, DividendPolicy),
-              ("executor", executor, HipExecutor), ("method", method, Lsm))
-    for name, value, cls in checks:
-        if type(value) is not cls:
-            raise InputError(f"{name} must be {cls.__name__}; observed {type(value).__name__}")
+def _call_rows(initial_spot, strike, schedule, curve, dividend_yield_curve, volatility,
+               dividend_policy, executor, method) -> list:
+    """Exact runtime types of a single-call request, in the reference's reporting order."""
+    return [("initial_spot", initial_spot, R.PLAIN_FLOAT), ("strike", strike, R.PLAIN_FLOAT),
+            ("schedule", schedule, R.instance_of(EventSchedule)),
+            ("curve", curve, R.instance_of(LogDiscountCurve)),
+            ("dividend_yield_curve", dividend_yield_curve,
+             R.instance_of(CumulativeDividendYieldCurve)),
+            ("volatility", volatility, R.instance_of(ConstantVolatility)),
+            ("dividend_policy", dividend_policy, R.instance_of(DividendPolicy)),
+            ("executor", executor, R.instance_of(HipExecutor)),
+            ("method", method, R.instance_of(Lsm))]
 
 
 def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typed
@@ -267,21 +263,22 @@ def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typ
     method: Lsm,
     greeks: tuple[Greek, ...] = (),
 ) -> LsmResult:
-    """Fit a call stopping policy and evaluate it only on its independent path population.
+    """Price a Bermudan call on the HIP device: fit the stopping policy by backward induction on
+    the training paths, then value it on the independent evaluation paths.
 
-    Exercise states are observed before any same-node cash dividend. Every estimator-defining
-    numerical value is carried by ``method``; failures never reduce degree, add regularization,
-    reuse training paths, or change the requested device or precision.
+    The request is taken as given: a failed fit raises instead of lowering the degree or
+    regularizing, the evaluation never reuses training paths, and neither the device nor the
+    executor's method is changed. ``greeks`` adds pathwise sensitivities of the price under the
+    fitted (frozen) policy.
 
     Raises:
-        InputError: If an explicit Python or financial specification is invalid.
-        LsmError: If simulation, regression, support, or fixed-policy evaluation fails.
-        ExecutionError: If the device resource fails.
+        InputError: If a runtime type or a financial input is invalid.
+        LsmError: If simulation, a regression, the exercise support or the evaluation fails.
+        ExecutionError: If the device fails.
 
     """
-    _validate_specs(initial_spot=initial_spot, strike=strike, schedule=schedule, curve=curve,
-                    dividend_yield_curve=dividend_yield_curve, volatility=volatility,
-                    dividend_policy=dividend_policy, executor=executor, method=method)
+    R.enforce(_call_rows(initial_spot, strike, schedule, curve, dividend_yield_curve, volatility,
+                         dividend_policy, executor, method))
     mask = greek_mask(greeks)
     f64 = lambda v: np.ascontiguousarray(v if len(v) else [0.0], dtype=np.float64)  # noqa: E731
     u32 = lambda v: np.ascontiguousarray(v if len(v) else [0], dtype=np.uint32)  # noqa: E731

@@ -172,7 +172,8 @@ class CpuExecutor:
 
 @dataclass(frozen=True, slots=True)
 class RandomStream:
-    """Explicit reproducible Philox namespace for one statistically distinct path set."""
+    """A Philox key (``seed``) and counter lane (``domain``) naming one path population: path p
+    of the population is the counter stream (block, p, domain) under that key."""
 
     seed: int
     domain: int
@@ -188,7 +189,8 @@ class RandomStream:
 
 @dataclass(frozen=True, slots=True)
 class GbmPaths:
-    """Owned time-major pre-event states at exercise nodes and expiry."""
+    """Simulated states, one row per observation node (exercise nodes, then expiry) and one column
+    per path; an exercise row holds the state before that node's cash dividend."""
 
     observation_nodes: tuple[int, ...]
     states: Float64Matrix
