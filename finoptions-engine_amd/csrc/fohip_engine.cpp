@@ -747,10 +747,62 @@ const char *regression_message(int code, char *buf, size_t cap, int degree, cons
 
 }  // namespace
 
+namespace {
+
+// Device buffers of one batch. Grown mode (the default) claims the executor's own buffers, growing
+// them with hipMalloc as needed, and stages transfers through its page-locked buffer. Fixed mode
+// carves every buffer out of one caller-provided device region (the allocation-free entry point):
+// claims only advance an offset, `overflow` records a request larger than the region, and
+// transfers go through pageable host memory, so no device or pinned allocation happens.
+struct Workspace {
+    fohip_executor *ex = nullptr;
+    char *region = nullptr;
+    size_t capacity = 0, used = 0, needed = 0;
+    bool overflow = false;
+    std::vector<char> pageable;
+    explicit Workspace(fohip_executor *e) : ex(e) {}
+    Workspace(fohip_executor *e, void *base, size_t bytes) : ex(e), region(static_cast<char *>(base)), capacity(bytes) {}
+    void begin_batch() {
+        used = needed = 0;
+        overflow = false;
+    }
+    hipError_t claim(DevBuf &own, size_t bytes, void **ptr) {
+        if (!region) {
+            hipError_t e = own.ensure(bytes);
+            *ptr = own.ptr;
+            return e;
+        }
+        const size_t off = (used + 255) & ~size_t(255);
+        needed = off + bytes;
+        if (needed > capacity) {
+            overflow = true;
+            *ptr = nullptr;
+            return hipSuccess;
+        }
+        *ptr = region + off;
+        used = needed;
+        return hipSuccess;
+    }
+    hipError_t host_staging(size_t bytes, char **ptr) {
+        if (!region) {
+            hipError_t e = ex->staging.ensure(bytes);
+            *ptr = ex->staging.ptr;
+            return e;
+        }
+        if (pageable.size() < bytes) pageable.resize(bytes);
+        *ptr = pageable.data();
+        return hipSuccess;
+    }
+};
+
+}  // namespace
+
 // Runs the full pipeline for a list of groups (processed in memory-bounded batches).
 static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, const std::vector<double> &strikes,
                       const RunSpec &spec, std::vector<RowOut> &out, std::vector<Model> *models_out,
-                      std::vector<uint32_t> *counts_out, fohip_error *err) {
+                      std::vector<uint32_t> *counts_out, fohip_error *err, Workspace *fixed = nullptr) {
+    Workspace grown(ex);
+    Workspace &ws = fixed ? *fixed : grown;
     HIP_TRY(hipSetDevice(ex->device));
     hipStream_t st = ex->stream;
     const bool f32 = spec.store32;
@@ -898,24 +950,39 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             offs.push_back(tables_bytes);
             tables_bytes += (b + 255) & ~size_t(255);
         }
-        HIP_TRY(ex->tables.ensure(tables_bytes));
-        HIP_TRY(ex->paths.ensure(std::max<uint64_t>(path_elems, 1) * real));
+        // the evaluation normals' region is needed before the launches too, so that every buffer of
+        // the batch is claimed (and a fixed workspace found large enough) before any work is queued
+        const bool stored_normals = spec.fused && tile > 1 && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0;
+        ws.begin_batch();
+        void *paths_d = nullptr, *values_d = nullptr, *stops_d = nullptr, *models_d = nullptr, *part_t_d = nullptr,
+             *part_e_d = nullptr, *counts_d = nullptr, *nosup_d = nullptr, *errors_d = nullptr, *fail_t_d = nullptr,
+             *fail_e_d = nullptr, *eval_out_d = nullptr, *summary_d = nullptr, *ties_d = nullptr, *tables_d = nullptr,
+             *znorm_d = nullptr, *replay_d = nullptr;
+        HIP_TRY(ws.claim(ex->tables, tables_bytes, &tables_d));
+        HIP_TRY(ws.claim(ex->paths, std::max<uint64_t>(path_elems, 1) * real, &paths_d));
         if (use_stops)
-            HIP_TRY(ex->stops.ensure((std::max<uint64_t>(value_elems, 1) + 3) & ~uint64_t(3)));
+            HIP_TRY(ws.claim(ex->stops, (std::max<uint64_t>(value_elems, 1) + 3) & ~uint64_t(3), &stops_d));
         else
-            HIP_TRY(ex->values.ensure(std::max<uint64_t>(value_elems, 1) * real));
-        HIP_TRY(ex->models.ensure(std::max<uint64_t>(model_count, 1) * sizeof(Model)));
-        HIP_TRY(ex->part_t.ensure(std::max<uint64_t>(part_t, 1) * kPartialStride * sizeof(double)));
-        HIP_TRY(ex->part_e.ensure(std::max<uint64_t>(part_e, 1) * kPartialStride * sizeof(double)));
-        HIP_TRY(ex->counts.ensure(std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t)));
-        HIP_TRY(ex->nosup.ensure(std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t)));
-        HIP_TRY(ex->errors.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowError)));
-        HIP_TRY(ex->fail_t.ensure(ngroups * sizeof(unsigned long long)));
-        HIP_TRY(ex->fail_e.ensure(ngroups * sizeof(unsigned long long)));
-        HIP_TRY(ex->eval_out.ensure(std::max<uint32_t>(nrows, 1) * 16 * sizeof(double)));
-        HIP_TRY(ex->summary.ensure(std::max<uint32_t>(nrows, 1) * sizeof(RowSummary)));
-        HIP_TRY(ex->ties.ensure(std::max<uint32_t>(nrows, 1) * sizeof(unsigned long long)));
-        char *tb = ex->tables.as<char>();
+            HIP_TRY(ws.claim(ex->values, std::max<uint64_t>(value_elems, 1) * real, &values_d));
+        HIP_TRY(ws.claim(ex->models, std::max<uint64_t>(model_count, 1) * sizeof(Model), &models_d));
+        HIP_TRY(ws.claim(ex->part_t, std::max<uint64_t>(part_t, 1) * kPartialStride * sizeof(double), &part_t_d));
+        HIP_TRY(ws.claim(ex->part_e, std::max<uint64_t>(part_e, 1) * kPartialStride * sizeof(double), &part_e_d));
+        HIP_TRY(ws.claim(ex->counts, std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t), &counts_d));
+        HIP_TRY(ws.claim(ex->nosup, std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t), &nosup_d));
+        HIP_TRY(ws.claim(ex->errors, std::max<uint32_t>(nrows, 1) * sizeof(RowError), &errors_d));
+        HIP_TRY(ws.claim(ex->fail_t, ngroups * sizeof(unsigned long long), &fail_t_d));
+        HIP_TRY(ws.claim(ex->fail_e, ngroups * sizeof(unsigned long long), &fail_e_d));
+        HIP_TRY(ws.claim(ex->eval_out, std::max<uint32_t>(nrows, 1) * 16 * sizeof(double), &eval_out_d));
+        HIP_TRY(ws.claim(ex->summary, std::max<uint32_t>(nrows, 1) * sizeof(RowSummary), &summary_d));
+        HIP_TRY(ws.claim(ex->ties, std::max<uint32_t>(nrows, 1) * sizeof(unsigned long long), &ties_d));
+        HIP_TRY(ws.claim(ex->replay, 4 * sizeof(double), &replay_d));
+        if (stored_normals) HIP_TRY(ws.claim(ex->znorm, znorm_elems * sizeof(double), &znorm_d));
+        if (ws.overflow) {
+            return fail(err, FOHIP_E_UNSUPPORTED, 0,
+                        "request needs %llu bytes of device scratch; the allocation-free entry point holds %llu",
+                        (unsigned long long)ws.needed, (unsigned long long)ws.capacity);
+        }
+        char *tb = static_cast<char *>(tables_d);
         const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
                                     gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
                                     gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
@@ -938,21 +1005,22 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // the fused pass's are in the row summary and the evaluation's in eval_out
         std::vector<unsigned long long> ties_h(nrows);
         const Download downloads[] = {
-            {fail_t.data(), ex->fail_t.ptr, ngroups * sizeof(unsigned long long)},
-            {fail_e.data(), ex->fail_e.ptr, ngroups * sizeof(unsigned long long)},
-            {row_err.data(), ex->errors.ptr, nrows * sizeof(RowError)},
-            {eval_out.data(), ex->eval_out.ptr, eval_out.size() * sizeof(double)},
-            {summary.data(), ex->summary.ptr, nrows * sizeof(RowSummary)},
-            {models_h.data(), ex->models.ptr, models_h.size() * sizeof(Model)},
-            {counts_h.data(), ex->counts.ptr, counts_h.size() * sizeof(uint32_t)},
-            {ties_h.data(), ex->ties.ptr, ties_h.size() * sizeof(unsigned long long)}};
+            {fail_t.data(), fail_t_d, ngroups * sizeof(unsigned long long)},
+            {fail_e.data(), fail_e_d, ngroups * sizeof(unsigned long long)},
+            {row_err.data(), errors_d, nrows * sizeof(RowError)},
+            {eval_out.data(), eval_out_d, eval_out.size() * sizeof(double)},
+            {summary.data(), summary_d, nrows * sizeof(RowSummary)},
+            {models_h.data(), models_d, models_h.size() * sizeof(Model)},
+            {counts_h.data(), counts_d, counts_h.size() * sizeof(uint32_t)},
+            {ties_h.data(), ties_d, ties_h.size() * sizeof(unsigned long long)}};
         size_t staged = tables_bytes;
         for (const Download &d : downloads) staged += (d.bytes + 255) & ~size_t(255);
         // one upload of every table, item list and row list of the batch
-        HIP_TRY(ex->staging.ensure(staged));
+        char *staging = nullptr;
+        HIP_TRY(ws.host_staging(staged, &staging));
         for (size_t i = 0; i < offs.size(); ++i)
-            if (tables_bytes_list[i]) std::memcpy(ex->staging.ptr + offs[i], host_parts[i], tables_bytes_list[i]);
-        HIP_TRY(hipMemcpyAsync(tb, ex->staging.ptr, tables_bytes, hipMemcpyHostToDevice, st));
+            if (tables_bytes_list[i]) std::memcpy(staging + offs[i], host_parts[i], tables_bytes_list[i]);
+        HIP_TRY(hipMemcpyAsync(tb, staging, tables_bytes, hipMemcpyHostToDevice, st));
         DevTables t;
         t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
         t.log_drift = reinterpret_cast<const double *>(tb + offs[1]);
@@ -982,51 +1050,51 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         auto add_fill = [&fills](void *ptr, size_t bytes, uint32_t value) {
             fills.region[fills.count++] = FillRegion{static_cast<uint32_t *>(ptr), bytes / 4, value, 0};
         };
-        add_fill(ex->models.ptr, model_count * sizeof(Model), 0u);
-        add_fill(ex->counts.ptr, count_slots * sizeof(uint32_t), 0u);
-        add_fill(ex->nosup.ptr, count_slots * sizeof(uint32_t), 0u);
-        add_fill(ex->errors.ptr, nrows * sizeof(RowError), 0xFFFFFFFFu);
-        add_fill(ex->fail_t.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
-        add_fill(ex->fail_e.ptr, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
-        add_fill(ex->ties.ptr, ties_h.size() * sizeof(unsigned long long), 0u);
+        add_fill(models_d, model_count * sizeof(Model), 0u);
+        add_fill(counts_d, count_slots * sizeof(uint32_t), 0u);
+        add_fill(nosup_d, count_slots * sizeof(uint32_t), 0u);
+        add_fill(errors_d, nrows * sizeof(RowError), 0xFFFFFFFFu);
+        add_fill(fail_t_d, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(fail_e_d, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(ties_d, ties_h.size() * sizeof(unsigned long long), 0u);
         // every path starts stopped at expiry: stopping byte 0
-        if (use_stops) add_fill(ex->stops.ptr, (value_elems + 3) & ~uint64_t(3), 0u);
-        uint8_t *stops = use_stops ? ex->stops.as<uint8_t>() : nullptr;
+        if (use_stops) add_fill(stops_d, (value_elems + 3) & ~uint64_t(3), 0u);
+        uint8_t *stops = use_stops ? static_cast<uint8_t *>(stops_d) : nullptr;
         HIP_TRY(launch_fill(st, fills));
-        unsigned long long *ties_t = ex->ties.as<unsigned long long>();
-        Model *models = ex->models.as<Model>();
-        RowError *errors = ex->errors.as<RowError>();
+        unsigned long long *ties_t = static_cast<unsigned long long *>(ties_d);
+        Model *models = static_cast<Model *>(models_d);
+        RowError *errors = static_cast<RowError *>(errors_d);
         FitParams fp = spec.fp;
         fp.tie_window = ex->near_tie_window;
         // ---- training simulation
-        PROF_LAUNCH("k_simulate", launch_simulate(st, f32, t, it, static_cast<uint32_t>(items_t.size()), ex->paths.ptr, 0,
-                                ex->fail_t.as<unsigned long long>()));
+        PROF_LAUNCH("k_simulate", launch_simulate(st, f32, t, it, static_cast<uint32_t>(items_t.size()), paths_d, 0,
+                                static_cast<unsigned long long *>(fail_t_d)));
         // ---- backward induction
         for (uint32_t j = 0; j < max_fit; ++j) {
             const uint32_t ni = step_items[j], nr = step_rows[j];
             if (spec.fused) {   // one fused HBM pass per backward step
-                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, ex->paths.ptr,
-                                                                    ex->values.ptr, stops, models, ex->part_t.as<double>(),
+                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, paths_d,
+                                                                    values_d, stops, models, static_cast<double *>(part_t_d),
                                                                     errors));
-                PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, ex->part_t.as<double>(),
+                PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, static_cast<double *>(part_t_d),
                                                                         models, errors));
                 continue;
             }
-            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, t, fp, j, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr, models,
-                                       ex->part_t.as<double>(), errors, ties_t));
-            PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, ex->part_t.as<double>(), models, errors));
+            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, t, fp, j, it, ni, max_strikes, paths_d, values_d, models,
+                                       static_cast<double *>(part_t_d), errors, ties_t));
+            PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, static_cast<double *>(part_t_d), models, errors));
             for (int k = 0; k <= fp.degree; ++k) {
-                PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, t, fp, j, k, it, ni, max_strikes, ex->paths.ptr, ex->values.ptr,
-                                           models, ex->part_t.as<double>(), errors));
-                PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, ex->part_t.as<double>(), models, errors));
+                PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, t, fp, j, k, it, ni, max_strikes, paths_d, values_d,
+                                           models, static_cast<double *>(part_t_d), errors));
+                PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, static_cast<double *>(part_t_d), models, errors));
             }
         }
         if (spec.fused)   // rank/condition gates of every rule of the backward pass at once
             PROF_LAUNCH("k_fit_gate", launch_fit_gate(st, t, fp, rows, nrows, models, static_cast<uint32_t>(model_count),
                                                       errors));
-        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, ex->paths.ptr,
-                                 ex->values.ptr, stops, models, ex->part_t.as<double>(), errors, ties_t));
-        PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, ex->part_t.as<double>(), models, errors));
+        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, paths_d,
+                                 values_d, stops, models, static_cast<double *>(part_t_d), errors, ties_t));
+        PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, static_cast<double *>(part_t_d), models, errors));
         // ---- evaluation
         bool escrow = false;   // the escrowed instantiation also serves spot-model groups
         for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
@@ -1034,33 +1102,32 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams). Single-
         // strike tiles always regenerate: that form has no stored-normal kernel.
         const double *znorm = nullptr;
-        if (spec.fused && tile > 1 && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0) {
-            HIP_TRY(ex->znorm.ensure(znorm_elems * sizeof(double)));
+        if (stored_normals) {
             PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ite,
                                                                static_cast<uint32_t>(items_e.size()),
-                                                               ex->znorm.as<double>(),
-                                                               ex->fail_e.as<unsigned long long>()));
-            znorm = ex->znorm.as<double>();
+                                                               static_cast<double *>(znorm_d),
+                                                               static_cast<unsigned long long *>(fail_e_d)));
+            znorm = static_cast<const double *>(znorm_d);
         }
         PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ite,
                                 static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
-                                ex->part_e.as<double>(), ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), errors,
-                                ex->fail_e.as<unsigned long long>(), znorm));
-        PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, ex->part_e.as<double>(), greeks ? 1 : 0, ex->eval_out.as<double>()));
-        PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, ex->counts.as<uint32_t>(), ex->nosup.as<uint32_t>(), nrows,
-                                   ex->summary.as<RowSummary>()));
+                                static_cast<double *>(part_e_d), static_cast<uint32_t *>(counts_d), static_cast<uint32_t *>(nosup_d), errors,
+                                static_cast<unsigned long long *>(fail_e_d), znorm));
+        PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, static_cast<double *>(part_e_d), greeks ? 1 : 0, static_cast<double *>(eval_out_d)));
+        PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, static_cast<uint32_t *>(counts_d), static_cast<uint32_t *>(nosup_d), nrows,
+                                   static_cast<RowSummary *>(summary_d)));
         // ---- results: DMA transfers into the staging area, copied out once the stream drains
         {
             size_t off = tables_bytes;
             for (const Download &d : downloads) {
-                if (d.bytes) HIP_TRY(hipMemcpyAsync(ex->staging.ptr + off, d.dev, d.bytes, hipMemcpyDeviceToHost, st));
+                if (d.bytes) HIP_TRY(hipMemcpyAsync(staging + off, d.dev, d.bytes, hipMemcpyDeviceToHost, st));
                 off += (d.bytes + 255) & ~size_t(255);
             }
             HIP_TRY(hipStreamSynchronize(st));
             HIP_TRY(hipGetLastError());
             off = tables_bytes;
             for (const Download &d : downloads) {
-                if (d.bytes) std::memcpy(d.host, ex->staging.ptr + off, d.bytes);
+                if (d.bytes) std::memcpy(d.host, staging + off, d.bytes);
                 off += (d.bytes + 255) & ~size_t(255);
             }
         }
@@ -1074,9 +1141,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             const int64_t first_row = static_cast<int64_t>(row0 + P.strike_begin);
             auto sim_failure = [&](int evaluation, unsigned long long path) -> int {
                 std::vector<double> info(4, 0.0);
-                (void)ex->replay.ensure(4 * sizeof(double));
-                HIP_TRY(launch_replay_path(st, t, g, evaluation, static_cast<uint32_t>(path), ex->replay.as<double>()));
-                HIP_TRY(hipMemcpyAsync(info.data(), ex->replay.ptr, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIP_TRY(launch_replay_path(st, t, g, evaluation, static_cast<uint32_t>(path), static_cast<double *>(replay_d)));
+                HIP_TRY(hipMemcpyAsync(info.data(), replay_d, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
                 HIP_TRY(hipStreamSynchronize(st));
                 const int phase = evaluation ? FOHIP_PHASE_EVALUATION_SIMULATION : FOHIP_PHASE_TRAINING_SIMULATION;
                 const char *role = evaluation ? "evaluation" : "training";
@@ -1241,11 +1307,10 @@ static bool same_grid(const double *a, size_t na, const double *b, size_t nb) {
     return true;
 }
 
-extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request *rq, fohip_call_result *res,
-                                    fohip_error *err) {
-    clear_error(err);
-    if (!ex || !rq || !res) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null executor, request or result");
-    std::lock_guard<std::mutex> lock(ex->mu);
+// fohip_price_lsm_call's body; `fixed` (the allocation-free entry point) carves the device
+// buffers from a fixed region instead of the executor's own.
+static int price_call(fohip_executor *ex, const fohip_call_request *rq, fohip_call_result *res, fohip_error *err,
+                      Workspace *fixed) {
     int rc;
     // lib.rs:1170-1248 parse order: financial model, regression spec, estimator
     if ((rc = fohip_validate_event_schedule(rq->schedule_times, rq->schedule_nodes, rq->dividend_nodes,
@@ -1319,7 +1384,7 @@ extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request
     std::vector<uint32_t> counts;
     std::vector<GroupPlan> plans{p};
     std::vector<double> strikes{rq->strike};
-    rc = run_groups(ex, plans, strikes, spec, out, &models, &counts, err);
+    rc = run_groups(ex, plans, strikes, spec, out, &models, &counts, err, fixed);
     if (rc) return rc;
     const RowOut &o = out[0];
     const uint32_t obs = static_cast<uint32_t>(p.obs_nodes.size());
@@ -1378,6 +1443,36 @@ extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request
         }
     }
     return FOHIP_OK;
+}
+
+extern "C" int fohip_price_lsm_call(fohip_executor *ex, const fohip_call_request *rq, fohip_call_result *res,
+                                    fohip_error *err) {
+    clear_error(err);
+    if (!ex || !rq || !res) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null executor, request or result");
+    std::lock_guard<std::mutex> lock(ex->mu);
+    return price_call(ex, rq, res, err, nullptr);
+}
+
+// One lock per device: every allocation-free call on a device uses that device's one region.
+static std::mutex g_noalloc_mu[kNoAllocMaxDevices];
+
+extern "C" int fohip_price_lsm_call_noalloc(fohip_executor *ex, const fohip_call_request *rq, fohip_call_result *res,
+                                            fohip_error *err) {
+    clear_error(err);
+    if (!ex || !rq || !res) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null executor, request or result");
+    if (ex->device < 0 || ex->device >= kNoAllocMaxDevices)
+        return fail(err, FOHIP_E_UNSUPPORTED, 0, "the allocation-free entry point supports devices 0..%d",
+                    kNoAllocMaxDevices - 1);
+    if (ex->profiling)
+        return fail(err, FOHIP_E_UNSUPPORTED, 0, "the allocation-free entry point needs per-kernel profiling off");
+    std::lock_guard<std::mutex> device_lock(g_noalloc_mu[ex->device]);
+    std::lock_guard<std::mutex> lock(ex->mu);
+    HIP_TRY(hipSetDevice(ex->device));
+    void *region = nullptr;
+    size_t bytes = 0;
+    HIP_TRY(noalloc_scratch(ex->device, &region, &bytes));
+    Workspace fixed(ex, region, bytes);
+    return price_call(ex, rq, res, err, &fixed);
 }
 
 // =========================================================================== grouped batch

@@ -12,6 +12,7 @@
 // order in a one-block-per-strike reduce kernel): results are independent of timing and tuning.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "fohip_device.hpp"
@@ -2036,6 +2037,31 @@ hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const Fi
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_reduce_origin, dim3(nrows), dim3(kBlock), 0, st, t, fp, rows, partials, models, errors);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- allocation-free scratch region
+
+// The buffers of an allocation-free pricing live here: kNoAllocScratchBytes per device, loaded with
+// the code object and held until the process ends. Host code must never pass `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns on each device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, void **region, size_t *bytes) {
+    static std::mutex mu;
+    static void *address[kNoAllocMaxDevices] = {};
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!address[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));   // on the current device
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        address[device] = p;
+    }
+    *region = address[device];
+    *bytes = kNoAllocScratchBytes;
+    return hipSuccess;
 }
 
 uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {

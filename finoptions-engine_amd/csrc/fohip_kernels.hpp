@@ -95,6 +95,12 @@ hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *m
 hipError_t launch_replay_path(hipStream_t st, const DevTables &t, uint32_t group, int evaluation,
                               uint32_t path, double *out);
 
+// Device region of the allocation-free entry point: `bytes` of a file-scope device array, its
+// address on `device` (which must be current) taken once and kept per device.
+constexpr size_t kNoAllocScratchBytes = FOHIP_NO_ALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch(int device, void **region, size_t *bytes);
+
 // Strike tiles available to the evaluation kernel for a given Greeks setting.
 uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested);
 

@@ -184,7 +184,7 @@ EXPORTED = (
     "fohip_validate_cumulative_yield_curve", "fohip_validate_event_schedule",
     "fohip_validate_constant_volatility", "fohip_validate_cash_dividend_amount",
     "fohip_validate_ragged_offsets", "fohip_set_profiling", "fohip_get_profile",
-    "fohip_reset_profile",
+    "fohip_reset_profile", "fohip_price_lsm_call_noalloc",
 )
 
 _lib: C.CDLL | None = None
@@ -233,6 +233,8 @@ def lib() -> C.CDLL:
     handle.fohip_set_near_tie_window.argtypes = [C.c_void_p, C.c_double]
     handle.fohip_price_lsm_call.argtypes = [C.c_void_p, C.POINTER(FohipCallRequest),
                                             C.POINTER(FohipCallResult), E]
+    handle.fohip_price_lsm_call_noalloc.argtypes = [C.c_void_p, C.POINTER(FohipCallRequest),
+                                                    C.POINTER(FohipCallResult), E]
     handle.fohip_price_lsm_grouped.argtypes = [C.c_void_p, C.POINTER(FohipGroupedRequest),
                                                C.POINTER(FohipBatchResult), E]
     handle.fohip_grouped_diagnostics.argtypes = [C.c_void_p, C.POINTER(FohipGroupedRequest),

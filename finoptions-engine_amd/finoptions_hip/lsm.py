@@ -262,6 +262,7 @@ def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typ
     executor: HipExecutor,
     method: Lsm,
     greeks: tuple[Greek, ...] = (),
+    allocation_free: bool = False,
 ) -> LsmResult:
     """Price a Bermudan call on the HIP device: fit the stopping policy by backward induction on
     the training paths, then value it on the independent evaluation paths.
@@ -269,7 +270,9 @@ def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typ
     The request is taken as given: a failed fit raises instead of lowering the degree or
     regularizing, the evaluation never reuses training paths, and neither the device nor the
     executor's method is changed. ``greeks`` adds pathwise sensitivities of the price under the
-    fitted (frozen) policy.
+    fitted (frozen) policy. ``allocation_free`` prices through fohip_price_lsm_call_noalloc, whose
+    device buffers come from a fixed per-device region instead of device allocations (bitwise the
+    same results; requests too large for the region raise ExecutionError).
 
     Raises:
         InputError: If a runtime type or a financial input is invalid.
@@ -311,8 +314,10 @@ def price_lsm_call(  # noqa: PLR0913 - financial inputs remain independently typ
     res.exercise_counts = ptr(counts, C.c_uint32)
     res.diagnostics = diags
     err = FohipError()
-    code = lib().fohip_price_lsm_call(executor._handle.ptr, C.byref(req), C.byref(res),  # noqa: SLF001
-                                      C.byref(err))
+    if type(allocation_free) is not bool:
+        raise InputError("allocation_free must be a plain bool")
+    entry = lib().fohip_price_lsm_call_noalloc if allocation_free else lib().fohip_price_lsm_call
+    code = entry(executor._handle.ptr, C.byref(req), C.byref(res), C.byref(err))  # noqa: SLF001
     raise_for(code, err)
     has_se = bool(res.has_standard_error)
     fit = tuple(_diagnostic(diags[r]) for r in range(obs - 1))

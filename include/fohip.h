@@ -252,6 +252,22 @@ typedef struct {
 int fohip_price_lsm_call(fohip_executor *executor, const fohip_call_request *request,
                          fohip_call_result *result, fohip_error *err);
 
+/* Allocation-free variant of fohip_price_lsm_call: the same request, result and errors, priced
+ * with bitwise the same results. No counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Every device buffer of the pricing is carved from one fixed region of
+ * FOHIP_NO_ALLOC_SCRATCH_BYTES per device (a file-scope device array, held by each device the
+ * library runs on until the process ends) and host transfers are staged through pageable memory,
+ * so the call makes no hipMalloc, hipFree or pinned-host allocation. A request whose buffers do
+ * not fit the region fails with FOHIP_E_UNSUPPORTED before any work is queued (about 2.5 M
+ * training + evaluation paths for a call with one exercise date). All calls on one device share
+ * the region and it keeps the last call's contents: calls that overlapped on one device would
+ * race, so the library serializes them per device within a process. Per-kernel profiling must be
+ * off (it creates events). */
+#define FOHIP_NO_ALLOC_SCRATCH_BYTES (64u << 20)
+int fohip_price_lsm_call_noalloc(fohip_executor *executor, const fohip_call_request *request,
+                                 fohip_call_result *result, fohip_error *err);
+
 /* ---- group-addressed strikes: price_lsm_call_grouped_batch ---- */
 enum { FOHIP_DIVIDENDS_CONTINUOUS_YIELD = 0, FOHIP_DIVIDENDS_CASH = 1, FOHIP_DIVIDENDS_ESCROWED_CASH = 2 };
 /* escrowed cash dividends (EscrowedVolatilityConvention, batch_lsm.py:52-56): the volatility
