@@ -73,14 +73,16 @@ __global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, 
 __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths, uint32_t count,
                                  double *out, uint32_t *draws, int *fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ uint64_t ring[kRingSlots * kBlock];
     load_zig(sx, sf);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= paths) return;
-    PathStream s;
-    s.init(seed, domain, path0 + p);
+    RingStream s;
+    s.init(seed, domain, path0 + p, ring, threadIdx.x, kBlock);
     for (uint32_t i = 0; i < count; ++i) {
         double z = 0.0;
         uint32_t d = 0;
+        s.top_up(true);
         if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, d)) atomicExch(fail, 1);
         out[static_cast<uint64_t>(p) * count + i] = z;
         draws[static_cast<uint64_t>(p) * count + i] = d;
@@ -99,14 +101,16 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
                                                      Store *__restrict__ paths_out, int evaluation,
                                                      unsigned long long *__restrict__ fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ uint64_t ring[kRingSlots * kBlock];
     load_zig(sx, sf);
     const uint2 item = items[blockIdx.x];
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = evaluation ? g.ne : g.nt;
     const uint32_t p = item.y * kChunk + blockIdx.y * kBlock + threadIdx.x;
     if (p >= n) return;
-    PathStream s;
-    s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p);
+    RingStream s;
+    s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p, ring,
+           threadIdx.x, kBlock);
     Store *out = paths_out + g.path_base + p;
     double spot = g.start;
     uint32_t row = 0;
@@ -118,6 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     for (uint32_t i = 0; i < g.intervals; ++i) {
         double z;
         uint32_t draws = 0;
+        s.top_up(true);
         if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
             ok = false;
             break;
@@ -1467,8 +1472,9 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
 #pragma unroll
         for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Tan(0);
     }
-    PathStream s;
-    if (!STORED) s.init(g->eval_seed, g->eval_domain, p);
+    __shared__ uint64_t ring[STORED ? 1 : kRingSlots * kBlock];
+    RingStream s;
+    if (!STORED) s.init(g->eval_seed, g->eval_domain, p, ring, threadIdx.x, kBlock);
     // STORED: this path's normals, time-major, written once by k_eval_normals (NaN marks an
     // exhausted draw budget)
     const double *zp = znorm + g->znorm_base + p;
@@ -1482,6 +1488,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             if (!(z == z)) ok = false;
         } else {
             uint32_t draws = 0;
+            s.top_up(ok);
             if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
         }
         const uint32_t si = step_base + iv;
@@ -1836,18 +1843,20 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
                                                          double *__restrict__ znorm,
                                                          unsigned long long *__restrict__ sim_fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ uint64_t ring[kRingSlots * kBlock];
     load_zig(sx, sf);
     const uint2 item = load_item(items, blockIdx.x);
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = g.ne, p = item.y * kEvalChunk + threadIdx.x;
     if (p >= n) return;
-    PathStream s;
-    s.init(g.eval_seed, g.eval_domain, p);
+    RingStream s;
+    s.init(g.eval_seed, g.eval_domain, p, ring, threadIdx.x, kBlock);
     double *out = znorm + g.znorm_base + p;
     bool ok = true;
     for (uint32_t iv = 0; iv < g.intervals; ++iv) {
         double z = 0.0;
         uint32_t draws = 0;
+        s.top_up(ok);
         if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
         out[static_cast<uint64_t>(iv) * n] = ok ? z : __builtin_nan("");
     }
