@@ -279,8 +279,14 @@ def main() -> int:
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the grouped-strike state, the grouped frame and the FP32-bulk method")
-    ap.add_argument("--only-c4", action="store_true",
-                    help="time the headline pricing only (for profiler runs)")
+    ap.add_argument("--only", choices=("c4", "c2", "c5"),
+                    help="time one workload's fused pricing (with delta/vega) and print a short "
+                         "line: c4 the ragged batch, c2 the grouped-strike state, c5 the grouped "
+                         "frame (for profiler runs)")
+    ap.add_argument("--fit-cashflows", choices=("auto", "value_columns", "stopping_rows"),
+                    default="auto", help="cashflow encoding of the fused fit step")
+    ap.add_argument("--batch-schedule", choices=("serial", "overlapped"), default="serial",
+                    help="how the memory-bounded batches of a request share the device")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the reference copy for the rocprofv3 FETCH_SIZE / WRITE_SIZE scale "
                          "factors: three elementwise copies of 256 MiB (64 Mi float32)")
@@ -296,7 +302,18 @@ def main() -> int:
     import finoptions_hip as fh
 
     fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
+    fast.tune(fit_cashflows=args.fit_cashflows, batch_schedule=args.batch_schedule)
     greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
+    if args.only in ("c2", "c5"):
+        if args.only == "c2":
+            fn, steps = (lambda: price_c2(fh, fast, greeks)), C2_INTERVALS * 2.0 * C2_PATHS
+        else:
+            run5, steps = c5_pricer(fh)
+            fn = lambda: run5(fast, greeks)   # noqa: E731
+        ms, _ = timed(args.only + " fast", fn, args.steps, args.warmup)
+        print(json.dumps({"workload": args.only, "fit_cashflows": args.fit_cashflows,
+                          "path_steps_per_s": steps / (ms / 1e3), "ms_per_step": ms}), flush=True)
+        return 0
     w, steps_per = c4_workload()
     path_steps = float(np.sum(steps_per))
     log(f"device: {fast.device_name}; ragged batch of {C4_CONTRACTS} contracts, "
@@ -306,8 +323,9 @@ def main() -> int:
     ms_fast, fast_res = timed("c4 fast", lambda: price_c4(fh, fast, w, greeks), args.steps,
                               args.warmup)
     value = path_steps / (ms_fast / 1e3)
-    if args.only_c4:
-        print(json.dumps({"metric": METRIC, "value": value, "ms_per_step": ms_fast}), flush=True)
+    if args.only == "c4":
+        print(json.dumps({"workload": "c4", "fit_cashflows": args.fit_cashflows,
+                          "path_steps_per_s": value, "ms_per_step": ms_fast}), flush=True)
         return 0
 
     # ---- per-kernel times of one headline pricing, HIP events on the executor's stream

@@ -269,6 +269,47 @@ extern "C" int fohip_validate_ragged_offsets(const int64_t *offsets, size_t len,
     return FOHIP_OK;
 }
 
+extern "C" int fohip_validate_ragged_exercise_times(const int64_t *offsets, size_t len, const double *times,
+                                                    fohip_error *err) {
+    clear_error(err);
+    if (len < 2 || !offsets) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "ragged offsets need at least two entries");
+    const int64_t base = offsets[0];
+    for (size_t r = 0; r + 1 < len; ++r) {
+        const double *t = times + (offsets[r] - base);
+        const size_t n = static_cast<size_t>(offsets[r + 1] - offsets[r]);
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(t[i]) || t[i] <= 0.0)
+                return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "row %zu exercise times must be finite and positive", r);
+        for (size_t i = 1; i < n; ++i)
+            if (!(t[i] > t[i - 1]))
+                return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "row %zu exercise times must be strictly increasing", r);
+    }
+    return FOHIP_OK;
+}
+
+extern "C" int fohip_validate_ragged_cash_dividends(const int64_t *offsets, size_t len, const double *times,
+                                                    const double *amounts, fohip_error *err) {
+    clear_error(err);
+    if (len < 2 || !offsets) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "ragged offsets need at least two entries");
+    const int64_t base = offsets[0];
+    for (size_t r = 0; r + 1 < len; ++r) {
+        const double *t = times + (offsets[r] - base), *a = amounts + (offsets[r] - base);
+        const size_t n = static_cast<size_t>(offsets[r + 1] - offsets[r]);
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(t[i]) || t[i] <= 0.0)
+                return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "row %zu cash-dividend times must be finite and positive",
+                            r);
+        for (size_t i = 1; i < n; ++i)
+            if (t[i] < t[i - 1])
+                return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "row %zu cash-dividend times must be nondecreasing", r);
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(a[i]) || a[i] < 0.0 || (a[i] == 0.0 && std::signbit(a[i])))
+                return fail(err, FOHIP_E_INVALID_ARGUMENT, 0,
+                            "row %zu cash-dividend amounts must be finite and nonnegative", r);
+    }
+    return FOHIP_OK;
+}
+
 // ============================================================================== executor
 
 namespace {
@@ -331,20 +372,39 @@ struct PinnedBuf {
     }
 };
 
+// The device and host buffers of one batch in flight, and the events that order its phases across
+// the executor's two streams: simulation done (generation stream), backward induction done
+// (fit stream), results downloaded (generation stream).
+struct BatchSlot {
+    DevBuf paths, values, stops, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, eval_out,
+        summary, replay, znorm, ties;
+    PinnedBuf staging;   // the batch's tables and items on the way in, results on the way out
+    hipEvent_t simulated = nullptr, fitted = nullptr, done = nullptr;
+    void release() {
+        for (DevBuf *b : {&paths, &values, &stops, &models, &part_t, &part_e, &counts, &nosup, &errors, &fail_t,
+                          &fail_e, &tables, &eval_out, &summary, &replay, &znorm, &ties})
+            b->release();
+        staging.release();
+        for (hipEvent_t e : {simulated, fitted, done})
+            if (e) (void)hipEventDestroy(e);
+        simulated = fitted = done = nullptr;
+    }
+};
+
 }  // namespace
 
 struct fohip_executor {
     int32_t device = 0;
     int32_t precision = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // backward induction; every single-stream entry point
+    hipStream_t gen = nullptr;      // uploads, simulation, evaluation and downloads of batches
     char name[256] = {0};
-    fohip_tuning tuning = {0, 0};
+    fohip_tuning tuning = {0, 0, 0, 0};
     double near_tie_window = 0.0;   // 0: exact ties; w > 0: |intrinsic - continuation| <= w |continuation|
-    size_t batch_budget = 0;   // bytes for paths + values per batch
+    size_t batch_budget = 0;   // default bytes for paths + values of a batch (serial schedule)
     std::mutex mu;
-    DevBuf paths, values, stops, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, items_t,
-        eval_out, summary, replay, probe_a, probe_b, probe_c, znorm, ties;
-    PinnedBuf staging;   // per-batch tables and items on the way in, results on the way out
+    BatchSlot slot[2];
+    DevBuf items_t, probe_a, probe_b, probe_c;
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
     std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
@@ -369,8 +429,15 @@ struct fohip_executor {
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
+    // Accumulates the launches whose stop event has completed; the others stay pending (a batch in
+    // flight on the other stream).
     void collect() {
+        std::vector<Pending> later;
         for (auto &p : pending) {
+            if (hipEventQuery(p.stop) != hipSuccess) {
+                later.push_back(p);
+                continue;
+            }
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
                 fohip_kernel_stat *st = nullptr;
@@ -389,27 +456,28 @@ struct fohip_executor {
             event_pool.push_back(p.start);
             event_pool.push_back(p.stop);
         }
-        pending.clear();
+        pending.swap(later);
     }
 };
 
 namespace {
-// Brackets one launch with events on the executor's stream when profiling is enabled.
+// Brackets one launch with events on the stream it is queued on when profiling is enabled.
 struct ProfScope {
     fohip_executor *ex;
+    hipStream_t stream;
     const char *name;
     hipEvent_t start = nullptr;
-    ProfScope(fohip_executor *e, const char *n) : ex(e), name(n) {
+    ProfScope(fohip_executor *e, hipStream_t s, const char *n) : ex(e), stream(s), name(n) {
         if (ex->profiling) {
             start = ex->take_event();
-            if (start) (void)hipEventRecord(start, ex->stream);
+            if (start) (void)hipEventRecord(start, stream);
         }
     }
     ~ProfScope() {
         if (ex->profiling && start) {
             hipEvent_t stop = ex->take_event();
             if (stop) {
-                (void)hipEventRecord(stop, ex->stream);
+                (void)hipEventRecord(stop, stream);
                 ex->pending.push_back({name, start, stop});
             }
         }
@@ -417,10 +485,10 @@ struct ProfScope {
 };
 }  // namespace
 
-#define PROF_LAUNCH(name, expr)          \
-    do {                                 \
-        ProfScope _scope(ex, name);      \
-        HIP_TRY(expr);                   \
+#define PROF_LAUNCH(stream, name, expr)          \
+    do {                                         \
+        ProfScope _scope(ex, stream, name);      \
+        HIP_TRY(expr);                           \
     } while (0)
 
 extern "C" int fohip_abi_version(void) { return FOHIP_ABI_VERSION; }
@@ -455,12 +523,18 @@ extern "C" int fohip_executor_create(int32_t device, int32_t precision, fohip_ex
     ex->device = device;
     ex->precision = precision;
     std::snprintf(ex->name, sizeof ex->name, "%s (%s)", prop.name, prop.gcnArchName);
-    // Paths + values of one batch of groups stay below a quarter of device memory.
+    // Paths + values of one batch of groups stay below a quarter of device memory (the two slots
+    // of the overlapped schedule hold an eighth each).
     ex->batch_budget = std::max<size_t>(prop.totalGlobalMem / 4, size_t(1) << 30);
+    // both streams at the default priority
     e = hipStreamCreateWithFlags(&ex->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ex->gen, hipStreamNonBlocking);
+    for (BatchSlot &b : ex->slot)
+        for (hipEvent_t *ev : {&b.simulated, &b.fitted, &b.done})
+            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete ex;
-        return fail(err, FOHIP_E_HIP, 0, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        (void)fohip_executor_destroy(ex);
+        return fail(err, FOHIP_E_HIP, 0, "stream or event creation failed: %s", hipGetErrorString(e));
     }
     *out = ex;
     return FOHIP_OK;
@@ -469,19 +543,17 @@ extern "C" int fohip_executor_create(int32_t device, int32_t precision, fohip_ex
 extern "C" int fohip_executor_destroy(fohip_executor *ex) {
     if (!ex) return FOHIP_OK;
     (void)hipSetDevice(ex->device);
-    (void)hipStreamSynchronize(ex->stream);
-    for (DevBuf *b : {&ex->paths, &ex->values, &ex->stops, &ex->models, &ex->part_t, &ex->part_e, &ex->counts, &ex->nosup,
-                      &ex->errors, &ex->fail_t, &ex->fail_e, &ex->tables, &ex->items_t,
-                      &ex->eval_out, &ex->summary, &ex->replay, &ex->probe_a, &ex->probe_b, &ex->probe_c, &ex->znorm,
-                      &ex->ties})
-        b->release();
-    ex->staging.release();
+    for (hipStream_t s : {ex->stream, ex->gen})
+        if (s) (void)hipStreamSynchronize(s);
+    for (BatchSlot &b : ex->slot) b.release();
+    for (DevBuf *b : {&ex->items_t, &ex->probe_a, &ex->probe_b, &ex->probe_c}) b->release();
     for (auto &p : ex->pending) {
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
     }
     for (hipEvent_t e : ex->event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(ex->stream);
+    for (hipStream_t s : {ex->stream, ex->gen})
+        if (s) (void)hipStreamDestroy(s);
     delete ex;
     return FOHIP_OK;
 }
@@ -519,7 +591,9 @@ extern "C" int fohip_reset_profile(fohip_executor *ex) {
 
 extern "C" int fohip_set_tuning(fohip_executor *ex, const fohip_tuning *t) {
     if (!ex || !t) return FOHIP_E_INVALID_ARGUMENT;
-    if (t->eval_strike_tile > 16 || t->fit_cashflows > FOHIP_CASHFLOWS_STOPPING_ROWS) return FOHIP_E_INVALID_ARGUMENT;
+    if (t->eval_strike_tile > 16 || t->fit_cashflows > FOHIP_CASHFLOWS_STOPPING_ROWS ||
+        t->batch_schedule > FOHIP_BATCHES_OVERLAPPED)
+        return FOHIP_E_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ex->mu);
     ex->tuning = *t;
     return FOHIP_OK;
@@ -783,10 +857,10 @@ struct Workspace {
         used = needed;
         return hipSuccess;
     }
-    hipError_t host_staging(size_t bytes, char **ptr) {
+    hipError_t host_staging(PinnedBuf &own, size_t bytes, char **ptr) {
         if (!region) {
-            hipError_t e = ex->staging.ensure(bytes);
-            *ptr = ex->staging.ptr;
+            hipError_t e = own.ensure(bytes);
+            *ptr = own.ptr;
             return e;
         }
         if (pageable.size() < bytes) pageable.resize(bytes);
@@ -797,23 +871,101 @@ struct Workspace {
 
 }  // namespace
 
-// Runs the full pipeline for a list of groups (processed in memory-bounded batches).
+namespace {
+
+// One memory-bounded batch of consecutive groups on its way through the device: the host tables
+// built for it, the slot buffers it claimed and the host copies its results come back into.
+struct BatchRun {
+    int slot = 0;
+    std::vector<GroupPlan> plans;
+    uint32_t row0 = 0, nrows = 0, ngroups = 0;
+    std::vector<double> bstrikes, binv;
+    GroupTables gt;
+    uint64_t path_elems = 0, value_elems = 0, model_count = 0, part_t = 0, part_e = 0, count_slots = 0;
+    int64_t bad_group = -1, bad_node = -1;
+    uint64_t znorm_elems = 0;
+    std::vector<uint32_t> strike_group;
+    uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
+    std::vector<uint2> items_t, items_e, items_b;
+    std::vector<uint32_t> rows_sorted, step_items, step_rows, step_items_b;
+    static constexpr int kTableParts = 22;
+    size_t part_bytes[kTableParts] = {};
+    const void *part_host[kTableParts] = {};
+    size_t part_off[kTableParts] = {};
+    size_t tables_bytes = 0;
+    bool stored_normals = false, escrow = false;
+    // device side
+    void *paths_d = nullptr, *values_d = nullptr, *stops_d = nullptr, *models_d = nullptr, *part_t_d = nullptr,
+         *part_e_d = nullptr, *counts_d = nullptr, *nosup_d = nullptr, *errors_d = nullptr, *fail_t_d = nullptr,
+         *fail_e_d = nullptr, *eval_out_d = nullptr, *summary_d = nullptr, *ties_d = nullptr, *tables_d = nullptr,
+         *znorm_d = nullptr, *replay_d = nullptr;
+    DevTables t = {};
+    const uint2 *it = nullptr, *ite = nullptr, *itb = nullptr;
+    const uint32_t *rows = nullptr;
+    char *staging = nullptr;
+    // host copies of the results
+    std::vector<unsigned long long> fail_t, fail_e, ties_h;
+    std::vector<RowError> row_err;
+    std::vector<double> eval_out;
+    std::vector<RowSummary> summary;
+    std::vector<Model> models_h;
+    std::vector<uint32_t> counts_h;
+    struct Download {
+        void *host;
+        void *const *dev;   // read when the transfer is queued, after the claims
+        size_t bytes;
+    };
+    std::vector<Download> downloads() {
+        return {{fail_t.data(), (&fail_t_d), ngroups * sizeof(unsigned long long)},
+                {fail_e.data(), (&fail_e_d), ngroups * sizeof(unsigned long long)},
+                {row_err.data(), (&errors_d), nrows * sizeof(RowError)},
+                {eval_out.data(), (&eval_out_d), eval_out.size() * sizeof(double)},
+                {summary.data(), (&summary_d), nrows * sizeof(RowSummary)},
+                {models_h.data(), (&models_d), models_h.size() * sizeof(Model)},
+                {counts_h.data(), (&counts_d), counts_h.size() * sizeof(uint32_t)},
+                {ties_h.data(), (&ties_d), ties_h.size() * sizeof(unsigned long long)}};
+    }
+};
+
+// Waits for both streams of the executor when a request returns, so that no batch of it is still
+// in flight when the next request reuses the slots.
+struct DrainOnExit {
+    fohip_executor *ex;
+    ~DrainOnExit() {
+        (void)hipStreamSynchronize(ex->gen);
+        (void)hipStreamSynchronize(ex->stream);
+    }
+};
+
+}  // namespace
+
+// Runs the full pipeline for a list of groups, in memory-bounded batches of consecutive groups.
+//
+// Serial schedule: each batch's simulation, backward induction and evaluation run in turn, while
+// the host builds the next batch's tables.
+// Overlapped schedule: batch b's backward induction runs on the executor's stream while batch
+// b + 1's simulation and batch b - 1's evaluation run on the generation stream. The host builds
+// batch b + 1's tables while the device works, finishes batch b - 1 (its results and failures,
+// in request order) before reusing its slot for b + 1, and queues b + 1's simulation ahead of b's
+// evaluation so that the generation stream never waits behind a fit it does not need. Results
+// do not depend on the schedule: every reduction tree is fixed per group.
 static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, const std::vector<double> &strikes,
                       const RunSpec &spec, std::vector<RowOut> &out, std::vector<Model> *models_out,
                       std::vector<uint32_t> *counts_out, fohip_error *err, Workspace *fixed = nullptr) {
     Workspace grown(ex);
     Workspace &ws = fixed ? *fixed : grown;
     HIP_TRY(hipSetDevice(ex->device));
-    hipStream_t st = ex->stream;
+    DrainOnExit drain{ex};
+    hipStream_t fit_st = ex->stream, gen_st = ex->gen;
+    // the fixed region of the allocation-free entry point holds one batch at a time
+    const bool overlapped = !fixed && ex->tuning.batch_schedule == FOHIP_BATCHES_OVERLAPPED;
     const bool f32 = spec.store32;
     const size_t real = f32 ? sizeof(float) : sizeof(double);
     out.assign(strikes.size(), RowOut());
-    // memory-bounded batches of consecutive groups
     // Cashflow encoding of the fused backward induction, one choice per request: stopping rows
     // (1 byte per strike and path) when every group has at most 255 observation rows and the
-    // tuning asks for them. The default keeps value columns: the fit step is bound by its
-    // instruction issue, not by the column bytes, and the stopping rows' gathers cost more than
-    // the bytes they save (measured on the 64-strike, 256-strike and single-strike shapes).
+    // tuning asks for them. The default keeps value columns: the stopping rows' gathers cost more
+    // than the bytes they save (measured on the 64-strike, 256-strike and single-strike shapes).
     bool use_stops = false;
     uint32_t most_strikes = 0;
     size_t most_obs = 0;
@@ -831,14 +983,17 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
     const uint32_t tile = ex->near_tie_window != 0.0 ? 1u
                                                      : eval_tile_for(most_strikes, greeks, ex->tuning.eval_strike_tile);
     // Batches of consecutive groups bounded by memory (paths, cashflows and stored normals within
-    // the batch budget) and by the launch grids: a dispatch holds at most 2^32 - 1 work-items, so
+    // the slot budget) and by the launch grids: a dispatch holds at most 2^32 - 1 work-items, so
     // no kernel of a batch may have more than kMaxBlocks blocks of 256 threads. A grid is (work
     // items of the batch, rounded up to 8 for the XCD mapping) x (strike columns of the batch).
+    // An explicit batch_mib gives both schedules the same batches.
     constexpr uint64_t kMaxBlocks = 0xFFFFFFFFull / 256;   // 256-thread blocks
-    auto grid_blocks = [&](uint64_t items_t, uint64_t items_e, uint64_t items_b, uint64_t strikes) {
-        const uint64_t fit = ((items_t + 7) / 8) * 8 * strikes;
-        const uint64_t eval = ((items_e + 7) / 8) * 8 * ((strikes + tile - 1) / tile);
-        const uint64_t bulk = ((items_b + 7) / 8) * 8 * ((strikes + kBulkStrikes - 1) / kBulkStrikes);
+    const size_t budget = ex->tuning.batch_mib ? static_cast<size_t>(ex->tuning.batch_mib) << 20
+                                               : overlapped ? ex->batch_budget / 2 : ex->batch_budget;
+    auto grid_blocks = [&](uint64_t items_t, uint64_t items_e, uint64_t items_b, uint64_t ns) {
+        const uint64_t fit = ((items_t + 7) / 8) * 8 * ns;
+        const uint64_t eval = ((items_e + 7) / 8) * 8 * ((ns + tile - 1) / tile);
+        const uint64_t bulk = ((items_b + 7) / 8) * 8 * ((ns + kBulkStrikes - 1) / kBulkStrikes);
         return std::max({fit, eval, bulk, items_e});
     };
     std::vector<std::pair<size_t, size_t>> batches;
@@ -857,7 +1012,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                 return fail(err, FOHIP_E_UNSUPPORTED, 0,
                             "group %zu (%llu strikes, %llu + %llu paths) exceeds one launch grid", g,
                             (unsigned long long)ns, (unsigned long long)p.nt, (unsigned long long)p.ne);
-            if (g > begin && (bytes + need > ex->batch_budget ||
+            if (g > begin && (bytes + need > budget ||
                               grid_blocks(it_t + ct, it_e + ce, it_b + cb, std::max(cols, ns)) > kMaxBlocks)) {
                 batches.push_back({begin, g});
                 begin = g;
@@ -875,275 +1030,307 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
     if (models_out) models_out->clear();
     if (counts_out) counts_out->clear();
     ex->last_models.clear();
-    for (auto [gb, ge] : batches) {
-        std::vector<GroupPlan> plans(all.begin() + gb, all.begin() + ge);
+
+    // ---- host side of a batch: plans rebased to the batch, tables, work items, result vectors
+    auto prepare = [&](size_t b) {
+        auto B = std::make_unique<BatchRun>();
+        BatchRun &R = *B;
+        R.slot = overlapped ? static_cast<int>(b & 1) : 0;
+        const auto [gb, ge] = batches[b];
+        R.plans.assign(all.begin() + gb, all.begin() + ge);
         // strike rows of this batch are contiguous: rebase to the batch
-        const uint32_t row0 = plans.front().strike_begin;
-        const uint32_t row1 = plans.back().strike_end;
-        const uint32_t nrows = row1 - row0;
-        std::vector<double> bstrikes(strikes.begin() + row0, strikes.begin() + row1);
-        std::vector<double> binv(bstrikes.size());   // 1/K, correctly rounded as on the device
-        for (size_t i = 0; i < bstrikes.size(); ++i) binv[i] = 1.0 / bstrikes[i];
-        for (auto &p : plans) {
-            p.strike_begin -= row0;
-            p.strike_end -= row0;
+        R.row0 = R.plans.front().strike_begin;
+        const uint32_t row1 = R.plans.back().strike_end;
+        R.nrows = row1 - R.row0;
+        R.bstrikes.assign(strikes.begin() + R.row0, strikes.begin() + row1);
+        R.binv.resize(R.bstrikes.size());   // 1/K, correctly rounded as on the device
+        for (size_t i = 0; i < R.bstrikes.size(); ++i) R.binv[i] = 1.0 / R.bstrikes[i];
+        for (auto &p : R.plans) {
+            p.strike_begin -= R.row0;
+            p.strike_end -= R.row0;
         }
-        GroupTables gt;
-        uint64_t path_elems, value_elems, model_count, part_t, part_e, count_slots;
-        int64_t bad_group, bad_node;
-        build_tables(plans, bstrikes, f32, gt, path_elems, value_elems, model_count, part_t, part_e, count_slots, err,
-                     bad_group, bad_node);
-        uint64_t znorm_elems = 0;   // evaluation normals, [intervals][ne] per group
-        for (DevGroup &dg : gt.groups) {
-            dg.znorm_base = znorm_elems;
-            znorm_elems += static_cast<uint64_t>(dg.intervals) * dg.ne;
+        build_tables(R.plans, R.bstrikes, f32, R.gt, R.path_elems, R.value_elems, R.model_count, R.part_t, R.part_e,
+                     R.count_slots, err, R.bad_group, R.bad_node);
+        for (DevGroup &dg : R.gt.groups) {   // evaluation normals, [intervals][ne] per group
+            dg.znorm_base = R.znorm_elems;
+            R.znorm_elems += static_cast<uint64_t>(dg.intervals) * dg.ne;
         }
-        const uint32_t ngroups = static_cast<uint32_t>(plans.size());
-        std::vector<uint32_t> strike_group(nrows);
-        uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
-        for (uint32_t g = 0; g < ngroups; ++g) {
-            for (uint32_t s = plans[g].strike_begin; s < plans[g].strike_end; ++s) strike_group[s] = g;
-            max_strikes = std::max(max_strikes, plans[g].strike_end - plans[g].strike_begin);
-            max_obs = std::max(max_obs, gt.groups[g].obs);
-            max_fit = std::max(max_fit, gt.groups[g].fit_rows);
+        R.ngroups = static_cast<uint32_t>(R.plans.size());
+        R.strike_group.resize(R.nrows);
+        for (uint32_t g = 0; g < R.ngroups; ++g) {
+            for (uint32_t s = R.plans[g].strike_begin; s < R.plans[g].strike_end; ++s) R.strike_group[s] = g;
+            R.max_strikes = std::max(R.max_strikes, R.plans[g].strike_end - R.plans[g].strike_begin);
+            R.max_obs = std::max(R.max_obs, R.gt.groups[g].obs);
+            R.max_fit = std::max(R.max_fit, R.gt.groups[g].fit_rows);
         }
         // work items ordered by fit rows (descending) so each backward step uses a prefix
-        std::vector<uint32_t> order(ngroups);
-        for (uint32_t g = 0; g < ngroups; ++g) order[g] = g;
+        std::vector<uint32_t> order(R.ngroups);
+        for (uint32_t g = 0; g < R.ngroups; ++g) order[g] = g;
         std::stable_sort(order.begin(), order.end(),
-                         [&](uint32_t a, uint32_t b) { return gt.groups[a].fit_rows > gt.groups[b].fit_rows; });
-        std::vector<uint2> items_t, items_e, items_b;
-        std::vector<uint32_t> rows_sorted;
-        std::vector<uint32_t> step_items(max_fit + 1, 0), step_rows(max_fit + 1, 0), step_items_b(max_fit + 1, 0);
+                         [&](uint32_t a, uint32_t c) { return R.gt.groups[a].fit_rows > R.gt.groups[c].fit_rows; });
+        R.step_items.assign(R.max_fit + 1, 0);
+        R.step_rows.assign(R.max_fit + 1, 0);
+        R.step_items_b.assign(R.max_fit + 1, 0);
         for (uint32_t g : order) {
-            for (uint32_t c = 0; c < gt.groups[g].chunks_t; ++c) items_t.push_back(make_uint2(g, c));
-            for (uint32_t c = 0; c < gt.groups[g].chunks_b; ++c) items_b.push_back(make_uint2(g, c));
-            for (uint32_t s = plans[g].strike_begin; s < plans[g].strike_end; ++s) rows_sorted.push_back(s);
+            for (uint32_t c = 0; c < R.gt.groups[g].chunks_t; ++c) R.items_t.push_back(make_uint2(g, c));
+            for (uint32_t c = 0; c < R.gt.groups[g].chunks_b; ++c) R.items_b.push_back(make_uint2(g, c));
+            for (uint32_t s = R.plans[g].strike_begin; s < R.plans[g].strike_end; ++s) R.rows_sorted.push_back(s);
         }
-        for (uint32_t g = 0; g < ngroups; ++g)
-            for (uint32_t c = 0; c < gt.groups[g].chunks_e; ++c) items_e.push_back(make_uint2(g, c));
-        for (uint32_t j = 0; j <= max_fit; ++j) {
+        for (uint32_t g = 0; g < R.ngroups; ++g)
+            for (uint32_t c = 0; c < R.gt.groups[g].chunks_e; ++c) R.items_e.push_back(make_uint2(g, c));
+        for (uint32_t j = 0; j <= R.max_fit; ++j) {
             uint32_t ni = 0, nr = 0, nb = 0;
             for (uint32_t g : order) {
-                if (gt.groups[g].fit_rows > j || j == max_fit) {
-                    ni += gt.groups[g].chunks_t;
-                    nb += gt.groups[g].chunks_b;
-                    nr += plans[g].strike_end - plans[g].strike_begin;
+                if (R.gt.groups[g].fit_rows > j || j == R.max_fit) {
+                    ni += R.gt.groups[g].chunks_t;
+                    nb += R.gt.groups[g].chunks_b;
+                    nr += R.plans[g].strike_end - R.plans[g].strike_begin;
                 }
             }
-            step_items[j] = ni;
-            step_items_b[j] = nb;
-            step_rows[j] = nr;
+            R.step_items[j] = ni;
+            R.step_items_b[j] = nb;
+            R.step_rows[j] = nr;
         }
-        // ---- device buffers
-        const size_t tables_bytes_list[] = {
+        const GroupTables &gt = R.gt;
+        const size_t bytes[BatchRun::kTableParts] = {
             gt.groups.size() * sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8,
             gt.log_drift32.size() * 4, gt.diffusion32.size() * 4, gt.dt.size() * 8, gt.sqrt_dt.size() * 8,
             gt.cash.size() * 8, gt.node_flags.size() * 4, gt.disc0.size() * 8, gt.disc_step.size() * 8,
-            gt.row_time.size() * 8, gt.row_node.size() * 4, bstrikes.size() * 8, strike_group.size() * 4,
-            gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, binv.size() * 8,
-            items_t.size() * sizeof(uint2), items_e.size() * sizeof(uint2), items_b.size() * sizeof(uint2),
-            rows_sorted.size() * sizeof(uint32_t)};
-        size_t tables_bytes = 0;
-        std::vector<size_t> offs;
-        for (size_t b : tables_bytes_list) {
-            offs.push_back(tables_bytes);
-            tables_bytes += (b + 255) & ~size_t(255);
+            gt.row_time.size() * 8, gt.row_node.size() * 4, R.bstrikes.size() * 8, R.strike_group.size() * 4,
+            gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, R.binv.size() * 8,
+            R.items_t.size() * sizeof(uint2), R.items_e.size() * sizeof(uint2), R.items_b.size() * sizeof(uint2),
+            R.rows_sorted.size() * sizeof(uint32_t)};
+        const void *host[BatchRun::kTableParts] = {
+            gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
+            gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
+            gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
+            gt.row_node.data(), R.bstrikes.data(), R.strike_group.data(),
+            gt.row_escrow.data(), gt.row_escrow_dr.data(), R.binv.data(),
+            R.items_t.data(), R.items_e.data(), R.items_b.data(), R.rows_sorted.data()};
+        for (int i = 0; i < BatchRun::kTableParts; ++i) {
+            R.part_bytes[i] = bytes[i];
+            R.part_host[i] = host[i];
+            R.part_off[i] = R.tables_bytes;
+            R.tables_bytes += (bytes[i] + 255) & ~size_t(255);
         }
-        // the evaluation normals' region is needed before the launches too, so that every buffer of
-        // the batch is claimed (and a fixed workspace found large enough) before any work is queued
-        const bool stored_normals = spec.fused && tile > 1 && (max_strikes + tile - 1) / tile >= 2 && znorm_elems > 0;
+        // Several strike tiles read one set of evaluation normals from HBM instead of each
+        // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams). Single-
+        // strike tiles always regenerate: that form has no stored-normal kernel.
+        R.stored_normals = spec.fused && tile > 1 && (R.max_strikes + tile - 1) / tile >= 2 && R.znorm_elems > 0;
+        for (const GroupPlan &P : R.plans) R.escrow = R.escrow || P.escrowed;
+        R.fail_t.resize(R.ngroups);
+        R.fail_e.resize(R.ngroups);
+        R.row_err.resize(R.nrows);
+        R.eval_out.resize(static_cast<size_t>(R.nrows) * 16);
+        R.summary.resize(R.nrows);
+        R.models_h.resize(models_out ? R.model_count : 0);
+        R.counts_h.resize(counts_out ? R.count_slots : 0);
+        // tied training decisions counted with atomics by the F64 fit passes and the last fitted
+        // row; the fused pass's are in the row summary and the evaluation's in eval_out
+        R.ties_h.resize(R.nrows);
+        return B;
+    };
+
+    // ---- slot buffers, upload and simulation (generation stream); backward induction (fit stream)
+    auto issue_front = [&](BatchRun &R) -> int {
+        BatchSlot &S = ex->slot[R.slot];
+        // every buffer of the batch is claimed (and a fixed workspace found large enough) before
+        // any work is queued
         ws.begin_batch();
-        void *paths_d = nullptr, *values_d = nullptr, *stops_d = nullptr, *models_d = nullptr, *part_t_d = nullptr,
-             *part_e_d = nullptr, *counts_d = nullptr, *nosup_d = nullptr, *errors_d = nullptr, *fail_t_d = nullptr,
-             *fail_e_d = nullptr, *eval_out_d = nullptr, *summary_d = nullptr, *ties_d = nullptr, *tables_d = nullptr,
-             *znorm_d = nullptr, *replay_d = nullptr;
-        HIP_TRY(ws.claim(ex->tables, tables_bytes, &tables_d));
-        HIP_TRY(ws.claim(ex->paths, std::max<uint64_t>(path_elems, 1) * real, &paths_d));
+        HIP_TRY(ws.claim(S.tables, R.tables_bytes, &R.tables_d));
+        HIP_TRY(ws.claim(S.paths, std::max<uint64_t>(R.path_elems, 1) * real, &R.paths_d));
         if (use_stops)
-            HIP_TRY(ws.claim(ex->stops, (std::max<uint64_t>(value_elems, 1) + 3) & ~uint64_t(3), &stops_d));
+            HIP_TRY(ws.claim(S.stops, (std::max<uint64_t>(R.value_elems, 1) + 3) & ~uint64_t(3), &R.stops_d));
         else
-            HIP_TRY(ws.claim(ex->values, std::max<uint64_t>(value_elems, 1) * real, &values_d));
-        HIP_TRY(ws.claim(ex->models, std::max<uint64_t>(model_count, 1) * sizeof(Model), &models_d));
-        HIP_TRY(ws.claim(ex->part_t, std::max<uint64_t>(part_t, 1) * kPartialStride * sizeof(double), &part_t_d));
-        HIP_TRY(ws.claim(ex->part_e, std::max<uint64_t>(part_e, 1) * kPartialStride * sizeof(double), &part_e_d));
-        HIP_TRY(ws.claim(ex->counts, std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t), &counts_d));
-        HIP_TRY(ws.claim(ex->nosup, std::max<uint64_t>(count_slots, 1) * sizeof(uint32_t), &nosup_d));
-        HIP_TRY(ws.claim(ex->errors, std::max<uint32_t>(nrows, 1) * sizeof(RowError), &errors_d));
-        HIP_TRY(ws.claim(ex->fail_t, ngroups * sizeof(unsigned long long), &fail_t_d));
-        HIP_TRY(ws.claim(ex->fail_e, ngroups * sizeof(unsigned long long), &fail_e_d));
-        HIP_TRY(ws.claim(ex->eval_out, std::max<uint32_t>(nrows, 1) * 16 * sizeof(double), &eval_out_d));
-        HIP_TRY(ws.claim(ex->summary, std::max<uint32_t>(nrows, 1) * sizeof(RowSummary), &summary_d));
-        HIP_TRY(ws.claim(ex->ties, std::max<uint32_t>(nrows, 1) * sizeof(unsigned long long), &ties_d));
-        HIP_TRY(ws.claim(ex->replay, 4 * sizeof(double), &replay_d));
-        if (stored_normals) HIP_TRY(ws.claim(ex->znorm, znorm_elems * sizeof(double), &znorm_d));
+            HIP_TRY(ws.claim(S.values, std::max<uint64_t>(R.value_elems, 1) * real, &R.values_d));
+        HIP_TRY(ws.claim(S.models, std::max<uint64_t>(R.model_count, 1) * sizeof(Model), &R.models_d));
+        HIP_TRY(ws.claim(S.part_t, std::max<uint64_t>(R.part_t, 1) * kPartialStride * sizeof(double), &R.part_t_d));
+        HIP_TRY(ws.claim(S.part_e, std::max<uint64_t>(R.part_e, 1) * kPartialStride * sizeof(double), &R.part_e_d));
+        HIP_TRY(ws.claim(S.counts, std::max<uint64_t>(R.count_slots, 1) * sizeof(uint32_t), &R.counts_d));
+        HIP_TRY(ws.claim(S.nosup, std::max<uint64_t>(R.count_slots, 1) * sizeof(uint32_t), &R.nosup_d));
+        HIP_TRY(ws.claim(S.errors, std::max<uint32_t>(R.nrows, 1) * sizeof(RowError), &R.errors_d));
+        HIP_TRY(ws.claim(S.fail_t, R.ngroups * sizeof(unsigned long long), &R.fail_t_d));
+        HIP_TRY(ws.claim(S.fail_e, R.ngroups * sizeof(unsigned long long), &R.fail_e_d));
+        HIP_TRY(ws.claim(S.eval_out, std::max<uint32_t>(R.nrows, 1) * 16 * sizeof(double), &R.eval_out_d));
+        HIP_TRY(ws.claim(S.summary, std::max<uint32_t>(R.nrows, 1) * sizeof(RowSummary), &R.summary_d));
+        HIP_TRY(ws.claim(S.ties, std::max<uint32_t>(R.nrows, 1) * sizeof(unsigned long long), &R.ties_d));
+        HIP_TRY(ws.claim(S.replay, 4 * sizeof(double), &R.replay_d));
+        if (R.stored_normals) HIP_TRY(ws.claim(S.znorm, R.znorm_elems * sizeof(double), &R.znorm_d));
         if (ws.overflow) {
             return fail(err, FOHIP_E_UNSUPPORTED, 0,
                         "request needs %llu bytes of device scratch; the allocation-free entry point holds %llu",
                         (unsigned long long)ws.needed, (unsigned long long)ws.capacity);
         }
-        char *tb = static_cast<char *>(tables_d);
-        const void *host_parts[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
-                                    gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
-                                    gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
-                                    gt.row_node.data(), bstrikes.data(), strike_group.data(),
-                                    gt.row_escrow.data(), gt.row_escrow_dr.data(), binv.data(),
-                                    items_t.data(), items_e.data(), items_b.data(), rows_sorted.data()};
-        // results come back through the staging area too, after the tables
-        struct Download {
-            void *host;
-            const void *dev;
-            size_t bytes;
-        };
-        std::vector<unsigned long long> fail_t(ngroups), fail_e(ngroups);
-        std::vector<RowError> row_err(nrows);
-        std::vector<double> eval_out(static_cast<size_t>(nrows) * 16);
-        std::vector<RowSummary> summary(nrows);
-        std::vector<Model> models_h(models_out ? model_count : 0);
-        std::vector<uint32_t> counts_h(counts_out ? count_slots : 0);
-        // tied training decisions counted with atomics by the F64 fit passes and the last fitted row;
-        // the fused pass's are in the row summary and the evaluation's in eval_out
-        std::vector<unsigned long long> ties_h(nrows);
-        const Download downloads[] = {
-            {fail_t.data(), fail_t_d, ngroups * sizeof(unsigned long long)},
-            {fail_e.data(), fail_e_d, ngroups * sizeof(unsigned long long)},
-            {row_err.data(), errors_d, nrows * sizeof(RowError)},
-            {eval_out.data(), eval_out_d, eval_out.size() * sizeof(double)},
-            {summary.data(), summary_d, nrows * sizeof(RowSummary)},
-            {models_h.data(), models_d, models_h.size() * sizeof(Model)},
-            {counts_h.data(), counts_d, counts_h.size() * sizeof(uint32_t)},
-            {ties_h.data(), ties_d, ties_h.size() * sizeof(unsigned long long)}};
-        size_t staged = tables_bytes;
-        for (const Download &d : downloads) staged += (d.bytes + 255) & ~size_t(255);
-        // one upload of every table, item list and row list of the batch
-        char *staging = nullptr;
-        HIP_TRY(ws.host_staging(staged, &staging));
-        for (size_t i = 0; i < offs.size(); ++i)
-            if (tables_bytes_list[i]) std::memcpy(staging + offs[i], host_parts[i], tables_bytes_list[i]);
-        HIP_TRY(hipMemcpyAsync(tb, staging, tables_bytes, hipMemcpyHostToDevice, st));
-        DevTables t;
-        t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
-        t.log_drift = reinterpret_cast<const double *>(tb + offs[1]);
-        t.diffusion = reinterpret_cast<const double *>(tb + offs[2]);
-        t.log_drift32 = reinterpret_cast<const float *>(tb + offs[3]);
-        t.diffusion32 = reinterpret_cast<const float *>(tb + offs[4]);
-        t.dt = reinterpret_cast<const double *>(tb + offs[5]);
-        t.sqrt_dt = reinterpret_cast<const double *>(tb + offs[6]);
-        t.cash = reinterpret_cast<const double *>(tb + offs[7]);
-        t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[8]);
-        t.disc0 = reinterpret_cast<const double *>(tb + offs[9]);
-        t.disc_step = reinterpret_cast<const double *>(tb + offs[10]);
-        t.row_time = reinterpret_cast<const double *>(tb + offs[11]);
-        t.row_node = reinterpret_cast<const uint32_t *>(tb + offs[12]);
-        t.strikes = reinterpret_cast<const double *>(tb + offs[13]);
-        t.strike_group = reinterpret_cast<const uint32_t *>(tb + offs[14]);
-        t.row_escrow = reinterpret_cast<const double *>(tb + offs[15]);
-        t.row_escrow_dr = reinterpret_cast<const double *>(tb + offs[16]);
-        t.inv_strikes = reinterpret_cast<const double *>(tb + offs[17]);
-        const uint2 *it = reinterpret_cast<const uint2 *>(tb + offs[18]);
-        const uint2 *ite = reinterpret_cast<const uint2 *>(tb + offs[19]);
-        const uint2 *itb = reinterpret_cast<const uint2 *>(tb + offs[20]);
-        const uint32_t *rows = reinterpret_cast<const uint32_t *>(tb + offs[21]);
+        // one upload of every table, item list and row list; results come back through the
+        // staging area too, after the tables
+        size_t staged = R.tables_bytes;
+        for (const auto &d : R.downloads()) staged += (d.bytes + 255) & ~size_t(255);
+        HIP_TRY(ws.host_staging(S.staging, staged, &R.staging));
+        for (int i = 0; i < BatchRun::kTableParts; ++i)
+            if (R.part_bytes[i]) std::memcpy(R.staging + R.part_off[i], R.part_host[i], R.part_bytes[i]);
+        char *tb = static_cast<char *>(R.tables_d);
+        HIP_TRY(hipMemcpyAsync(tb, R.staging, R.tables_bytes, hipMemcpyHostToDevice, gen_st));
+        DevTables &t = R.t;
+        const size_t *o = R.part_off;
+        t.groups = reinterpret_cast<const DevGroup *>(tb + o[0]);
+        t.log_drift = reinterpret_cast<const double *>(tb + o[1]);
+        t.diffusion = reinterpret_cast<const double *>(tb + o[2]);
+        t.log_drift32 = reinterpret_cast<const float *>(tb + o[3]);
+        t.diffusion32 = reinterpret_cast<const float *>(tb + o[4]);
+        t.dt = reinterpret_cast<const double *>(tb + o[5]);
+        t.sqrt_dt = reinterpret_cast<const double *>(tb + o[6]);
+        t.cash = reinterpret_cast<const double *>(tb + o[7]);
+        t.node_flags = reinterpret_cast<const uint32_t *>(tb + o[8]);
+        t.disc0 = reinterpret_cast<const double *>(tb + o[9]);
+        t.disc_step = reinterpret_cast<const double *>(tb + o[10]);
+        t.row_time = reinterpret_cast<const double *>(tb + o[11]);
+        t.row_node = reinterpret_cast<const uint32_t *>(tb + o[12]);
+        t.strikes = reinterpret_cast<const double *>(tb + o[13]);
+        t.strike_group = reinterpret_cast<const uint32_t *>(tb + o[14]);
+        t.row_escrow = reinterpret_cast<const double *>(tb + o[15]);
+        t.row_escrow_dr = reinterpret_cast<const double *>(tb + o[16]);
+        t.inv_strikes = reinterpret_cast<const double *>(tb + o[17]);
+        R.it = reinterpret_cast<const uint2 *>(tb + o[18]);
+        R.ite = reinterpret_cast<const uint2 *>(tb + o[19]);
+        R.itb = reinterpret_cast<const uint2 *>(tb + o[20]);
+        R.rows = reinterpret_cast<const uint32_t *>(tb + o[21]);
         // rules zeroed, exercise and no-support counters zeroed, failure keys set to "none"
         static_assert(sizeof(Model) % 4 == 0 && sizeof(RowError) % 4 == 0, "fills write 32-bit words");
         FillList fills = {};
-        auto add_fill = [&fills](void *ptr, size_t bytes, uint32_t value) {
-            fills.region[fills.count++] = FillRegion{static_cast<uint32_t *>(ptr), bytes / 4, value, 0};
+        auto add_fill = [&fills](void *ptr, size_t nbytes, uint32_t value) {
+            fills.region[fills.count++] = FillRegion{static_cast<uint32_t *>(ptr), nbytes / 4, value, 0};
         };
-        add_fill(models_d, model_count * sizeof(Model), 0u);
-        add_fill(counts_d, count_slots * sizeof(uint32_t), 0u);
-        add_fill(nosup_d, count_slots * sizeof(uint32_t), 0u);
-        add_fill(errors_d, nrows * sizeof(RowError), 0xFFFFFFFFu);
-        add_fill(fail_t_d, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
-        add_fill(fail_e_d, ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
-        add_fill(ties_d, ties_h.size() * sizeof(unsigned long long), 0u);
+        add_fill(R.models_d, R.model_count * sizeof(Model), 0u);
+        add_fill(R.counts_d, R.count_slots * sizeof(uint32_t), 0u);
+        add_fill(R.nosup_d, R.count_slots * sizeof(uint32_t), 0u);
+        add_fill(R.errors_d, R.nrows * sizeof(RowError), 0xFFFFFFFFu);
+        add_fill(R.fail_t_d, R.ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(R.fail_e_d, R.ngroups * sizeof(unsigned long long), 0xFFFFFFFFu);
+        add_fill(R.ties_d, R.ties_h.size() * sizeof(unsigned long long), 0u);
         // every path starts stopped at expiry: stopping byte 0
-        if (use_stops) add_fill(stops_d, (value_elems + 3) & ~uint64_t(3), 0u);
-        uint8_t *stops = use_stops ? static_cast<uint8_t *>(stops_d) : nullptr;
-        HIP_TRY(launch_fill(st, fills));
-        unsigned long long *ties_t = static_cast<unsigned long long *>(ties_d);
-        Model *models = static_cast<Model *>(models_d);
-        RowError *errors = static_cast<RowError *>(errors_d);
+        if (use_stops) add_fill(R.stops_d, (R.value_elems + 3) & ~uint64_t(3), 0u);
+        HIP_TRY(launch_fill(gen_st, fills));
+        // ---- training simulation
+        PROF_LAUNCH(gen_st, "k_simulate",
+                    launch_simulate(gen_st, f32, t, R.it, static_cast<uint32_t>(R.items_t.size()), R.paths_d, 0,
+                                    static_cast<unsigned long long *>(R.fail_t_d)));
+        HIP_TRY(hipEventRecord(S.simulated, gen_st));
+        HIP_TRY(hipStreamWaitEvent(fit_st, S.simulated, 0));
+        // ---- backward induction
+        uint8_t *stops = use_stops ? static_cast<uint8_t *>(R.stops_d) : nullptr;
+        unsigned long long *ties_t = static_cast<unsigned long long *>(R.ties_d);
+        Model *models = static_cast<Model *>(R.models_d);
+        RowError *errors = static_cast<RowError *>(R.errors_d);
+        double *part_t = static_cast<double *>(R.part_t_d);
         FitParams fp = spec.fp;
         fp.tie_window = ex->near_tie_window;
-        // ---- training simulation
-        PROF_LAUNCH("k_simulate", launch_simulate(st, f32, t, it, static_cast<uint32_t>(items_t.size()), paths_d, 0,
-                                static_cast<unsigned long long *>(fail_t_d)));
-        // ---- backward induction
-        for (uint32_t j = 0; j < max_fit; ++j) {
-            const uint32_t ni = step_items[j], nr = step_rows[j];
+        for (uint32_t j = 0; j < R.max_fit; ++j) {
+            const uint32_t ni = R.step_items[j], nr = R.step_rows[j];
             if (spec.fused) {   // one fused HBM pass per backward step
-                PROF_LAUNCH("k_fit_bulk_step", launch_fit_bulk_step(st, f32, t, fp, j, itb, step_items_b[j], max_strikes, paths_d,
-                                                                    values_d, stops, models, static_cast<double *>(part_t_d),
-                                                                    errors));
-                PROF_LAUNCH("k_fit_bulk_reduce", launch_fit_bulk_reduce(st, t, fp, j, rows, nr, static_cast<double *>(part_t_d),
-                                                                        models, errors));
+                PROF_LAUNCH(fit_st, "k_fit_bulk_step",
+                            launch_fit_bulk_step(fit_st, f32, t, fp, j, R.itb, R.step_items_b[j], R.max_strikes,
+                                                 R.paths_d, R.values_d, stops, models, part_t, errors));
+                PROF_LAUNCH(fit_st, "k_fit_bulk_reduce",
+                            launch_fit_bulk_reduce(fit_st, t, fp, j, R.rows, nr, part_t, models, errors));
                 continue;
             }
-            PROF_LAUNCH("k_fit_prepare", launch_fit_prepare(st, t, fp, j, it, ni, max_strikes, paths_d, values_d, models,
-                                       static_cast<double *>(part_t_d), errors, ties_t));
-            PROF_LAUNCH("k_fit_reduce_welford", launch_fit_reduce_welford(st, t, fp, j, rows, nr, static_cast<double *>(part_t_d), models, errors));
+            PROF_LAUNCH(fit_st, "k_fit_prepare",
+                        launch_fit_prepare(fit_st, t, fp, j, R.it, ni, R.max_strikes, R.paths_d, R.values_d, models,
+                                           part_t, errors, ties_t));
+            PROF_LAUNCH(fit_st, "k_fit_reduce_welford",
+                        launch_fit_reduce_welford(fit_st, t, fp, j, R.rows, nr, part_t, models, errors));
             for (int k = 0; k <= fp.degree; ++k) {
-                PROF_LAUNCH("k_fit_moments", launch_fit_moments(st, t, fp, j, k, it, ni, max_strikes, paths_d, values_d,
-                                           models, static_cast<double *>(part_t_d), errors));
-                PROF_LAUNCH("k_fit_reduce_moments", launch_fit_reduce_moments(st, t, fp, j, k, rows, nr, static_cast<double *>(part_t_d), models, errors));
+                PROF_LAUNCH(fit_st, "k_fit_moments",
+                            launch_fit_moments(fit_st, t, fp, j, k, R.it, ni, R.max_strikes, R.paths_d, R.values_d,
+                                               models, part_t, errors));
+                PROF_LAUNCH(fit_st, "k_fit_reduce_moments",
+                            launch_fit_reduce_moments(fit_st, t, fp, j, k, R.rows, nr, part_t, models, errors));
             }
         }
         if (spec.fused)   // rank/condition gates of every rule of the backward pass at once
-            PROF_LAUNCH("k_fit_gate", launch_fit_gate(st, t, fp, rows, nrows, models, static_cast<uint32_t>(model_count),
-                                                      errors));
-        PROF_LAUNCH("k_fit_final", launch_fit_final(st, f32, spec.fused, t, fp, it, static_cast<uint32_t>(items_t.size()), max_strikes, paths_d,
-                                 values_d, stops, models, static_cast<double *>(part_t_d), errors, ties_t));
-        PROF_LAUNCH("k_fit_reduce_origin", launch_fit_reduce_origin(st, t, fp, rows, nrows, static_cast<double *>(part_t_d), models, errors));
-        // ---- evaluation
-        bool escrow = false;   // the escrowed instantiation also serves spot-model groups
-        for (const GroupPlan &P : plans) escrow = escrow || P.escrowed;
-        // Several strike tiles read one set of evaluation normals from HBM instead of each
-        // regenerating Philox and ZIGNOR (the normals are bitwise those of the streams). Single-
-        // strike tiles always regenerate: that form has no stored-normal kernel.
+            PROF_LAUNCH(fit_st, "k_fit_gate",
+                        launch_fit_gate(fit_st, t, fp, R.rows, R.nrows, models,
+                                        static_cast<uint32_t>(R.model_count), errors));
+        PROF_LAUNCH(fit_st, "k_fit_final",
+                    launch_fit_final(fit_st, f32, spec.fused, t, fp, R.it, static_cast<uint32_t>(R.items_t.size()),
+                                     R.max_strikes, R.paths_d, R.values_d, stops, models, part_t, errors, ties_t));
+        PROF_LAUNCH(fit_st, "k_fit_reduce_origin",
+                    launch_fit_reduce_origin(fit_st, t, fp, R.rows, R.nrows, part_t, models, errors));
+        HIP_TRY(hipEventRecord(S.fitted, fit_st));
+        return FOHIP_OK;
+    };
+
+    // ---- evaluation and downloads (generation stream, after the batch's backward induction)
+    auto issue_back = [&](BatchRun &R) -> int {
+        BatchSlot &S = ex->slot[R.slot];
+        const DevTables &t = R.t;
+        FitParams fp = spec.fp;
+        fp.tie_window = ex->near_tie_window;
+        Model *models = static_cast<Model *>(R.models_d);
+        HIP_TRY(hipStreamWaitEvent(gen_st, S.fitted, 0));
         const double *znorm = nullptr;
-        if (stored_normals) {
-            PROF_LAUNCH("k_eval_normals", launch_eval_normals(st, t, ite,
-                                                               static_cast<uint32_t>(items_e.size()),
-                                                               static_cast<double *>(znorm_d),
-                                                               static_cast<unsigned long long *>(fail_e_d)));
-            znorm = static_cast<const double *>(znorm_d);
+        if (R.stored_normals) {
+            PROF_LAUNCH(gen_st, "k_eval_normals",
+                        launch_eval_normals(gen_st, t, R.ite, static_cast<uint32_t>(R.items_e.size()),
+                                            static_cast<double *>(R.znorm_d),
+                                            static_cast<unsigned long long *>(R.fail_e_d)));
+            znorm = static_cast<const double *>(R.znorm_d);
         }
-        PROF_LAUNCH("k_evaluate", launch_evaluate(st, spec.fused, greeks, escrow, tile, t, fp, ite,
-                                static_cast<uint32_t>(items_e.size()), max_strikes, max_obs, models,
-                                static_cast<double *>(part_e_d), static_cast<uint32_t *>(counts_d), static_cast<uint32_t *>(nosup_d), errors,
-                                static_cast<unsigned long long *>(fail_e_d), znorm));
-        PROF_LAUNCH("k_eval_reduce", launch_eval_reduce(st, t, rows, nrows, static_cast<double *>(part_e_d), greeks ? 1 : 0, static_cast<double *>(eval_out_d)));
-        PROF_LAUNCH("k_row_summary", launch_row_summary(st, t, models, static_cast<uint32_t *>(counts_d), static_cast<uint32_t *>(nosup_d), nrows,
-                                   static_cast<RowSummary *>(summary_d)));
-        // ---- results: DMA transfers into the staging area, copied out once the stream drains
+        // the escrowed instantiation also serves spot-model groups
+        PROF_LAUNCH(gen_st, "k_evaluate",
+                    launch_evaluate(gen_st, spec.fused, greeks, R.escrow, tile, t, fp, R.ite,
+                                    static_cast<uint32_t>(R.items_e.size()), R.max_strikes, R.max_obs, models,
+                                    static_cast<double *>(R.part_e_d), static_cast<uint32_t *>(R.counts_d),
+                                    static_cast<uint32_t *>(R.nosup_d), static_cast<RowError *>(R.errors_d),
+                                    static_cast<unsigned long long *>(R.fail_e_d), znorm));
+        PROF_LAUNCH(gen_st, "k_eval_reduce",
+                    launch_eval_reduce(gen_st, t, R.rows, R.nrows, static_cast<double *>(R.part_e_d), greeks ? 1 : 0,
+                                       static_cast<double *>(R.eval_out_d)));
+        PROF_LAUNCH(gen_st, "k_row_summary",
+                    launch_row_summary(gen_st, t, models, static_cast<uint32_t *>(R.counts_d),
+                                       static_cast<uint32_t *>(R.nosup_d), R.nrows,
+                                       static_cast<RowSummary *>(R.summary_d)));
+        // results: DMA transfers into the staging area, copied out once the batch is done
+        size_t off = R.tables_bytes;
+        for (const auto &d : R.downloads()) {
+            if (d.bytes) HIP_TRY(hipMemcpyAsync(R.staging + off, *d.dev, d.bytes, hipMemcpyDeviceToHost, gen_st));
+            off += (d.bytes + 255) & ~size_t(255);
+        }
+        HIP_TRY(hipEventRecord(S.done, gen_st));
+        return FOHIP_OK;
+    };
+
+    // ---- results of a finished batch: failures in the reference's processing order, row outputs
+    auto finish = [&](BatchRun &R) -> int {
+        BatchSlot &S = ex->slot[R.slot];
+        HIP_TRY(hipEventSynchronize(S.done));
+        HIP_TRY(hipGetLastError());
         {
-            size_t off = tables_bytes;
-            for (const Download &d : downloads) {
-                if (d.bytes) HIP_TRY(hipMemcpyAsync(staging + off, d.dev, d.bytes, hipMemcpyDeviceToHost, st));
-                off += (d.bytes + 255) & ~size_t(255);
-            }
-            HIP_TRY(hipStreamSynchronize(st));
-            HIP_TRY(hipGetLastError());
-            off = tables_bytes;
-            for (const Download &d : downloads) {
-                if (d.bytes) std::memcpy(d.host, staging + off, d.bytes);
+            size_t off = R.tables_bytes;
+            for (const auto &d : R.downloads()) {
+                if (d.bytes) std::memcpy(d.host, R.staging + off, d.bytes);
                 off += (d.bytes + 255) & ~size_t(255);
             }
         }
         if (ex->profiling) ex->collect();
-        // ---- failures, in the reference's processing order: groups in order; within a group the
-        // training simulation, then strike rows' fits in row order, then the evaluation simulation,
-        // then strike rows' evaluations.
-        for (uint32_t g = 0; g < ngroups; ++g) {
-            const DevGroup &G = gt.groups[g];
-            const GroupPlan &P = plans[g];
-            const int64_t first_row = static_cast<int64_t>(row0 + P.strike_begin);
+        const DevTables &t = R.t;
+        FitParams fp = spec.fp;
+        // Failures, in the reference's processing order: groups in order; within a group the
+        // training simulation, then strike rows' fits in row order, then the evaluation
+        // simulation, then strike rows' evaluations.
+        for (uint32_t g = 0; g < R.ngroups; ++g) {
+            const DevGroup &G = R.gt.groups[g];
+            const GroupPlan &P = R.plans[g];
+            const int64_t first_row = static_cast<int64_t>(R.row0 + P.strike_begin);
             auto sim_failure = [&](int evaluation, unsigned long long path) -> int {
+                // the replay runs alone on the fit stream once every batch in flight has drained
+                HIP_TRY(hipStreamSynchronize(gen_st));
+                HIP_TRY(hipStreamSynchronize(fit_st));
                 std::vector<double> info(4, 0.0);
-                HIP_TRY(launch_replay_path(st, t, g, evaluation, static_cast<uint32_t>(path), static_cast<double *>(replay_d)));
-                HIP_TRY(hipMemcpyAsync(info.data(), replay_d, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-                HIP_TRY(hipStreamSynchronize(st));
+                HIP_TRY(launch_replay_path(fit_st, t, g, evaluation, static_cast<uint32_t>(path),
+                                           static_cast<double *>(R.replay_d)));
+                HIP_TRY(hipMemcpyAsync(info.data(), R.replay_d, 4 * sizeof(double), hipMemcpyDeviceToHost, fit_st));
+                HIP_TRY(hipStreamSynchronize(fit_st));
                 const int phase = evaluation ? FOHIP_PHASE_EVALUATION_SIMULATION : FOHIP_PHASE_TRAINING_SIMULATION;
                 const char *role = evaluation ? "evaluation" : "training";
                 const uint32_t interval = static_cast<uint32_t>(info[0] < 0 ? 0 : info[0]);
@@ -1158,20 +1345,21 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                             "non-finite spot from state %.17g and variate %.17g",
                             role, interval, interval + 1, path, info[1], info[2]);
             };
-            if (fail_t[g] != kNoError) return sim_failure(0, fail_t[g]);
-            if (bad_group == static_cast<int64_t>(g)) {
-                set_fields(err, first_row, bad_node, -1, -1, 0, 0);
+            if (R.fail_t[g] != kNoError) return sim_failure(0, R.fail_t[g]);
+            if (R.bad_group == static_cast<int64_t>(g)) {
+                set_fields(err, first_row, R.bad_node, -1, -1, 0, 0);
                 return fail(err, FOHIP_E_UNREPRESENTABLE_DISCOUNT, FOHIP_PHASE_FIT,
-                            "discount factor to observation node %lld is not finite and positive", (long long)bad_node);
+                            "discount factor to observation node %lld is not finite and positive",
+                            (long long)R.bad_node);
             }
             for (uint32_t s = P.strike_begin; s < P.strike_end; ++s) {
-                const RowError &e = row_err[s];
+                const RowError &e = R.row_err[s];
                 if (e.key == kNoError) continue;
                 const uint32_t step = static_cast<uint32_t>(e.key >> 44);
                 if (step >= kEvalStepBase) continue;
                 const uint32_t stage = static_cast<uint32_t>((e.key >> 36) & 0xFF);
                 const uint64_t path = e.key & ((1ull << 36) - 1);
-                const int64_t row = static_cast<int64_t>(row0 + s);
+                const int64_t row = static_cast<int64_t>(R.row0 + s);
                 uint32_t node = 0;
                 if (step < G.fit_rows) node = P.obs_nodes[G.obs - 2 - step];
                 if (stage == kStageRegression) {
@@ -1194,11 +1382,11 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                 return fail(err, FOHIP_E_NON_FINITE_DISCOUNTED_PAYOFF, FOHIP_PHASE_FIT,
                             "LSM evaluation produced non-finite discounted payoff at path %llu", (unsigned long long)path);
             }
-            if (fail_e[g] != kNoError) return sim_failure(1, fail_e[g]);
+            if (R.fail_e[g] != kNoError) return sim_failure(1, R.fail_e[g]);
             for (uint32_t s = P.strike_begin; s < P.strike_end; ++s) {
-                const RowError &e = row_err[s];
-                const RowSummary &rs = summary[s];
-                const int64_t row = static_cast<int64_t>(row0 + s);
+                const RowError &e = R.row_err[s];
+                const RowSummary &rs = R.summary[s];
+                const int64_t row = static_cast<int64_t>(R.row0 + s);
                 int64_t err_row_obs = -1;
                 uint32_t stage = 0;
                 uint64_t path = 0;
@@ -1229,25 +1417,25 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             }
         }
         // ---- row outputs
-        for (uint32_t s = 0; s < nrows; ++s) {
-            const uint32_t g = strike_group[s];
-            const GroupPlan &P = plans[g];
-            RowOut &o = out[row0 + s];
-            o.sum = summary[s];
-            o.ties_training = ties_h[s] + static_cast<uint64_t>(summary[s].ties_training);
-            o.ties_evaluation = static_cast<uint64_t>(eval_out[static_cast<size_t>(s) * 16 + 15]);
+        for (uint32_t s = 0; s < R.nrows; ++s) {
+            const uint32_t g = R.strike_group[s];
+            const GroupPlan &P = R.plans[g];
+            RowOut &o = out[R.row0 + s];
+            o.sum = R.summary[s];
+            o.ties_training = R.ties_h[s] + static_cast<uint64_t>(R.summary[s].ties_training);
+            o.ties_evaluation = static_cast<uint64_t>(R.eval_out[static_cast<size_t>(s) * 16 + 15]);
             const double n = static_cast<double>(P.ne);
-            if (summary[s].origin_fitted && summary[s].origin_exercised) {
+            if (R.summary[s].origin_fitted && R.summary[s].origin_exercised) {
                 // policy.rs:672-682: deterministic origin exercise
-                o.price = std::max(P.S0 - bstrikes[s], 0.0);
+                o.price = std::max(P.S0 - R.bstrikes[s], 0.0);
                 o.sd = o.se = 0.0;
                 o.has_sd = true;
                 o.sum.early = static_cast<uint32_t>(P.ne);
                 o.sum.expiry = 0;
-                o.greek[0] = P.S0 > bstrikes[s] ? 1.0 : 0.0;
+                o.greek[0] = P.S0 > R.bstrikes[s] ? 1.0 : 0.0;
                 continue;
             }
-            const double *e = &eval_out[static_cast<size_t>(s) * 16];
+            const double *e = &R.eval_out[static_cast<size_t>(s) * 16];
             o.price = e[0];
             if (P.ne > 1) {
                 o.sd = std::sqrt(e[1] / (n - 1.0));
@@ -1259,13 +1447,54 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                 o.greek_se[a] = e[7 + a];
             }
         }
-        if (models_out) {
-            // append with strike rows in batch order: models are [strike][obs] per group
-            models_out->insert(models_out->end(), models_h.begin(), models_h.end());
-        }
-        if (counts_out) counts_out->insert(counts_out->end(), counts_h.begin(), counts_h.end());
+        // append with strike rows in batch order: models are [strike][obs] per group
+        if (models_out) models_out->insert(models_out->end(), R.models_h.begin(), R.models_h.end());
+        if (counts_out) counts_out->insert(counts_out->end(), R.counts_h.begin(), R.counts_h.end());
         // keep the last batch's layout for diagnostics
-        ex->last_models = models_h;
+        ex->last_models = R.models_h;
+        return FOHIP_OK;
+    };
+
+    int rc = FOHIP_OK;
+    if (!overlapped) {
+        // the host builds batch b + 1's tables while the device runs batch b
+        std::unique_ptr<BatchRun> cur = prepare(0);
+        for (size_t b = 0; b < batches.size(); ++b) {
+            if ((rc = issue_front(*cur)) || (rc = issue_back(*cur))) return rc;
+            std::unique_ptr<BatchRun> next = b + 1 < batches.size() ? prepare(b + 1) : nullptr;
+            if ((rc = finish(*cur))) return rc;
+            cur = std::move(next);
+        }
+    } else {
+        std::unique_ptr<BatchRun> prev, cur = prepare(0);
+        if ((rc = issue_front(*cur))) return rc;
+        for (size_t b = 0; b < batches.size(); ++b) {
+            std::unique_ptr<BatchRun> next;
+            int rc_next = FOHIP_OK;
+            if (b + 1 < batches.size()) {
+                next = prepare(b + 1);   // host work while the device runs batches b - 1 and b
+                if (prev) {              // b - 1 holds the slot b + 1 is about to use
+                    if ((rc = finish(*prev))) return rc;
+                    prev.reset();
+                }
+                rc_next = issue_front(*next);
+            }
+            // a failure to queue b + 1 is reported after batch b's own results, which precede it
+            if ((rc = issue_back(*cur))) return rc;
+            if (rc_next) {
+                if (prev && (rc = finish(*prev))) return rc;
+                if ((rc = finish(*cur))) return rc;
+                return rc_next;
+            }
+            if (prev && (rc = finish(*prev))) return rc;
+            prev = std::move(cur);
+            cur = std::move(next);
+        }
+        if (prev && (rc = finish(*prev))) return rc;
+    }
+    if (ex->profiling) {
+        HIP_TRY(hipStreamSynchronize(fit_st));
+        ex->collect();
     }
     return FOHIP_OK;
 }
@@ -2084,14 +2313,14 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
         offs[i] = total;
         total += (tb_bytes[i] + 255) & ~size_t(255);
     }
-    HIP_TRY(ex->tables.ensure(total));
-    if (!device_out) HIP_TRY(ex->paths.ensure(path_elems * real));
-    void *dst = device_out ? device_out : ex->paths.ptr;
-    HIP_TRY(ex->fail_t.ensure(8));
+    HIP_TRY(ex->slot[0].tables.ensure(total));
+    if (!device_out) HIP_TRY(ex->slot[0].paths.ensure(path_elems * real));
+    void *dst = device_out ? device_out : ex->slot[0].paths.ptr;
+    HIP_TRY(ex->slot[0].fail_t.ensure(8));
     std::vector<uint2> items;
     for (uint32_t c = 0; c < gt.groups[0].chunks_t; ++c) items.push_back(make_uint2(0, c));
     HIP_TRY(ex->items_t.ensure(items.size() * sizeof(uint2)));
-    char *tb = ex->tables.as<char>();
+    char *tb = ex->slot[0].tables.as<char>();
     hipStream_t st = ex->stream;
     if (after && after != st) {
         hipEvent_t ready;
@@ -2103,7 +2332,7 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
     }
     for (int i = 0; i < 7; ++i) HIP_TRY(hipMemcpyAsync(tb + offs[i], tb_src[i], tb_bytes[i], hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(ex->fail_t.ptr, 0xFF, 8, st));
+    HIP_TRY(hipMemsetAsync(ex->slot[0].fail_t.ptr, 0xFF, 8, st));
     DevTables t;
     std::memset(&t, 0, sizeof t);
     t.groups = reinterpret_cast<const DevGroup *>(tb + offs[0]);
@@ -2114,9 +2343,9 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
     t.cash = reinterpret_cast<const double *>(tb + offs[5]);
     t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[6]);
     HIP_TRY(launch_simulate(st, f32, t, ex->items_t.as<uint2>(), static_cast<uint32_t>(items.size()), dst, 0,
-                            ex->fail_t.as<unsigned long long>()));
+                            ex->slot[0].fail_t.as<unsigned long long>()));
     unsigned long long failed = 0;
-    HIP_TRY(hipMemcpyAsync(&failed, ex->fail_t.ptr, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&failed, ex->slot[0].fail_t.ptr, 8, hipMemcpyDeviceToHost, st));
     if (device_out) {
         HIP_TRY(hipStreamSynchronize(st));
     } else if (f32) {

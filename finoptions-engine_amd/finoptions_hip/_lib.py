@@ -51,7 +51,8 @@ class FohipKernelStat(C.Structure):
 
 
 class FohipTuning(C.Structure):
-    _fields_ = [("eval_strike_tile", C.c_uint32), ("fit_cashflows", C.c_uint32)]
+    _fields_ = [("eval_strike_tile", C.c_uint32), ("fit_cashflows", C.c_uint32),
+                ("batch_schedule", C.c_uint32), ("batch_mib", C.c_uint32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -183,7 +184,8 @@ EXPORTED = (
     "fohip_stream_normals", "fohip_validate_time_grid", "fohip_validate_log_discount_curve",
     "fohip_validate_cumulative_yield_curve", "fohip_validate_event_schedule",
     "fohip_validate_constant_volatility", "fohip_validate_cash_dividend_amount",
-    "fohip_validate_ragged_offsets", "fohip_set_profiling", "fohip_get_profile",
+    "fohip_validate_ragged_offsets", "fohip_validate_ragged_exercise_times",
+    "fohip_validate_ragged_cash_dividends", "fohip_set_profiling", "fohip_get_profile",
     "fohip_reset_profile", "fohip_price_lsm_call_noalloc",
 )
 
@@ -257,6 +259,8 @@ def lib() -> C.CDLL:
     handle.fohip_validate_constant_volatility.argtypes = [C.c_double, E]
     handle.fohip_validate_cash_dividend_amount.argtypes = [C.c_double, E]
     handle.fohip_validate_ragged_offsets.argtypes = [_i64p, C.c_size_t, C.c_size_t, E]
+    handle.fohip_validate_ragged_exercise_times.argtypes = [_i64p, C.c_size_t, _dp, E]
+    handle.fohip_validate_ragged_cash_dividends.argtypes = [_i64p, C.c_size_t, _dp, _dp, E]
     handle.fohip_device_count.argtypes = [C.POINTER(C.c_int32)]
     handle.fohip_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     handle.fohip_get_profile.argtypes = [C.c_void_p, C.POINTER(FohipKernelStat), C.c_size_t,

@@ -101,13 +101,9 @@ class RaggedExerciseTimes:
         code = lib().fohip_validate_ragged_offsets(ptr(self.offsets, C.c_int64), self.offsets.size,
                                                    self.times.size, C.byref(err))
         raise_for(code, err)
-        base = int(self.offsets[0])
-        for row in range(self.rows):
-            t = self.times[int(self.offsets[row]) - base:int(self.offsets[row + 1]) - base]
-            if t.size and (not np.all(np.isfinite(t)) or np.any(t <= 0.0)):
-                raise InputError(f"row {row} exercise times must be finite and positive")
-            if t.size > 1 and np.any(np.diff(t) <= 0.0):
-                raise InputError(f"row {row} exercise times must be strictly increasing")
+        code = lib().fohip_validate_ragged_exercise_times(
+            ptr(self.offsets, C.c_int64), self.offsets.size, ptr(self.times, C.c_double), C.byref(err))
+        raise_for(code, err)
 
     @property
     def rows(self) -> int:

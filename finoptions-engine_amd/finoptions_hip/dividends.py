@@ -63,18 +63,10 @@ class RaggedCashDividends:
         code = lib().fohip_validate_ragged_offsets(ptr(self.offsets, C.c_int64), self.offsets.size,
                                                    self.times.size, C.byref(err))
         raise_for(code, err)
-        base = int(self.offsets[0])
-        for row in range(self.rows):
-            a, b = int(self.offsets[row]) - base, int(self.offsets[row + 1]) - base
-            t = self.times[a:b]
-            if t.size and (not np.all(np.isfinite(t)) or np.any(t <= 0.0)):
-                raise InputError(f"row {row} cash-dividend times must be finite and positive")
-            if t.size > 1 and np.any(np.diff(t) < 0.0):
-                raise InputError(f"row {row} cash-dividend times must be nondecreasing")
-            amt = self.amounts[a:b]
-            if amt.size and (not np.all(np.isfinite(amt)) or np.any(amt < 0.0)
-                             or np.any((amt == 0.0) & np.signbit(amt))):
-                raise InputError(f"row {row} cash-dividend amounts must be finite and nonnegative")
+        code = lib().fohip_validate_ragged_cash_dividends(
+            ptr(self.offsets, C.c_int64), self.offsets.size, ptr(self.times, C.c_double),
+            ptr(self.amounts, C.c_double), C.byref(err))
+        raise_for(code, err)
 
     @property
     def rows(self) -> int:

@@ -88,7 +88,7 @@ class HipExecutor:
     device: int
     precision: HipPrecision = HipPrecision.F64
     _handle: _Handle = field(init=False, repr=False, compare=False)
-    _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto"))
+    _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto", "serial", 0))
 
     def __post_init__(self) -> None:
         if type(self.device) is not int or self.device < 0:
@@ -119,26 +119,39 @@ class HipExecutor:
         return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms))
                 for i in range(n.value)}
 
-    def tune(self, *, eval_strike_tile: int = 0, fit_cashflows: str = "auto") -> None:
+    def tune(self, *, eval_strike_tile: int = 0, fit_cashflows: str = "auto",
+             batch_schedule: str = "serial", batch_mib: int = 0) -> None:
         """Set launch parameters; results are bitwise invariant to them.
 
         ``eval_strike_tile``: strikes per evaluation launch column (0: by shape). ``fit_cashflows``
         (fused methods): "value_columns" keeps each strike's cashflow per path in HBM,
         "stopping_rows" keeps one byte per strike and path and rebuilds the cashflow from the
         group's spot matrix (8x less cashflow memory; slower on the shapes measured), "auto"
-        keeps value columns.
+        keeps value columns. ``batch_schedule``: "serial" runs the memory-bounded batches one
+        after another; "overlapped" runs one batch's backward induction beside the next batch's
+        simulation and the previous batch's evaluation on a second stream, in two slots of half
+        the serial batch memory. ``batch_mib``: paths and cashflows per batch slot in MiB (0: a
+        quarter of device memory, an eighth per slot when overlapped).
         """
-        codes = {"auto": 0, "value_columns": 1, "stopping_rows": 2}
-        if fit_cashflows not in codes:
-            raise InputError(f"fit_cashflows must be one of {sorted(codes)}; observed {fit_cashflows!r}")
-        t = FohipTuning(int(eval_strike_tile), codes[fit_cashflows])
+        cash_codes = {"auto": 0, "value_columns": 1, "stopping_rows": 2}
+        schedule_codes = {"serial": 0, "overlapped": 1}
+        if fit_cashflows not in cash_codes:
+            raise InputError(f"fit_cashflows must be one of {sorted(cash_codes)}; observed {fit_cashflows!r}")
+        if batch_schedule not in schedule_codes:
+            raise InputError(
+                f"batch_schedule must be one of {sorted(schedule_codes)}; observed {batch_schedule!r}")
+        if type(batch_mib) is not int or not 0 <= batch_mib < 2**32:
+            raise InputError(f"batch_mib must be a plain int in [0, 2^32); observed {batch_mib!r}")
+        t = FohipTuning(int(eval_strike_tile), cash_codes[fit_cashflows], schedule_codes[batch_schedule],
+                        batch_mib)
         if lib().fohip_set_tuning(self._handle.ptr, C.byref(t)) != 0:
             raise InputError(f"unsupported tuning eval_strike_tile={eval_strike_tile}")
-        object.__setattr__(self, "_tuning", (int(eval_strike_tile), fit_cashflows))
+        object.__setattr__(self, "_tuning", (int(eval_strike_tile), fit_cashflows, batch_schedule, batch_mib))
 
     @property
-    def tuning(self) -> tuple[int, str]:
-        """The launch parameters last set with :meth:`tune` (tile, cashflow encoding)."""
+    def tuning(self) -> tuple[int, str, str, int]:
+        """The launch parameters last set with :meth:`tune` (tile, cashflow encoding, batch
+        schedule, slot MiB)."""
         return self._tuning
 
     def set_near_tie_window(self, window: float = 0.0) -> None:

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define FOHIP_ABI_VERSION 3
+#define FOHIP_ABI_VERSION 4
 
 /* ---- error codes (the numbering is shared with the CPU oracle) ---- */
 enum {
@@ -146,6 +146,7 @@ int fohip_abi_version(void);
 /* Launch parameters an autotuner may change. Results are bitwise invariant to them: reduction
  * trees are fixed by the path chunks, not by these values. Zero fields keep defaults. */
 enum { FOHIP_CASHFLOWS_AUTO = 0, FOHIP_CASHFLOWS_VALUE_COLUMNS = 1, FOHIP_CASHFLOWS_STOPPING_ROWS = 2 };
+enum { FOHIP_BATCHES_SERIAL = 0, FOHIP_BATCHES_OVERLAPPED = 1 };
 typedef struct {
     uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16; Greeks 1, 4, 6, 8) */
     /* Fused methods: how the backward induction keeps each (strike, path) cashflow. Value
@@ -154,6 +155,18 @@ typedef struct {
      * <= 255): 8x less cashflow memory and traffic, bitwise the same FUSED_F64 results, but
      * slower on the shapes measured. Auto keeps value columns. */
     uint32_t fit_cashflows;
+    /* How the memory-bounded batches of one request share the device. Serial (the default)
+     * runs them one after another, each within a quarter of device memory. Overlapped keeps two
+     * batches in flight in two slots of an eighth each: one batch's backward induction, bound by
+     * HBM bandwidth, runs on the executor's stream while the next batch's simulation and the
+     * previous batch's evaluation, bound by vector ALU issue, run on a second stream. On the
+     * measured shapes the overlap gains less than the halved batches cost: the generation
+     * kernels fill every register slot, so the fit's blocks rarely run beside them. */
+    uint32_t batch_schedule;
+    /* Bytes of paths and cashflows per batch slot, in MiB (0: a quarter of device memory under
+     * the serial schedule, an eighth per slot when overlapped). Smaller slots mean more, shorter
+     * batches; results do not depend on the batch split. */
+    uint32_t batch_mib;
 } fohip_tuning;
 int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
 
@@ -400,6 +413,17 @@ int fohip_validate_constant_volatility(double volatility, fohip_error *err);
 int fohip_validate_cash_dividend_amount(double amount, fohip_error *err);
 int fohip_validate_ragged_offsets(const int64_t *offsets, size_t offsets_len, size_t values_len,
                                   fohip_error *err);
+/* Per-row checks of a ragged exercise schedule (times[offsets[r] - offsets[0] ..]): every time
+ * finite and positive, then strictly increasing; the first failing row is reported, as
+ * RaggedExerciseTimes in python/finoptions/batch_lsm.py does. The offsets must have passed
+ * fohip_validate_ragged_offsets. */
+int fohip_validate_ragged_exercise_times(const int64_t *offsets, size_t offsets_len,
+                                         const double *times, fohip_error *err);
+/* Per-row checks of ragged cash dividends (python/finoptions/dividends.py): times finite and
+ * positive, then nondecreasing, then amounts finite, nonnegative and not -0.0. */
+int fohip_validate_ragged_cash_dividends(const int64_t *offsets, size_t offsets_len,
+                                         const double *times, const double *amounts,
+                                         fohip_error *err);
 
 #ifdef __cplusplus
 }

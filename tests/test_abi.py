@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(L.EXPORTED)
     for name in sorted(declared):
         assert hasattr(handle, name), name
-    assert handle.fohip_abi_version() == 3
+    assert handle.fohip_abi_version() == 4
 
 
 def test_library_has_gfx950_code_object():
@@ -153,3 +153,45 @@ def test_ctypes_layouts_match_the_header(tmp_path):
         assert C.sizeof(cls) == c_layout[(cname, "size")], cname
         for field, *_ in cls._fields_:
             assert getattr(cls, field).offset == c_layout[(cname, field)], (cname, field)
+
+
+@pytest.mark.parametrize("base", [0, 5])
+def test_ragged_schedules_report_the_first_failing_row_and_check(base):
+    """Per-row checks of RaggedExerciseTimes and RaggedCashDividends (python/finoptions/
+    batch_lsm.py, dividends.py): within a row the time checks come before the amount checks, and
+    the first failing row in row order is named; sliced offsets address the same rows."""
+    import finoptions_hip as fh
+
+    off = np.array([0, 2, 2, 4, 6], dtype=np.int64) + base
+
+    def ex(times):
+        return fh.RaggedExerciseTimes(offsets=off, times=np.array(times))
+
+    def cash(times, amounts):
+        return fh.RaggedCashDividends(offsets=off, times=np.array(times), amounts=np.array(amounts))
+
+    ex([0.1, 0.2, 0.3, 0.4, 0.5, 0.6])
+    # offsets [0, 2, 2, 4, 6]: row 1 is empty, row 2 holds times 2..3 and row 3 times 4..5
+    with pytest.raises(fh.InputError, match=r"^row 3 exercise times must be finite and positive$"):
+        ex([0.1, 0.2, 0.3, 0.4, -1.0, 0.6])
+    with pytest.raises(fh.InputError, match=r"^row 2 exercise times must be strictly increasing$"):
+        ex([0.1, 0.2, 0.3, 0.3, 0.5, 0.6])
+    # row 2 is increasing but row 3 holds a NaN: the earlier row's check passes first
+    with pytest.raises(fh.InputError, match=r"^row 3 exercise times must be finite and positive$"):
+        ex([0.1, 0.2, 0.3, 0.4, 0.5, float("nan")])
+    # a row failing both checks reports the positivity check
+    with pytest.raises(fh.InputError, match=r"^row 0 exercise times must be finite and positive$"):
+        ex([0.2, -0.1, 0.3, 0.4, 0.5, 0.6])
+
+    cash([0.1, 0.1, 0.3, 0.4, 0.5, 0.6], [1.0, 0.0, 1.0, 1.0, 1.0, 1.0])   # ties and zero are fine
+    with pytest.raises(fh.InputError, match=r"^row 2 cash-dividend times must be nondecreasing$"):
+        cash([0.1, 0.2, 0.4, 0.3, 0.5, 0.6], [1.0] * 6)
+    with pytest.raises(fh.InputError, match=r"^row 0 cash-dividend times must be finite and positive$"):
+        cash([0.0, 0.2, 0.3, 0.4, 0.5, 0.6], [1.0] * 6)
+    with pytest.raises(fh.InputError, match=r"^row 3 cash-dividend amounts must be finite and nonnegative$"):
+        cash([0.1, 0.2, 0.3, 0.4, 0.5, 0.6], [1.0, 1.0, 1.0, 1.0, 1.0, -0.0])
+    with pytest.raises(fh.InputError, match=r"^row 2 cash-dividend amounts must be finite and nonnegative$"):
+        cash([0.1, 0.2, 0.3, 0.4, 0.5, 0.6], [1.0, 1.0, float("inf"), 1.0, 1.0, 1.0])
+    # the time check of a row precedes its amount check
+    with pytest.raises(fh.InputError, match=r"^row 2 cash-dividend times must be nondecreasing$"):
+        cash([0.1, 0.2, 0.4, 0.3, 0.5, 0.6], [1.0, 1.0, float("nan"), 1.0, 1.0, 1.0])
