@@ -58,6 +58,7 @@ def build(jobs: int = 2) -> Path:
     if not Path(HIPCC).exists() and shutil.which(HIPCC) is None:
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     subprocess.run([sys.executable, str(CSRC / "gen_zig_tables.py")], check=True)
+    subprocess.run([sys.executable, str(CSRC / "gen_exp_table.py")], check=True)
     BUILD.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
         objs = list(pool.map(_compile, SOURCES))

@@ -2,13 +2,15 @@
 //
 // Random stream, normal transform and GBM transition restate the reference kernels bit for bit
 // where the arithmetic allows it (integer Philox, uniform bit maps, table lookups, FP64 products);
-// transcendental calls use the device libm (exp/log), whose results can differ from glibc by an
-// ulp -- parity tests count those differences instead of assuming them away.
+// the exponentials of the GBM step and of the ZIGNOR wedge test are a table-driven binary64 exp
+// and the ZIGNOR tail calls the device libm's log; either can differ from glibc by an ulp --
+// parity tests count those differences instead of assuming them away.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "exp_table.inc"
 #include "fohip_layout.hpp"
 
 namespace fohip {
@@ -139,6 +141,35 @@ struct RingStream {
     }
 };
 
+// ----------------------------------------------------------------- exponential
+
+// Binary64 exponential of the GBM step and the ZIGNOR wedge test: Tang's table-driven method (P. T. P. Tang, ACM TOMS 15(2),
+// 1989) with 32 entries. x = (32 m + j) ln2/32 + r with n = 32 m + j = rint(32 x / ln2) and r by a
+// two-part Cody-Waite reduction (|r| <= ln2/64 + tiny); exp(r) - 1 by its Taylor polynomial of
+// degree 6 (truncation below 4e-18 relative); 2^(j/32) as a leading + trailing binary64 pair;
+// exp(x) = 2^m (T_j + T_j p(r) + t_j) with one final rounding. Within one ulp of the correctly
+// rounded value (glibc's exp agrees on 99.5 % of arguments in [-5, 5]). The reduction is exact for
+// |n| < 2^20, which covers |x| <= 745: there the final scaling overflows to +inf above ln(DBL_MAX)
+// and rounds into the subnormals below -708.4 (one extra rounding); beyond 745 the result is +inf
+// or +0, and NaN stays NaN. No libm call remains, so the loop keeps no libm constants in registers.
+struct ExpEntry {
+    double hi, lo;
+};
+__device__ __forceinline__ double table_exp(double x, const ExpEntry *__restrict__ tbl) {
+    if (!(fabs(x) <= 745.0)) return x > 0.0 ? __builtin_huge_val() : (x < 0.0 ? 0.0 : x);
+    const double n = rint(x * FOHIP_EXP_INV_L);
+    double r = fma(-n, FOHIP_EXP_L1, x);
+    r = fma(-n, FOHIP_EXP_L2, r);
+    const int ni = static_cast<int>(n);
+    const ExpEntry e = tbl[ni & 31];
+    double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    q = fma(r, q, 1.0 / 24.0);
+    q = fma(r, q, 1.0 / 6.0);
+    q = fma(r, q, 0.5);
+    const double p = fma(r * r, q, r);
+    return ldexp(e.hi + fma(e.hi, p, e.lo), ni >> 5);
+}
+
 // ------------------------------------------------------------ ZIGNOR normal (normal.rs:33-109)
 
 constexpr uint64_t kFractionMask = (uint64_t(1) << 52) - 1;
@@ -164,7 +195,7 @@ template <typename Stream>
 __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double unit, double value,
                                              const double *__restrict__ X,
                                              const double *__restrict__ F, double zr, double &out,
-                                             uint32_t &draws) {
+                                             uint32_t &draws, const ExpEntry *__restrict__ et) {
     for (int it = 0; it < kMaxRejectionIterations; ++it) {
         if (index == 0) {
             for (int tail = 0; tail < kMaxRejectionIterations; ++tail) {
@@ -180,7 +211,7 @@ __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double uni
         }
         const double ordinate = F[index + 1] + (F[index] - F[index + 1]) * standard_unit(s.next_u64());
         draws += 1;
-        if (ordinate < exp(-0.5 * value * value)) {
+        if (ordinate < table_exp(-0.5 * value * value, et)) {
             out = value;
             return true;
         }
@@ -200,7 +231,7 @@ __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double uni
 template <typename Stream>
 __device__ __forceinline__ bool zignor_normal(Stream &s, const double *__restrict__ X,
                                               const double *__restrict__ F, double zr, double &out,
-                                              uint32_t &draws) {
+                                              uint32_t &draws, const ExpEntry *__restrict__ et) {
     const uint64_t bits = s.next_u64();
     draws += 1;
     const int index = static_cast<int>(bits & 0xff);
@@ -210,15 +241,15 @@ __device__ __forceinline__ bool zignor_normal(Stream &s, const double *__restric
         out = value;
         return true;
     }
-    return zignor_rejected(s, index, unit, value, X, F, zr, out, draws);
+    return zignor_rejected(s, index, unit, value, X, F, zr, out, draws, et);
 }
 
 // ------------------------------------------------------- GBM transition and cash events
 
 // step.rs:89-95: zero is absorbing; S' = S * exp(fma(diffusion, Z, drift)).
 __device__ __forceinline__ double gbm_advance(double spot, double diffusion, double drift, double z,
-                                              double &growth) {
-    growth = exp(fma(diffusion, z, drift));
+                                              double &growth, const ExpEntry *__restrict__ exp_table) {
+    growth = table_exp(fma(diffusion, z, drift), exp_table);
     return spot == 0.0 ? 0.0 : spot * growth;
 }
 __device__ __forceinline__ float gbm_advance(float spot, float diffusion, float drift, float z,

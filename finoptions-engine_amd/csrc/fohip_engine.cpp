@@ -2401,6 +2401,22 @@ extern "C" int fohip_philox_blocks(fohip_executor *ex, uint64_t seed, uint32_t d
     return FOHIP_OK;
 }
 
+extern "C" int fohip_gbm_exp_probe(fohip_executor *ex, const double *x, size_t n, double *out, fohip_error *err) {
+    clear_error(err);
+    if (!ex || (n && (!x || !out))) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "null argument");
+    if (n > 0xFFFFFFFFu) return fail(err, FOHIP_E_INVALID_ARGUMENT, 0, "at most 2^32 - 1 arguments per call");
+    if (n == 0) return FOHIP_OK;
+    std::lock_guard<std::mutex> lock(ex->mu);
+    HIP_TRY(hipSetDevice(ex->device));
+    HIP_TRY(ex->probe_a.ensure(n * 8));
+    HIP_TRY(ex->probe_b.ensure(n * 8));
+    HIP_TRY(hipMemcpyAsync(ex->probe_a.ptr, x, n * 8, hipMemcpyHostToDevice, ex->stream));
+    HIP_TRY(launch_gbm_exp(ex->stream, ex->probe_a.as<double>(), ex->probe_b.as<double>(), static_cast<uint32_t>(n)));
+    HIP_TRY(hipMemcpyAsync(out, ex->probe_b.ptr, n * 8, hipMemcpyDeviceToHost, ex->stream));
+    HIP_TRY(hipStreamSynchronize(ex->stream));
+    return FOHIP_OK;
+}
+
 extern "C" int fohip_stream_normals(fohip_executor *ex, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
                                     uint32_t count, double *out, uint32_t *draws, fohip_error *err) {
     clear_error(err);

@@ -23,6 +23,7 @@ namespace fohip {
 #include "zig_tables.inc"
 
 __device__ const double g_zig_x[257] = FOHIP_ZIG_X_INIT;
+__device__ const ExpEntry g_exp_table[32] = FOHIP_EXP_TABLE_INIT;
 __device__ const double g_zig_f[257] = FOHIP_ZIG_F_INIT;
 
 __device__ __forceinline__ void load_zig(double *sx, double *sf) {
@@ -31,6 +32,12 @@ __device__ __forceinline__ void load_zig(double *sx, double *sf) {
         sf[i] = g_zig_f[i];
     }
     __syncthreads();
+}
+
+// the ZIGNOR tables and the exponential's table of a kernel that steps paths
+__device__ __forceinline__ void load_zig_exp(double *sx, double *sf, ExpEntry *se) {
+    for (int i = threadIdx.x; i < 32; i += blockDim.x) se[i] = g_exp_table[i];
+    load_zig(sx, sf);
 }
 
 // Path-level failures only lower the row's key; the failure kind is a function of the key's stage
@@ -73,8 +80,9 @@ __global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, 
 __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths, uint32_t count,
                                  double *out, uint32_t *draws, int *fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig(sx, sf);
+    load_zig_exp(sx, sf, se);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= paths) return;
     RingStream s;
@@ -83,7 +91,7 @@ __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0,
         double z = 0.0;
         uint32_t d = 0;
         s.top_up(true);
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, d)) atomicExch(fail, 1);
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, d, se)) atomicExch(fail, 1);
         out[static_cast<uint64_t>(p) * count + i] = z;
         draws[static_cast<uint64_t>(p) * count + i] = d;
     }
@@ -101,8 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
                                                      Store *__restrict__ paths_out, int evaluation,
                                                      unsigned long long *__restrict__ fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig(sx, sf);
+    load_zig_exp(sx, sf, se);
     const uint2 item = items[blockIdx.x];
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = evaluation ? g.ne : g.nt;
@@ -123,12 +132,12 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
         double z;
         uint32_t draws = 0;
         s.top_up(true);
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) {
             ok = false;
             break;
         }
         double growth;
-        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth);
+        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth, se);
         if (!isfinite(next)) {
             ok = false;
             break;
@@ -1395,6 +1404,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ __align__(16) uint32_t hist[];   // [TS][obs] when use_lds_hist, then DEC's tables
     __shared__ double sx[257], sf[257];
+    __shared__ ExpEntry se[32];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
     __shared__ uint32_t tie_lds[TS];   // tied decisions of this tile, written with its partials
@@ -1415,6 +1425,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     const Model *tile_models = models + g->model_base + local0 * obs;   // [strike][row]
     const int degree = fp.degree, tie = fp.tie;
     if (!STORED) load_zig(sx, sf);
+    for (int i = threadIdx.x; i < 32; i += kBlock) se[i] = g_exp_table[i];
     if (threadIdx.x < TS) tie_lds[threadIdx.x] = 0;
     __syncthreads();
     if (use_lds_hist) {
@@ -1489,11 +1500,11 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
         } else {
             uint32_t draws = 0;
             s.top_up(ok);
-            if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+            if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) ok = false;
         }
         const uint32_t si = step_base + iv;
         double growth;
-        const double next = gbm_advance(spot, *uniform(t.diffusion + si), *uniform(t.log_drift + si), z, growth);
+        const double next = gbm_advance(spot, *uniform(t.diffusion + si), *uniform(t.log_drift + si), z, growth, se);
         if (!isfinite(next)) ok = false;
         if (!ok) active = 0u;
         const double before = spot;
@@ -1843,8 +1854,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
                                                          double *__restrict__ znorm,
                                                          unsigned long long *__restrict__ sim_fail) {
     __shared__ double sx[257], sf[257];
+    __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig(sx, sf);
+    load_zig_exp(sx, sf, se);
     const uint2 item = load_item(items, blockIdx.x);
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = g.ne, p = item.y * kEvalChunk + threadIdx.x;
@@ -1857,7 +1869,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
         double z = 0.0;
         uint32_t draws = 0;
         s.top_up(ok);
-        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) ok = false;
+        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) ok = false;
         out[static_cast<uint64_t>(iv) * n] = ok ? z : __builtin_nan("");
     }
     if (!ok) atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
@@ -1867,7 +1879,8 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
 // report a deterministic error, seeded.rs:155-186): out = {interval, spot, normal, kind}.
 __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint32_t path, double *out) {
     __shared__ double sx[257], sf[257];
-    load_zig(sx, sf);
+    __shared__ ExpEntry se[32];
+    load_zig_exp(sx, sf, se);
     if (threadIdx.x != 0) return;
     const DevGroup &g = t.groups[group];
     PathStream s;
@@ -1877,7 +1890,7 @@ __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint3
     for (uint32_t i = 0; i < g.intervals; ++i) {
         double z;
         uint32_t draws = 0;
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws)) {
+        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) {
             out[0] = i;
             out[1] = spot;
             out[2] = 0.0;
@@ -1885,7 +1898,7 @@ __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint3
             return;
         }
         double growth;
-        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth);
+        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth, se);
         if (!isfinite(next)) {
             out[0] = i;
             out[1] = spot;
@@ -1906,6 +1919,21 @@ inline uint32_t xcd_blocks(uint32_t nitems, uint32_t columns) {
     return ((nitems + 7u) / 8u) * 8u * columns;
 }
 }  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_gbm_exp(const double *__restrict__ x, double *__restrict__ out,
+                                                    uint32_t n) {
+    __shared__ ExpEntry se[32];
+    for (int i = threadIdx.x; i < 32; i += kBlock) se[i] = g_exp_table[i];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = table_exp(x[i], se);
+}
+
+hipError_t launch_gbm_exp(hipStream_t st, const double *x, double *out, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gbm_exp, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, x, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths,
                                 uint32_t blocks, uint32_t *out) {

@@ -37,6 +37,7 @@ hipError_t launch_fill(hipStream_t st, const FillList &fills);
 
 hipError_t launch_philox_blocks(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
                                 uint32_t paths, uint32_t blocks, uint32_t *out);
+hipError_t launch_gbm_exp(hipStream_t st, const double *x, double *out, uint32_t n);
 hipError_t launch_stream_normals(hipStream_t st, uint64_t seed, uint32_t domain, uint32_t path0,
                                  uint32_t paths, uint32_t count, double *out, uint32_t *draws,
                                  int *fail);

@@ -180,7 +180,7 @@ EXPORTED = (
     "fohip_executor_info", "fohip_set_tuning", "fohip_set_near_tie_window", "fohip_price_lsm_call",
     "fohip_price_lsm_grouped",
     "fohip_grouped_diagnostics", "fohip_simulate_gbm_seeded", "fohip_simulate_gbm_seeded_device",
-    "fohip_philox_blocks",
+    "fohip_philox_blocks", "fohip_gbm_exp_probe",
     "fohip_stream_normals", "fohip_validate_time_grid", "fohip_validate_log_discount_curve",
     "fohip_validate_cumulative_yield_curve", "fohip_validate_event_schedule",
     "fohip_validate_constant_volatility", "fohip_validate_cash_dividend_amount",
@@ -247,6 +247,7 @@ def lib() -> C.CDLL:
     handle.fohip_simulate_gbm_seeded_device.argtypes = [
         C.c_void_p, C.POINTER(FohipPathPlan), C.c_double, C.c_uint64, C.c_uint64, C.c_uint32,
         C.c_void_p, C.c_int32, C.c_void_p, E]
+    handle.fohip_gbm_exp_probe.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp, E]
     handle.fohip_philox_blocks.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
                                            C.c_uint32, C.c_uint32, _u32p, E]
     handle.fohip_stream_normals.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,

@@ -357,6 +357,17 @@ def philox_blocks(executor: HipExecutor, *, seed: int, domain: int, path0: int, 
     return out
 
 
+def gbm_exp(executor: HipExecutor, x: np.ndarray) -> np.ndarray:
+    """exp(x) as the device's GBM step computes it (parity probe)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    err = FohipError()
+    code = lib().fohip_gbm_exp_probe(executor._handle.ptr, ptr(x, C.c_double), x.size,  # noqa: SLF001
+                                     ptr(out, C.c_double), C.byref(err))
+    raise_for(code, err, lsm=False)
+    return out
+
+
 def stream_normals(executor: HipExecutor, *, seed: int, domain: int, path0: int, paths: int,
                    count: int) -> tuple[np.ndarray, np.ndarray]:
     """Device ZIGNOR normals and the u64 draws each consumed, shape [paths, count]."""
@@ -378,6 +389,7 @@ __all__ = [
     "HipPrecision",
     "RandomStream",
     "SimulationError",
+    "gbm_exp",
     "philox_blocks",
     "simulate_gbm_seeded",
     "simulate_gbm_seeded_tensor",

@@ -393,6 +393,11 @@ int fohip_simulate_gbm_seeded_device(fohip_executor *executor, const fohip_path_
 int fohip_philox_blocks(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,
                         uint32_t paths, uint32_t blocks, uint32_t *out, fohip_error *err);
 
+/* The GBM step's binary64 exponential, out[i] = exp(x[i]) as the simulation kernels compute it
+ * (a table-driven exp within one ulp of the correctly rounded value; parity probe). */
+int fohip_gbm_exp_probe(fohip_executor *executor, const double *x, size_t n, double *out,
+                        fohip_error *err);
+
 /* ZIGNOR normals of paths path0..path0+paths-1: out[p * count + i]; draws[p * count + i] is the
  * number of u64 draws normal i consumed (1 on the fast path). */
 int fohip_stream_normals(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,

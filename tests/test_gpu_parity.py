@@ -93,6 +93,32 @@ def test_fp64_paths_match_the_oracle(hip_f64, oracle):
     assert err < 1e-13
 
 
+def test_gbm_exponential_is_within_one_ulp_of_the_c_library(hip_f64):
+    """The device's table-driven exp of the GBM step against the C library's (Python's math.exp):
+    at most one ulp apart on random arguments of the step's range and on wide ones, equal on the
+    large majority, and equal on the special values the step can produce."""
+    import math
+
+    import finoptions_hip as fh
+
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-5.0, 5.0, 200_000), rng.uniform(-1e-3, 1e-3, 50_000),
+                        rng.uniform(-700.0, 700.0, 50_000)])
+    got = fh.simulation.gbm_exp(hip_f64, x)
+    ref = np.array([math.exp(v) for v in x])
+    ulps = np.abs(got.view(np.int64) - ref.view(np.int64))
+    print(f"table exp: max {ulps.max()} ulp, equal fraction {np.mean(ulps == 0):.4f}")
+    assert ulps.max() <= 1
+    assert np.mean(ulps == 0) > 0.99
+    special = np.array([0.0, -0.0, 1e-300, -1e-300, 700.0, -700.0, 709.5, -745.2, 710.0, -800.0,
+                        np.inf, -np.inf])
+    sg = fh.simulation.gbm_exp(hip_f64, special)
+    for v, g in zip(special, sg):
+        r = math.exp(v) if v < 709.79 else math.inf
+        assert g == r or abs(np.float64(g).view(np.int64) - np.float64(r).view(np.int64)) <= 1, (v, g, r)
+    assert np.isnan(fh.simulation.gbm_exp(hip_f64, np.array([np.nan]))[0])
+
+
 def test_device_tensor_export_is_the_host_export(hip_f64):
     """The observation matrix left in HBM as a torch tensor is bitwise the host export (binary64),
     or its single rounding (binary32); torch work queued before the call is ordered first."""
