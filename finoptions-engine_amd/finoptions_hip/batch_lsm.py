@@ -341,7 +341,7 @@ def _dividend_fields(req: FohipGroupedRequest, dividends: BatchDividends, groups
 def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates, volatilities,
            dividends: BatchDividends, mesh: ExerciseMesh, executor: HipExecutor, method: Lsm,
            greeks: tuple[Greek, ...], *, what: str,
-           path_counts: RaggedPathCounts | None = None) -> LsmBatchResult:
+           path_counts: RaggedPathCounts | None = None, tune: bool = True) -> LsmBatchResult:
     if type(executor) is not HipExecutor:
         raise InputError("executor must be HipExecutor")
     if type(method) is not Lsm:
@@ -357,6 +357,19 @@ def _price(group_offsets: np.ndarray, initial_spots, strikes, maturities, rates,
     mf = _mesh_fields(mesh, groups)
     if mf["mesh_kind"] == 1 and type(dividends) is ContinuousDividendYieldBatch:
         raise InputError("CashDividendExerciseMesh requires cash-dividend inputs")
+    if tune and executor.autotune and groups > 0:
+        from .autotune import tune_request
+
+        def sample_price(**cols):
+            return _price(cols["group_offsets"], cols["initial_spots"], cols["strikes"],
+                          cols["maturities"], cols["rates"], cols["volatilities"], cols["dividends"],
+                          cols["mesh"], executor, method, greeks, what=what,
+                          path_counts=cols["path_counts"], tune=False)
+
+        tune_request(executor, sample_price, group_offsets=group_offsets, initial_spots=initial_spots,
+                     strikes=strikes, maturities=maturities, rates=rates, volatilities=volatilities,
+                     dividends=dividends, mesh=mesh, method=method, greeks=greeks,
+                     path_counts=path_counts)
     rows = int(strikes.size)
     keep: list = []
     req = FohipGroupedRequest()

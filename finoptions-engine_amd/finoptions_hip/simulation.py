@@ -82,11 +82,14 @@ class HipExecutor:
     """Reusable MI355X resource with an exact device and numerical method.
 
     Parameters mirror ``CpuExecutor(threads, math_backend)``: ``device`` is the HIP device index and
-    ``precision`` the explicit :class:`HipPrecision`. Neither is inferred or substituted.
+    ``precision`` the explicit :class:`HipPrecision`. Neither is inferred or substituted. With
+    ``autotune`` the batch entry points choose the launch parameters per request shape (see
+    ``finoptions_hip.autotune``); results are the same bits either way.
     """
 
     device: int
     precision: HipPrecision = HipPrecision.F64
+    autotune: bool = False
     _handle: _Handle = field(init=False, repr=False, compare=False)
     _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto", "serial", 0))
 
@@ -97,6 +100,8 @@ class HipExecutor:
         if type(self.precision) is not HipPrecision:
             msg = f"precision must be HipPrecision; observed {type(self.precision).__name__}"
             raise InputError(msg)
+        if type(self.autotune) is not bool:
+            raise InputError(f"autotune must be a plain bool; observed {type(self.autotune).__name__}")
         object.__setattr__(self, "_handle", _Handle(self.device, self.precision))
 
     @property

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import c2_columns, hip_groups, hip_method
+from conftest import c2_columns, hip_groups, hip_method, oracle_grouped
 
 pytestmark = pytest.mark.gpu
 
@@ -51,7 +51,21 @@ def test_greeks_do_not_change_prices(hip_f64):
     np.testing.assert_array_equal(plain.early_exercise_paths, with_greeks.early_exercise_paths)
 
 
-def test_autograd_backward_is_the_pathwise_vjp(hip_f64):
+def _oracle_vjp(oracle, req, rows_per_group, weights, names):
+    """Per-group sums of weight x the oracle's pathwise sensitivity, group by group."""
+    out = {n: [] for n in names}
+    start = 0
+    for g, n_rows in enumerate(rows_per_group):
+        ref = oracle.grouped_pathwise_greeks(req, g)
+        for n in names:
+            out[n].append(float(np.sum(ref[n] * weights[start:start + n_rows])))
+        start += n_rows
+    return {n: np.array(v) for n, v in out.items()}
+
+
+def test_autograd_gradient_is_the_oracle_pathwise_vjp(hip_f64, oracle):
+    """Two continuous-yield groups: backward equals the vector-Jacobian product of the oracle's
+    pathwise sensitivities (frozen policy) in S0, sigma, r and q."""
     import finoptions_hip as fh
     import torch
     from finoptions_hip.autograd import lsm_grouped_prices
@@ -74,12 +88,51 @@ def test_autograd_backward_is_the_pathwise_vjp(hip_f64):
                                 executor=hip_f64, method=method)
     weights = torch.linspace(0.5, 1.5, prices.numel(), dtype=torch.float64)
     (prices * weights).sum().backward()
-    direct = fh.price_lsm_call_grouped_batch(
-        groups=hip_groups(cols), mesh=mesh, executor=hip_f64, method=method,
-        greeks=tuple(getattr(fh.Greek, g) for g in ALL_GREEKS))
+    req = oracle_grouped(cols, base_intervals=10, training_paths=16_384, evaluation_paths=16_384,
+                         seed=4, eval_domain=3, insufficient_support=1)
+    names = ("delta", "vega", "rho", "dividend_yield")
+    ref = _oracle_vjp(oracle, req, (3, 5), weights.numpy(), names)
+    for leaf, name in zip(leaves, names):
+        err = float(np.max(np.abs(leaf.grad.numpy() - ref[name]) / np.abs(ref[name])))
+        print(f"autograd d/d{name}: max rel err vs oracle {err:.2e}")
+        assert err < 1e-9
+
+
+def test_autograd_with_cash_dividends_differentiates_spot_vol_and_rate(hip_f64, oracle):
+    """Cash-dividend groups (liquidator): gradients in S0, sigma and r equal the oracle's pathwise
+    VJP; only the sensitivities of inputs that require grad are computed."""
+    import finoptions_hip as fh
+    import torch
+    from finoptions_hip.autograd import lsm_grouped_prices
+
+    cols = c2_columns(strikes=6)
+    cols["group_offsets"] = np.array([0, 2, 6], dtype=np.uint32)
+    cols["initial_spots"] = np.array([100.0, 95.0])
+    cols["maturities"] = np.array([1.0, 1.0])
+    cols["rates"] = np.array([0.03, 0.04])
+    cols["volatilities"] = np.array([0.25, 0.35])
+    cols.pop("yields")
+    cols["cash_offsets"] = np.array([0, 1, 3], dtype=np.int64)
+    cols["cash_times"] = np.array([0.5, 0.3, 0.7])
+    cols["cash_amounts"] = np.array([2.0, 1.0, 1.5])
+    mesh = fh.UniformExerciseMesh(base_intervals=10, origin_exercise=fh.OriginExercise.EXCLUDED)
+    method = hip_method(16_384, 16_384, 6, eval_domain=3,
+                        support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    spots = torch.tensor(cols["initial_spots"], dtype=torch.float64, requires_grad=True)
+    vols = torch.tensor(cols["volatilities"], dtype=torch.float64, requires_grad=True)
+    rates = torch.tensor(cols["rates"], dtype=torch.float64, requires_grad=True)
+    dividends = hip_groups(cols).dividends
+    prices = lsm_grouped_prices(spots, vols, rates, group_offsets=cols["group_offsets"],
+                                strikes=cols["strikes"], maturities=cols["maturities"], mesh=mesh,
+                                executor=hip_f64, method=method, dividends=dividends)
+    direct = fh.price_lsm_call_grouped_batch(groups=hip_groups(cols), mesh=mesh, executor=hip_f64,
+                                             method=method)
     np.testing.assert_array_equal(prices.detach().numpy(), direct.prices)
-    w = weights.numpy()
-    for leaf, name in zip(leaves, ALL_GREEKS):
-        g = direct.greeks[getattr(fh.Greek, name)] * w
-        expected = np.array([g[0:3].sum(), g[3:8].sum()])
-        np.testing.assert_allclose(leaf.grad.numpy(), expected, rtol=1e-12)
+    prices.sum().backward()
+    req = oracle_grouped(cols, base_intervals=10, training_paths=16_384, evaluation_paths=16_384,
+                         seed=6, eval_domain=3, insufficient_support=1)
+    ref = _oracle_vjp(oracle, req, (2, 4), np.ones(6), ("delta", "vega", "rho"))
+    for leaf, name in ((spots, "delta"), (vols, "vega"), (rates, "rho")):
+        err = float(np.max(np.abs(leaf.grad.numpy() - ref[name]) / np.abs(ref[name])))
+        print(f"cash autograd d/d{name}: max rel err vs oracle {err:.2e}")
+        assert err < 1e-9
