@@ -56,6 +56,16 @@ __device__ __forceinline__ bool xcd_item_column(uint32_t nitems, uint32_t column
     return item < nitems;
 }
 
+// Backward steps alternate the direction in which a launch walks its work items: odd steps take
+// them last to first. Blocks are dispatched roughly in block order, so the first blocks of a step
+// read the spot rows and cashflow columns that the previous step read last, while those lines are
+// still held by the 256 MB memory-side Infinity Cache (about a quarter of a large step's reads
+// then come from it instead of HBM). Every item keeps its own partial slots, so results do not
+// depend on the direction.
+__device__ __forceinline__ uint32_t serpentine_item(uint32_t step, uint32_t nitems, uint32_t wi) {
+    return (step & 1u) ? nitems - 1u - wi : wi;
+}
+
 // Work item wi as wave-uniform scalars.
 __device__ __forceinline__ uint2 load_item(const uint2 *items, uint32_t wi) {
     const uint32_t *w = reinterpret_cast<const uint32_t *>(items + wi);
@@ -762,6 +772,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
     __shared__ double transpose[kWaves][14 * 66];
     uint32_t wi, set;
     if (!xcd_item_column(nitems, sets, wi, set)) return;
+    wi = serpentine_item(step, nitems, wi);
     const uint2 item = load_item(items, wi);
     const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
     const uint32_t strike_begin = g->strike_begin, strike_end = g->strike_end;
@@ -946,6 +957,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_stop_step(DevTables t, FitPar
     __shared__ double disc_rows[256];   // exp(logD) of the group's observation rows (obs <= 255)
     uint32_t wi, set;
     if (!xcd_item_column(nitems, sets, wi, set)) return;
+    wi = serpentine_item(step, nitems, wi);
     const uint2 item = load_item(items, wi);
     const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
     const uint32_t strike_begin = g->strike_begin, strike_end = g->strike_end;
