@@ -22,6 +22,18 @@ constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 
+// A wave-uniform word moved into a vector register, where the compiler treats it as varying by
+// lane. The Philox keys go through it: the nine bumped keys of a block are loop invariants that
+// the compiler hoists out of the path loop, and held in scalar registers (18 of them) they push
+// the generation kernels into spilling scalars to vector lanes, reloaded by a vector instruction
+// at every use inside the loop; held in vector registers they cost nothing the kernels' LDS-bound
+// occupancy does not already leave free.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t x) {
+    uint32_t r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+
 // Random123 Philox4x32 with ten rounds (philox.rs:18-41). Round output lanes are
 // [hi(M1*c2)^c1^k0, lo(M1*c2), hi(M0*c0)^c3^k1, lo(M0*c0)]; the key is bumped before rounds 2..10.
 __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -56,8 +68,8 @@ struct PathStream {
     bool has_spare;
 
     __device__ __forceinline__ void init(uint64_t seed, uint32_t domain_id, uint32_t path_id) {
-        k0 = static_cast<uint32_t>(seed);
-        k1 = static_cast<uint32_t>(seed >> 32);
+        k0 = in_vgpr(static_cast<uint32_t>(seed));
+        k1 = in_vgpr(static_cast<uint32_t>(seed >> 32));
         path = path_id;
         domain = domain_id;
         block_lo = 0;
@@ -103,8 +115,8 @@ struct RingStream {
     // words; `lane` indexes this thread's column.
     __device__ __forceinline__ void init(uint64_t seed, uint32_t domain_id, uint32_t path_id, uint64_t *lds,
                                          uint32_t lane, uint32_t columns) {
-        k0 = static_cast<uint32_t>(seed);
-        k1 = static_cast<uint32_t>(seed >> 32);
+        k0 = in_vgpr(static_cast<uint32_t>(seed));
+        k1 = in_vgpr(static_cast<uint32_t>(seed >> 32));
         path = path_id;
         domain = domain_id;
         block_lo = block_hi = 0;
@@ -191,9 +203,15 @@ __device__ __forceinline__ double standard_unit(uint64_t bits) {
 
 // Rejected draws: wedge test, redraw, or tail (normal.rs:50-91). Returns false at the bounded
 // rejection limit. `draws` counts consumed u64 draws (parity probe only).
+// The wedge test "F[i+1] + (F[i] - F[i+1]) u < exp(-value^2 / 2)" is decided first by the layer's
+// quadratic squeeze (SQ[i] = {1/w, c, gu, gl}, see gen_zig_tables.py): with t = (|value| - X[i+1]) / w
+// and q = t + c t (t - 1), u - q > gu means the test passes and u - q < -gl that it fails, with a
+// margin that covers the binary32 arithmetic below (u from the draw's top 24 bits) and every
+// rounding of the exact test; only inside the band does the exact test run, reading F from global
+// memory. The outcome, and so every normal and draw count, is that of the exact test.
 template <typename Stream>
 __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double unit, double value,
-                                             const double *__restrict__ X,
+                                             const double *__restrict__ X, const float4 *__restrict__ SQ,
                                              const double *__restrict__ F, double zr, double &out,
                                              uint32_t &draws, const ExpEntry *__restrict__ et) {
     for (int it = 0; it < kMaxRejectionIterations; ++it) {
@@ -209,9 +227,18 @@ __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double uni
             }
             return false;
         }
-        const double ordinate = F[index + 1] + (F[index] - F[index + 1]) * standard_unit(s.next_u64());
+        const uint64_t ubits = s.next_u64();
         draws += 1;
-        if (ordinate < table_exp(-0.5 * value * value, et)) {
+        const float4 sq = SQ[index];
+        const float t = static_cast<float>(fabs(value) - X[index + 1]) * sq.x;
+        const float q = fmaf(sq.y, t * (t - 1.0f), t);
+        const float d = static_cast<float>(static_cast<uint32_t>(ubits >> 40)) * 0x1p-24f - q;
+        bool accept = d > sq.z;
+        if (__builtin_expect(!accept && !(d < -sq.w), 0)) {
+            const double ordinate = F[index + 1] + (F[index] - F[index + 1]) * standard_unit(ubits);
+            accept = ordinate < table_exp(-0.5 * value * value, et);
+        }
+        if (accept) {
             out = value;
             return true;
         }
@@ -230,8 +257,9 @@ __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double uni
 
 template <typename Stream>
 __device__ __forceinline__ bool zignor_normal(Stream &s, const double *__restrict__ X,
-                                              const double *__restrict__ F, double zr, double &out,
-                                              uint32_t &draws, const ExpEntry *__restrict__ et) {
+                                              const float4 *__restrict__ SQ, const double *__restrict__ F,
+                                              double zr, double &out, uint32_t &draws,
+                                              const ExpEntry *__restrict__ et) {
     const uint64_t bits = s.next_u64();
     draws += 1;
     const int index = static_cast<int>(bits & 0xff);
@@ -241,7 +269,7 @@ __device__ __forceinline__ bool zignor_normal(Stream &s, const double *__restric
         out = value;
         return true;
     }
-    return zignor_rejected(s, index, unit, value, X, F, zr, out, draws, et);
+    return zignor_rejected(s, index, unit, value, X, SQ, F, zr, out, draws, et);
 }
 
 // ------------------------------------------------------- GBM transition and cash events

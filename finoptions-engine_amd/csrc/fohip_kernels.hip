@@ -24,20 +24,21 @@ namespace fohip {
 
 __device__ const double g_zig_x[257] = FOHIP_ZIG_X_INIT;
 __device__ const ExpEntry g_exp_table[32] = FOHIP_EXP_TABLE_INIT;
+// F is read only by the exact wedge test inside the squeeze's band (about 1e-5 of the normals),
+// straight from global memory; the layer abscissae and the squeeze live in LDS.
 __device__ const double g_zig_f[257] = FOHIP_ZIG_F_INIT;
+__device__ const float4 g_zig_squeeze[256] = FOHIP_ZIG_SQUEEZE_INIT;
 
-__device__ __forceinline__ void load_zig(double *sx, double *sf) {
-    for (int i = threadIdx.x; i < 257; i += blockDim.x) {
-        sx[i] = g_zig_x[i];
-        sf[i] = g_zig_f[i];
-    }
+__device__ __forceinline__ void load_zig(double *sx, float4 *ssq) {
+    for (int i = threadIdx.x; i < 257; i += blockDim.x) sx[i] = g_zig_x[i];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) ssq[i] = g_zig_squeeze[i];
     __syncthreads();
 }
 
 // the ZIGNOR tables and the exponential's table of a kernel that steps paths
-__device__ __forceinline__ void load_zig_exp(double *sx, double *sf, ExpEntry *se) {
+__device__ __forceinline__ void load_zig_exp(double *sx, float4 *ssq, ExpEntry *se) {
     for (int i = threadIdx.x; i < 32; i += blockDim.x) se[i] = g_exp_table[i];
-    load_zig(sx, sf);
+    load_zig(sx, ssq);
 }
 
 // Path-level failures only lower the row's key; the failure kind is a function of the key's stage
@@ -89,10 +90,11 @@ __global__ void k_philox_blocks(uint64_t seed, uint32_t domain, uint32_t path0, 
 
 __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0, uint32_t paths, uint32_t count,
                                  double *out, uint32_t *draws, int *fail) {
-    __shared__ double sx[257], sf[257];
+    __shared__ double sx[257];
+    __shared__ float4 ssq[256];
     __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig_exp(sx, sf, se);
+    load_zig_exp(sx, ssq, se);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= paths) return;
     RingStream s;
@@ -101,7 +103,7 @@ __global__ void k_stream_normals(uint64_t seed, uint32_t domain, uint32_t path0,
         double z = 0.0;
         uint32_t d = 0;
         s.top_up(true);
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, d, se)) atomicExch(fail, 1);
+        if (!zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, d, se)) atomicExch(fail, 1);
         out[static_cast<uint64_t>(p) * count + i] = z;
         draws[static_cast<uint64_t>(p) * count + i] = d;
     }
@@ -118,10 +120,11 @@ template <typename Store>
 __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *__restrict__ items,
                                                      Store *__restrict__ paths_out, int evaluation,
                                                      unsigned long long *__restrict__ fail) {
-    __shared__ double sx[257], sf[257];
+    __shared__ double sx[257];
+    __shared__ float4 ssq[256];
     __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig_exp(sx, sf, se);
+    load_zig_exp(sx, ssq, se);
     const uint2 item = items[blockIdx.x];
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = evaluation ? g.ne : g.nt;
@@ -142,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
         double z;
         uint32_t draws = 0;
         s.top_up(true);
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) {
+        if (!zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, draws, se)) {
             ok = false;
             break;
         }
@@ -1415,7 +1418,8 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                                                      const double *__restrict__ znorm) {
     using Tan = typename std::conditional<FAST, float, double>::type;
     extern __shared__ __align__(16) uint32_t hist[];   // [TS][obs] when use_lds_hist, then DEC's tables
-    __shared__ double sx[257], sf[257];
+    __shared__ double sx[257];
+    __shared__ float4 ssq[256];
     __shared__ ExpEntry se[32];
     constexpr int NACC = GREEKS ? 10 : 2;
     __shared__ double scratch[NACC * kWaves];
@@ -1436,7 +1440,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     const uint32_t local0 = s_first - strike_begin;
     const Model *tile_models = models + g->model_base + local0 * obs;   // [strike][row]
     const int degree = fp.degree, tie = fp.tie;
-    if (!STORED) load_zig(sx, sf);
+    if (!STORED) load_zig(sx, ssq);
     for (int i = threadIdx.x; i < 32; i += kBlock) se[i] = g_exp_table[i];
     if (threadIdx.x < TS) tie_lds[threadIdx.x] = 0;
     __syncthreads();
@@ -1512,7 +1516,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
         } else {
             uint32_t draws = 0;
             s.top_up(ok);
-            if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) ok = false;
+            if (ok && !zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, draws, se)) ok = false;
         }
         const uint32_t si = step_base + iv;
         double growth;
@@ -1865,10 +1869,11 @@ __global__ __launch_bounds__(kBlock) void k_fit_gate_scan(DevTables t, const uin
 __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint2 *__restrict__ items,
                                                          double *__restrict__ znorm,
                                                          unsigned long long *__restrict__ sim_fail) {
-    __shared__ double sx[257], sf[257];
+    __shared__ double sx[257];
+    __shared__ float4 ssq[256];
     __shared__ ExpEntry se[32];
     __shared__ uint64_t ring[kRingSlots * kBlock];
-    load_zig_exp(sx, sf, se);
+    load_zig_exp(sx, ssq, se);
     const uint2 item = load_item(items, blockIdx.x);
     const DevGroup &g = t.groups[item.x];
     const uint32_t n = g.ne, p = item.y * kEvalChunk + threadIdx.x;
@@ -1881,7 +1886,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
         double z = 0.0;
         uint32_t draws = 0;
         s.top_up(ok);
-        if (ok && !zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) ok = false;
+        if (ok && !zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, draws, se)) ok = false;
         out[static_cast<uint64_t>(iv) * n] = ok ? z : __builtin_nan("");
     }
     if (!ok) atomicMin(sim_fail + item.x, static_cast<unsigned long long>(p));
@@ -1890,9 +1895,10 @@ __global__ __launch_bounds__(kBlock) void k_eval_normals(DevTables t, const uint
 // Serial replay of one failed path (the reference replays the lowest failing index serially to
 // report a deterministic error, seeded.rs:155-186): out = {interval, spot, normal, kind}.
 __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint32_t path, double *out) {
-    __shared__ double sx[257], sf[257];
+    __shared__ double sx[257];
+    __shared__ float4 ssq[256];
     __shared__ ExpEntry se[32];
-    load_zig_exp(sx, sf, se);
+    load_zig_exp(sx, ssq, se);
     if (threadIdx.x != 0) return;
     const DevGroup &g = t.groups[group];
     PathStream s;
@@ -1902,7 +1908,7 @@ __global__ void k_replay_path(DevTables t, uint32_t group, int evaluation, uint3
     for (uint32_t i = 0; i < g.intervals; ++i) {
         double z;
         uint32_t draws = 0;
-        if (!zignor_normal(s, sx, sf, FOHIP_ZIG_R, z, draws, se)) {
+        if (!zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, draws, se)) {
             out[0] = i;
             out[1] = spot;
             out[2] = 0.0;

@@ -75,6 +75,33 @@ def test_normals_match_the_oracle_stream(hip_f64, oracle, golden):
     assert flips <= 1
 
 
+def test_squeezed_wedge_tests_match_the_oracle_stream(hip_f64, oracle):
+    """2^21 normals (about 31,000 wedge tests, most decided by the layer squeeze, a few dozen by
+    the exponential inside its band): every draw count equal to the oracle's exact stream,
+    single-draw normals bit for bit, wedge-accepted normals too (they are value = unit * X)."""
+    import finoptions_hip.simulation as sim
+
+    paths, count = 16384, 128
+    dev, dev_draws = sim.stream_normals(hip_f64, seed=2024, domain=7, path0=0, paths=paths,
+                                        count=count)
+    mismatched = 0
+    wedge = 0
+    for p in range(paths):
+        ref, ref_draws = oracle.normals(2024, 7, p, count)
+        if not np.array_equal(ref_draws, dev_draws[p]):
+            mismatched += 1
+            continue
+        wedge += int(np.sum(ref_draws > 1))
+        # every normal but a tail one is unit * X[index], the same product on both sides
+        same = np.abs(ref) < 3.6541528853610088
+        assert np.array_equal(ref[same].view(np.uint64), dev[p][same].view(np.uint64))
+        np.testing.assert_allclose(dev[p][~same], ref[~same], rtol=1e-14)
+    print(f"squeezed stream: {paths * count} normals, {wedge} multi-draw normals, "
+          f"{mismatched} paths with a different draw count")
+    assert wedge > 20_000
+    assert mismatched <= 1   # a tail decision may flip on the device libm's log (last bit)
+
+
 def test_fp64_paths_match_the_oracle(hip_f64, oracle):
     import finoptions_hip as fh
 
