@@ -1278,7 +1278,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         }
         // the escrowed instantiation also serves spot-model groups
         PROF_LAUNCH(gen_st, "k_evaluate",
-                    launch_evaluate(gen_st, spec.fused, greeks, R.escrow, tile, t, fp, R.ite,
+                    launch_evaluate(gen_st, spec.fused, spec.greeks, R.escrow, tile, t, fp, R.ite,
                                     static_cast<uint32_t>(R.items_e.size()), R.max_strikes, R.max_obs, models,
                                     static_cast<double *>(R.part_e_d), static_cast<uint32_t *>(R.counts_d),
                                     static_cast<uint32_t *>(R.nosup_d), static_cast<RowError *>(R.errors_d),
@@ -1628,9 +1628,10 @@ static int price_call(fohip_executor *ex, const fohip_call_request *rq, fohip_ca
         if (res->observation_nodes) res->observation_nodes[r] = p.obs_nodes[r];
         if (res->exercise_counts) res->exercise_counts[r] = origin_exercise ? (r == 0 ? res->paths : 0) : counts[r];
     }
-    for (int a = 0; a < 4; ++a) {
-        res->greeks[a] = o.greek[a];
-        res->greek_standard_errors[a] = o.greek_se[a];
+    for (int a = 0; a < 4; ++a) {   // unrequested sensitivities are not computed
+        const bool wanted = (rq->greeks & (1u << a)) != 0;
+        res->greeks[a] = wanted ? o.greek[a] : std::nan("");
+        res->greek_standard_errors[a] = wanted ? o.greek_se[a] : std::nan("");
     }
     res->near_tie_training_decisions = o.ties_training;
     res->near_tie_evaluation_decisions = o.ties_evaluation;

@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "fohip_device.hpp"
+#include "fohip.h"
 #include "fohip_kernels.hpp"
 
 namespace fohip {
@@ -1389,9 +1390,13 @@ __device__ __noinline__ void record_eval_errors(RowError *e, uint32_t row, uint3
 // Waves per SIMD asked of an instantiation. The fused Greek evaluation over 6- and 8-strike tiles
 // of stored normals runs at 5 (96 VGPRs) rather than 4: the extra wave hides more of the decision
 // chain's latency. The 6-strike form, the default, fits without spills; the 8-strike form spills
-// its payoff tangents. Every other form keeps the compiler's own choice.
-template <bool FAST, int TS, bool GREEKS, bool STORED>
-constexpr int eval_waves() { return (FAST && GREEKS && STORED && (TS == 6 || TS == 8)) ? 5 : 1; }
+// its payoff tangents. Single-strike tiles that regenerate their paths run at 6 (80 VGPRs, which
+// their LDS allows) when they carry at most the two tangents of delta and vega; with all four
+// they would spill and keep the compiler's own choice, as every other form does.
+template <bool FAST, int TS, int GREEKS, bool STORED>
+constexpr int eval_waves() {
+    return (FAST && GREEKS && STORED && (TS == 6 || TS == 8)) ? 5 : ((TS == 1 && !STORED && GREEKS != 4) ? 6 : 1);
+}
 
 // Decision data of one (row, strike) as staged in LDS: K, a, b, hz0..hz4.
 constexpr int kDecDoubles = 8;
@@ -1409,7 +1414,7 @@ __device__ __forceinline__ double continuation_staged(const double *d, double sp
 // DEC: the tile's decision data (kDecDoubles per (row, strike), [row][strike]) and each row's rule
 // kinds (4 bits per strike) are staged in LDS after the exercise histogram, so that a decision
 // reads LDS broadcasts instead of waiting on scalar loads of rules that lie obs rules apart.
-template <bool FAST, int TS, bool GREEKS, bool ESCROW, bool STORED, bool DEC, bool WIN>
+template <bool FAST, int TS, int GREEKS, bool ESCROW, bool STORED, bool DEC, bool WIN>
 __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) void k_evaluate(DevTables t, FitParams fp, const uint2 *__restrict__ items,
                                                      uint32_t nitems, uint32_t tiles, const Model *__restrict__ models,
                                                      double *__restrict__ partials, uint32_t *__restrict__ counts,
@@ -1486,7 +1491,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     bool ok = p < n;
     uint32_t active = ok ? live : 0u;
     double pay[TS];
-    Tan dpay[TS][GREEKS ? 4 : 1];
+    Tan dpay[TS][GREEKS ? GREEKS : 1];
     // DEC: each lane keeps its stopping row per strike, 8 bits each (0xFF: no stop; the launch
     // guarantees obs <= 255), and the exercise histogram is added once at the end rather than
     // with a ballot and an atomic at every stop
@@ -1497,7 +1502,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     for (int i = 0; i < TS; ++i) {
         pay[i] = 0.0;
 #pragma unroll
-        for (int a = 0; a < (GREEKS ? 4 : 1); ++a) dpay[i][a] = Tan(0);
+        for (int a = 0; a < (GREEKS ? GREEKS : 1); ++a) dpay[i][a] = Tan(0);
     }
     __shared__ uint64_t ring[STORED ? 1 : kRingSlots * kBlock];
     RingStream s;
@@ -1532,8 +1537,10 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                 const double dt = *uniform(t.dt + si);
                 tan[0] = tan[0] * growth;
                 tan[1] = fma(tan[1], growth, spot * fma(*uniform(t.sqrt_dt + si), z, -sigma * dt));
-                tan[2] = fma(tan[2], growth, spot * dt);
-                tan[3] = fma(tan[3], growth, -spot * dt);
+                if (GREEKS == 4) {
+                    tan[2] = fma(tan[2], growth, spot * dt);
+                    tan[3] = fma(tan[3], growth, -spot * dt);
+                }
             }
         }
         const uint32_t node = node_base + iv + 1;
@@ -1550,18 +1557,25 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             const double t_row = GREEKS ? *uniform(t.row_time + row_base + row) : 0.0;
             // the exercise decision observes the stock: F + A(node) for escrowed groups
             const double S = ESCROW ? spot + *uniform(t.row_escrow + row_base + row) : spot;
-            // discounted observed tangents, shared by every strike that stops on this row
-            double od[4] = {0.0, 0.0, 0.0, 0.0};
-            if (GREEKS) {
+            // discounted observed tangents, shared by every strike that stops on this row: with
+            // several strikes per tile formed once per row (and kept from being recomputed per
+            // strike), with one strike only inside the branch a stopping lane takes
+            Tan od0 = Tan(0), od1 = Tan(0), od3 = Tan(0);
+            double od2 = 0.0;
+            auto form_od = [&]() {
                 double o[4] = {tan[0], tan[1], tan[2], tan[3]};
                 if (ESCROW) observed_tangents(tan, uniform(g->chain), *uniform(t.row_escrow_dr + row_base + row), o);
-#pragma unroll
-                for (int a = 0; a < 4; ++a) od[a] = o[a] * disc;
-            }
-            Tan od0 = static_cast<Tan>(od[0]), od1 = static_cast<Tan>(od[1]), od3 = static_cast<Tan>(od[3]);
-            double od2 = od[2];
-            if (GREEKS) {   // formed once per row: keep the compiler from recomputing them per strike
-                asm volatile("" : "+v"(od0), "+v"(od1), "+v"(od3), "+v"(od2));
+                od0 = static_cast<Tan>(o[0] * disc);
+                od1 = static_cast<Tan>(o[1] * disc);
+                if (GREEKS == 4) {
+                    od2 = o[2] * disc;
+                    od3 = static_cast<Tan>(o[3] * disc);
+                }
+            };
+            if constexpr (GREEKS != 0 && TS > 1) {
+                form_od();
+                if constexpr (GREEKS == 4) asm volatile("" : "+v"(od0), "+v"(od1), "+v"(od3), "+v"(od2));
+                else asm volatile("" : "+v"(od0), "+v"(od1));
             }
             const uint32_t kinds_row = DEC ? __builtin_amdgcn_readfirstlane(dec_kind[row]) : 0u;
             const double *dec_row = DEC ? dec + static_cast<size_t>(row) * TS * kDecDoubles : nullptr;
@@ -1591,11 +1605,14 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                         const double pv = intr * disc;
                         if (ex) {
                             pay[i] = pv;
-                            if (GREEKS) {
+                            if constexpr (GREEKS != 0) {
+                                if constexpr (TS == 1) form_od();
                                 dpay[i][0] = od0;
                                 dpay[i][1] = od1;
-                                dpay[i][2] = static_cast<Tan>(od2 - t_row * pv);
-                                dpay[i][3] = od3;
+                                if constexpr (GREEKS == 4) {
+                                    dpay[i][2] = static_cast<Tan>(od2 - t_row * pv);
+                                    dpay[i][3] = od3;
+                                }
                             }
                             const int sh = 8 * (i & 3);
                             stop_rows[i >> 2] = (stop_rows[i >> 2] & ~(0xFFu << sh)) | (row << sh);
@@ -1610,11 +1627,14 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                     if (tm != 0ull && lane == 0) atomicAdd(&tie_lds[i], static_cast<uint32_t>(__popcll(tm)));
                     const double pv = intr * disc;
                     pay[i] = ex ? pv : pay[i];
-                    if (GREEKS) {
+                    if constexpr (GREEKS != 0) {
+                        if constexpr (TS == 1) form_od();
                         dpay[i][0] = ex ? od0 : dpay[i][0];
                         dpay[i][1] = ex ? od1 : dpay[i][1];
-                        dpay[i][2] = ex ? static_cast<Tan>(od2 - t_row * pv) : dpay[i][2];
-                        dpay[i][3] = ex ? od3 : dpay[i][3];
+                        if constexpr (GREEKS == 4) {
+                            dpay[i][2] = ex ? static_cast<Tan>(od2 - t_row * pv) : dpay[i][2];
+                            dpay[i][3] = ex ? od3 : dpay[i][3];
+                        }
                     }
                     active &= ~(((ex || bad) ? 1u : 0u) << i);
                     const bool payoff_bad = ex && !isfinite(pv);
@@ -1675,12 +1695,14 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
                         atomicAdd(&counts[count_base + (local0 + i) * obs + last], c);
                 }
             }
-            if (GREEKS) {
+            if constexpr (GREEKS != 0) {
                 const bool itm = hold && spot > K;
                 dpay[i][0] = itm ? static_cast<Tan>(o[0] * disc) : dpay[i][0];
                 dpay[i][1] = itm ? static_cast<Tan>(o[1] * disc) : dpay[i][1];
-                dpay[i][2] = itm ? static_cast<Tan>(o[2] * disc - t_last * pv) : dpay[i][2];
-                dpay[i][3] = itm ? static_cast<Tan>(o[3] * disc) : dpay[i][3];
+                if constexpr (GREEKS == 4) {
+                    dpay[i][2] = itm ? static_cast<Tan>(o[2] * disc - t_last * pv) : dpay[i][2];
+                    dpay[i][3] = itm ? static_cast<Tan>(o[3] * disc) : dpay[i][3];
+                }
             }
         }
     }
@@ -1696,8 +1718,8 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
         vals[1] = v * v;
         if (GREEKS) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double d = contributes ? static_cast<double>(dpay[i][a]) : 0.0;
+            for (int a = 0; a < 4; ++a) {   // tangents not carried contribute zeros
+                const double d = (a < GREEKS && contributes) ? static_cast<double>(dpay[i][a < GREEKS ? a : 0]) : 0.0;
                 vals[2 + a] = d;
                 vals[6 + a] = d * d;
             }
@@ -2131,7 +2153,7 @@ uint32_t eval_tile_for(uint32_t max_strikes, bool greeks, uint32_t requested) {
     return tile;
 }
 
-template <bool FAST, int TS, bool G, bool E, bool Z, bool WIN = false>
+template <bool FAST, int TS, int G, bool E, bool Z, bool WIN = false>
 static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitParams &fp, const uint2 *items,
                                 uint32_t nitems, uint32_t max_strikes, uint32_t max_obs, const Model *models,
                                 double *partials, uint32_t *counts, uint32_t *nosupport, RowError *errors,
@@ -2156,7 +2178,7 @@ static hipError_t launch_eval_t(hipStream_t st, const DevTables &t, const FitPar
 }
 
 template <bool FAST>
-static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
+static hipError_t launch_eval_r(hipStream_t st, int gk, bool escrow, uint32_t tile, const DevTables &t, const FitParams &fp,
                                 const uint2 *items, uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                                 const Model *models, double *partials, uint32_t *counts, uint32_t *nosupport,
                                 RowError *errors, unsigned long long *sim_fail, const double *znorm) {
@@ -2166,44 +2188,53 @@ static hipError_t launch_eval_r(hipStream_t st, bool greeks, bool escrow, uint32
     // windowed near-tie counts: a diagnostic form, single-strike tiles regenerating their normals
     if (fp.tie_window != 0.0) {
         if (tile != 1 || znorm != nullptr) return hipErrorInvalidValue;
-        if (greeks)
-            return escrow ? launch_eval_t<FAST, 1, true, true, false, true>(FOHIP_EVAL_ARGS)
-                          : launch_eval_t<FAST, 1, true, false, false, true>(FOHIP_EVAL_ARGS);
-        return escrow ? launch_eval_t<FAST, 1, false, true, false, true>(FOHIP_EVAL_ARGS)
-                      : launch_eval_t<FAST, 1, false, false, false, true>(FOHIP_EVAL_ARGS);
+        if (gk)
+            return escrow ? launch_eval_t<FAST, 1, 4, true, false, true>(FOHIP_EVAL_ARGS)
+                          : launch_eval_t<FAST, 1, 4, false, false, true>(FOHIP_EVAL_ARGS);
+        return escrow ? launch_eval_t<FAST, 1, 0, true, false, true>(FOHIP_EVAL_ARGS)
+                      : launch_eval_t<FAST, 1, 0, false, false, true>(FOHIP_EVAL_ARGS);
     }
 #define FOHIP_EVAL_CASE(TSV, GV, ZV)                                                                               \
-    if (tile == TSV && greeks == GV && (znorm != nullptr) == ZV)                                                   \
+    if (tile == TSV && gk == GV && (znorm != nullptr) == ZV)                                                       \
         return escrow ? launch_eval_t<FAST, TSV, GV, true, ZV>(FOHIP_EVAL_ARGS)                                    \
                       : launch_eval_t<FAST, TSV, GV, false, ZV>(FOHIP_EVAL_ARGS);
-    FOHIP_EVAL_CASE(1, false, false)
-    FOHIP_EVAL_CASE(4, false, false)
-    FOHIP_EVAL_CASE(16, false, false)
-    FOHIP_EVAL_CASE(1, true, false)
-    FOHIP_EVAL_CASE(4, true, false)
-    FOHIP_EVAL_CASE(6, true, false)
-    FOHIP_EVAL_CASE(8, true, false)
+    // single-strike tiles (one group per contract) carry only the tangents the requested Greeks
+    // need; wider tiles carry all four
+    if (tile == 1 && gk == 2) {
+        FOHIP_EVAL_CASE(1, 2, false)
+    }
+    if (gk == 2) gk = 4;
+    FOHIP_EVAL_CASE(1, 0, false)
+    FOHIP_EVAL_CASE(4, 0, false)
+    FOHIP_EVAL_CASE(16, 0, false)
+    FOHIP_EVAL_CASE(1, 4, false)
+    FOHIP_EVAL_CASE(4, 4, false)
+    FOHIP_EVAL_CASE(6, 4, false)
+    FOHIP_EVAL_CASE(8, 4, false)
     if (FAST) {
-        FOHIP_EVAL_CASE(4, false, true)
-        FOHIP_EVAL_CASE(16, false, true)
-        FOHIP_EVAL_CASE(4, true, true)
-        FOHIP_EVAL_CASE(6, true, true)
-        FOHIP_EVAL_CASE(8, true, true)
+        FOHIP_EVAL_CASE(4, 0, true)
+        FOHIP_EVAL_CASE(16, 0, true)
+        FOHIP_EVAL_CASE(4, 4, true)
+        FOHIP_EVAL_CASE(6, 4, true)
+        FOHIP_EVAL_CASE(8, 4, true)
     }
 #undef FOHIP_EVAL_CASE
 #undef FOHIP_EVAL_ARGS
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile, const DevTables &t,
-                           const FitParams &fp, const uint2 *items, uint32_t nitems, uint32_t max_strikes,
-                           uint32_t max_obs, const Model *models, double *partials, uint32_t *counts,
-                           uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail, const double *znorm) {
+hipError_t launch_evaluate(hipStream_t st, bool fast, uint32_t greek_mask, bool escrow, uint32_t tile,
+                           const DevTables &t, const FitParams &fp, const uint2 *items, uint32_t nitems,
+                           uint32_t max_strikes, uint32_t max_obs, const Model *models, double *partials,
+                           uint32_t *counts, uint32_t *nosupport, RowError *errors, unsigned long long *sim_fail,
+                           const double *znorm) {
     if (nitems == 0) return hipSuccess;
+    // tangents carried: none; d/dS0 and d/dsigma (delta and vega, in every dividend model); all four
+    const int gk = greek_mask == 0 ? 0 : ((greek_mask & ~(FOHIP_GREEK_DELTA | FOHIP_GREEK_VEGA)) == 0 ? 2 : 4);
     if (fast)
-        return launch_eval_r<true>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
+        return launch_eval_r<true>(st, gk, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
                                     partials, counts, nosupport, errors, sim_fail, znorm);
-    return launch_eval_r<false>(st, greeks, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
+    return launch_eval_r<false>(st, gk, escrow, tile, t, fp, items, nitems, max_strikes, max_obs, models,
                                  partials, counts, nosupport, errors, sim_fail, nullptr);
 }
 

@@ -75,7 +75,7 @@ hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTable
 hipError_t launch_fit_reduce_origin(hipStream_t st, const DevTables &t, const FitParams &fp,
                                     const uint32_t *rows, uint32_t nrows, double *partials,
                                     Model *models, RowError *errors);
-hipError_t launch_evaluate(hipStream_t st, bool fast, bool greeks, bool escrow, uint32_t tile,
+hipError_t launch_evaluate(hipStream_t st, bool fast, uint32_t greek_mask, bool escrow, uint32_t tile,
                            const DevTables &t, const FitParams &fp, const uint2 *items,
                            uint32_t nitems, uint32_t max_strikes, uint32_t max_obs,
                            const Model *models, double *partials, uint32_t *counts,

@@ -253,7 +253,7 @@ typedef struct {
     uint32_t *observation_nodes;        /* [exercise_count + 1], caller buffer */
     uint32_t *exercise_counts;          /* [exercise_count + 1], caller buffer */
     fohip_node_diagnostic *diagnostics; /* [exercise_count], caller buffer */
-    double greeks[4];                   /* delta, vega, rho, dividend_yield (when requested) */
+    double greeks[4];                   /* delta, vega, rho, dividend_yield; NaN when not requested */
     double greek_standard_errors[4];
     /* Tied rule applications (see fohip_set_near_tie_window), training (backward pass) and
      * evaluation: by default the exact ties, whose outcome the tie rule settles. No counterpart in
