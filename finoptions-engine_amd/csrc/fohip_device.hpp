@@ -90,6 +90,8 @@ struct PathStream {
         has_spare = true;
         return static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32);
     }
+
+    __device__ __forceinline__ uint64_t first_u64() { return next_u64(); }
 };
 
 // The same stream with its draws staged in a per-lane ring of kRingSlots u64 in LDS. In a wave that
@@ -107,7 +109,9 @@ constexpr uint32_t kRingLow = 3;
 struct RingStream {
     uint32_t k0, k1, path, domain;
     uint32_t block_lo, block_hi;
-    uint32_t head, count;
+    // draws [head, tail) are in the ring; blocks arrive in pairs of draws, so tail is always even
+    // and a block's two draws sit in slots 2k and 2k + 1 (one paired LDS store)
+    uint32_t head, tail;
     uint64_t *slot;   // this lane's slots, slot[i * stride]
     uint32_t stride;
 
@@ -120,35 +124,43 @@ struct RingStream {
         path = path_id;
         domain = domain_id;
         block_lo = block_hi = 0;
-        head = count = 0;
+        head = tail = 0;
         slot = lds + lane;
         stride = columns;
     }
+
+    __device__ __forceinline__ uint32_t count() const { return tail - head; }
 
     __device__ __forceinline__ void produce() {
         const uint4 w = philox4x32_10(block_lo, block_hi, path, domain, k0, k1);
         block_lo += 1u;
         block_hi += block_lo == 0u ? 1u : 0u;
-        const uint32_t tail = head + count;
-        slot[(tail & (kRingSlots - 1)) * stride] = static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32);
-        slot[((tail + 1) & (kRingSlots - 1)) * stride] = static_cast<uint64_t>(w.z) | (static_cast<uint64_t>(w.w) << 32);
-        count += 2;
+        uint64_t *d = slot + (tail & (kRingSlots - 2)) * stride;
+        d[0] = static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32);
+        d[stride] = static_cast<uint64_t>(w.z) | (static_cast<uint64_t>(w.w) << 32);
+        tail += 2;
     }
 
     // Called by every active lane of the wave at the same point of its loop; `drawing` is false for
     // a lane that draws nothing more (past its population, or failed), which then neither asks
-    // for nor receives blocks.
+    // for nor receives blocks. Afterwards every drawing lane holds at least two draws.
     __device__ __forceinline__ void top_up(bool drawing) {
-        if (__ballot(drawing && count < kRingLow) != 0ull) {   // wave-uniform
-            if (drawing && count <= kRingSlots - 2) produce();
+        if (__ballot(drawing && count() < kRingLow) != 0ull) {   // wave-uniform
+            if (drawing && count() <= kRingSlots - 2) produce();
         }
     }
 
     __device__ __forceinline__ uint64_t next_u64() {
-        if (count == 0) produce();
+        if (count() == 0) produce();
+        return pop();
+    }
+
+    // The first draw after top_up(true): the ring is not empty.
+    __device__ __forceinline__ uint64_t first_u64() { return pop(); }
+
+    __device__ __forceinline__ uint64_t pop() {
         const uint64_t v = slot[(head & (kRingSlots - 1)) * stride];
         head += 1;
-        count -= 1;
         return v;
     }
 };
@@ -160,15 +172,19 @@ struct RingStream {
 // two-part Cody-Waite reduction (|r| <= ln2/64 + tiny); exp(r) - 1 by its Taylor polynomial of
 // degree 6 (truncation below 4e-18 relative); 2^(j/32) as a leading + trailing binary64 pair;
 // exp(x) = 2^m (T_j + T_j p(r) + t_j) with one final rounding. Within one ulp of the correctly
-// rounded value (glibc's exp agrees on 99.5 % of arguments in [-5, 5]). The reduction is exact for
-// |n| < 2^20, which covers |x| <= 745: there the final scaling overflows to +inf above ln(DBL_MAX)
-// and rounds into the subnormals below -708.4 (one extra rounding); beyond 745 the result is +inf
-// or +0, and NaN stays NaN. No libm call remains, so the loop keeps no libm constants in registers.
+// rounded value (glibc's exp agrees on 99.5 % of arguments in [-5, 5]). The argument is clamped to
+// [-746, 746], where the reduction is exact (|n| < 2^20): the final scaling overflows to +inf above
+// ln(DBL_MAX) and rounds into the subnormals below -708.4 (one extra rounding) and to +0 below about
+// -745.1, so an argument beyond the clamp gives the same +inf or +0 (and +-inf give +inf and +0).
+// The clamp is two min/max instructions where a range test would be a divergent block on every
+// call. The argument must not be NaN (the GBM step's and the wedge test's never are: both are
+// formed from finite tables and finite normals). No libm call remains, so the loop keeps no libm
+// constants in registers.
 struct ExpEntry {
     double hi, lo;
 };
 __device__ __forceinline__ double table_exp(double x, const ExpEntry *__restrict__ tbl) {
-    if (!(fabs(x) <= 745.0)) return x > 0.0 ? __builtin_huge_val() : (x < 0.0 ? 0.0 : x);
+    x = fmin(fmax(x, -746.0), 746.0);
     const double n = rint(x * FOHIP_EXP_INV_L);
     double r = fma(-n, FOHIP_EXP_L1, x);
     r = fma(-n, FOHIP_EXP_L2, r);
@@ -260,7 +276,7 @@ __device__ __forceinline__ bool zignor_normal(Stream &s, const double *__restric
                                               const float4 *__restrict__ SQ, const double *__restrict__ F,
                                               double zr, double &out, uint32_t &draws,
                                               const ExpEntry *__restrict__ et) {
-    const uint64_t bits = s.next_u64();
+    const uint64_t bits = s.first_u64();   // a ring stream is topped up before every normal
     draws += 1;
     const int index = static_cast<int>(bits & 0xff);
     const double unit = symmetric_unit(bits);

@@ -1966,7 +1966,7 @@ __global__ __launch_bounds__(kBlock) void k_gbm_exp(const double *__restrict__ x
     for (int i = threadIdx.x; i < 32; i += kBlock) se[i] = g_exp_table[i];
     __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) out[i] = table_exp(x[i], se);
+    if (i < n) out[i] = x[i] != x[i] ? x[i] : table_exp(x[i], se);   // NaN passes through, as in exp
 }
 
 hipError_t launch_gbm_exp(hipStream_t st, const double *x, double *out, uint32_t n) {
