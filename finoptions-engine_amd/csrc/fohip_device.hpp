@@ -222,7 +222,7 @@ __device__ __forceinline__ double standard_unit(uint64_t bits) {
 // The wedge test "F[i+1] + (F[i] - F[i+1]) u < exp(-value^2 / 2)" is decided first by the layer's
 // quadratic squeeze (SQ[i] = {1/w, c, gu, gl}, see gen_zig_tables.py): with t = (|value| - X[i+1]) / w
 // and q = t + c t (t - 1), u - q > gu means the test passes and u - q < -gl that it fails, with a
-// margin that covers the binary32 arithmetic below (u from the draw's top 24 bits) and every
+// margin that covers the binary32 arithmetic below (u from the draw's top 23 bits) and every
 // rounding of the exact test; only inside the band does the exact test run, reading F from global
 // memory. The outcome, and so every normal and draw count, is that of the exact test.
 template <typename Stream>
@@ -248,7 +248,9 @@ __device__ __forceinline__ bool zignor_rejected(Stream &s, int index, double uni
         const float4 sq = SQ[index];
         const float t = static_cast<float>(fabs(value) - X[index + 1]) * sq.x;
         const float q = fmaf(sq.y, t * (t - 1.0f), t);
-        const float d = static_cast<float>(static_cast<uint32_t>(ubits >> 40)) * 0x1p-24f - q;
+        // u to 23 bits, truncated: the top 23 bits as the mantissa of a binary32 in [1, 2)
+        const float u23 = __uint_as_float(0x3F800000u | (static_cast<uint32_t>(ubits >> 32) >> 9)) - 1.0f;
+        const float d = u23 - q;
         bool accept = d > sq.z;
         if (__builtin_expect(!accept && !(d < -sq.w), 0)) {
             const double ordinate = F[index + 1] + (F[index] - F[index + 1]) * standard_unit(ubits);

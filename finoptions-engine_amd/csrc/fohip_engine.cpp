@@ -654,6 +654,7 @@ struct GroupTables {
     std::vector<double> log_drift, diffusion, dt, sqrt_dt, cash, disc0, disc_step, row_time, row_escrow, row_escrow_dr;
     std::vector<float> log_drift32, diffusion32;
     std::vector<uint32_t> node_flags, row_node;
+    std::vector<StepRec> steps;
 };
 
 // Host-side discount checks (policy.rs:1081-1098): every factor used by the fit and evaluation
@@ -778,6 +779,24 @@ static int build_tables(const std::vector<GroupPlan> &plans, const std::vector<d
             bad_node = 0;
         }
         g.disc_final = dfinal;
+        // the generation kernels' packed form of the same tables
+        for (uint32_t to = 1, r = g.origin_row; to < nodes; ++to) {
+            StepRec rec;
+            std::memset(&rec, 0, sizeof rec);
+            const size_t si = g.step_base + to - 1, ni = g.node_base + to;
+            rec.log_drift = gt.log_drift[si];
+            rec.diffusion = gt.diffusion[si];
+            rec.dt = gt.dt[si];
+            rec.sqrt_dt = gt.sqrt_dt[si];
+            rec.cash = gt.cash[ni];
+            rec.flags = gt.node_flags[ni];
+            if (rec.flags & kNodeObserved) {
+                rec.disc0 = gt.disc0[g.row_base + r];
+                rec.row_time = gt.row_time[g.row_base + r];
+                ++r;
+            }
+            gt.steps.push_back(rec);
+        }
         gt.groups.push_back(g);
     }
     (void)err;
@@ -888,7 +907,7 @@ struct BatchRun {
     uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
     std::vector<uint2> items_t, items_e, items_b;
     std::vector<uint32_t> rows_sorted, step_items, step_rows, step_items_b;
-    static constexpr int kTableParts = 22;
+    static constexpr int kTableParts = 23;
     size_t part_bytes[kTableParts] = {};
     const void *part_host[kTableParts] = {};
     size_t part_off[kTableParts] = {};
@@ -1099,14 +1118,14 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             gt.row_time.size() * 8, gt.row_node.size() * 4, R.bstrikes.size() * 8, R.strike_group.size() * 4,
             gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, R.binv.size() * 8,
             R.items_t.size() * sizeof(uint2), R.items_e.size() * sizeof(uint2), R.items_b.size() * sizeof(uint2),
-            R.rows_sorted.size() * sizeof(uint32_t)};
+            R.rows_sorted.size() * sizeof(uint32_t), gt.steps.size() * sizeof(StepRec)};
         const void *host[BatchRun::kTableParts] = {
             gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
             gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
             gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
             gt.row_node.data(), R.bstrikes.data(), R.strike_group.data(),
             gt.row_escrow.data(), gt.row_escrow_dr.data(), R.binv.data(),
-            R.items_t.data(), R.items_e.data(), R.items_b.data(), R.rows_sorted.data()};
+            R.items_t.data(), R.items_e.data(), R.items_b.data(), R.rows_sorted.data(), gt.steps.data()};
         for (int i = 0; i < BatchRun::kTableParts; ++i) {
             R.part_bytes[i] = bytes[i];
             R.part_host[i] = host[i];
@@ -1194,6 +1213,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         R.ite = reinterpret_cast<const uint2 *>(tb + o[19]);
         R.itb = reinterpret_cast<const uint2 *>(tb + o[20]);
         R.rows = reinterpret_cast<const uint32_t *>(tb + o[21]);
+        t.steps = reinterpret_cast<const StepRec *>(tb + o[22]);
         // rules zeroed, exercise and no-support counters zeroed, failure keys set to "none"
         static_assert(sizeof(Model) % 4 == 0 && sizeof(RowError) % 4 == 0, "fills write 32-bit words");
         FillList fills = {};
@@ -2305,12 +2325,14 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
         }
     }
     const size_t tb_bytes[] = {sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8, gt.log_drift32.size() * 4,
-                               gt.diffusion32.size() * 4, gt.cash.size() * 8, gt.node_flags.size() * 4};
+                               gt.diffusion32.size() * 4, gt.cash.size() * 8, gt.node_flags.size() * 4,
+                               gt.steps.size() * sizeof(StepRec)};
     const void *tb_src[] = {gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
-                            gt.diffusion32.data(), gt.cash.data(), gt.node_flags.data()};
+                            gt.diffusion32.data(), gt.cash.data(), gt.node_flags.data(), gt.steps.data()};
+    constexpr int kParts = 8;
     size_t total = 0;
-    size_t offs[7];
-    for (int i = 0; i < 7; ++i) {
+    size_t offs[kParts];
+    for (int i = 0; i < kParts; ++i) {
         offs[i] = total;
         total += (tb_bytes[i] + 255) & ~size_t(255);
     }
@@ -2331,7 +2353,7 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
         (void)hipEventDestroy(ready);
         HIP_TRY(e);
     }
-    for (int i = 0; i < 7; ++i) HIP_TRY(hipMemcpyAsync(tb + offs[i], tb_src[i], tb_bytes[i], hipMemcpyHostToDevice, st));
+    for (int i = 0; i < kParts; ++i) HIP_TRY(hipMemcpyAsync(tb + offs[i], tb_src[i], tb_bytes[i], hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ex->items_t.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(ex->slot[0].fail_t.ptr, 0xFF, 8, st));
     DevTables t;
@@ -2343,6 +2365,7 @@ static int simulate_export(fohip_executor *ex, const fohip_path_plan *pl, double
     t.diffusion32 = reinterpret_cast<const float *>(tb + offs[4]);
     t.cash = reinterpret_cast<const double *>(tb + offs[5]);
     t.node_flags = reinterpret_cast<const uint32_t *>(tb + offs[6]);
+    t.steps = reinterpret_cast<const StepRec *>(tb + offs[7]);
     HIP_TRY(launch_simulate(st, f32, t, ex->items_t.as<uint2>(), static_cast<uint32_t>(items.size()), dst, 0,
                             ex->slot[0].fail_t.as<unsigned long long>()));
     unsigned long long failed = 0;

@@ -151,24 +151,24 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
             break;
         }
         double growth;
-        const double next = gbm_advance(spot, t.diffusion[g.step_base + i], t.log_drift[g.step_base + i], z, growth, se);
+        const FOHIP_UNIFORM StepRec *sr = uniform(t.steps + g.step_base + i);
+        const double next = gbm_advance(spot, sr->diffusion, sr->log_drift, z, growth, se);
         if (!isfinite(next)) {
             ok = false;
             break;
         }
         spot = next;
-        const uint32_t node = g.node_base + i + 1;
-        const uint32_t flags = t.node_flags[node];
+        const uint32_t flags = sr->flags;
         const bool cash = flags & kNodeCash;
         const bool terminal = flags & kNodeTerminal;
-        if (cash && terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
+        if (cash && terminal) spot = apply_cash<double>(g.policy, sr->cash, spot);
         if (flags & kNodeObserved) {
             // escrowed groups observe F + A(node) (path.rs:844-880); A is 0 at expiry
             const double observed = g.escrowed ? spot + t.row_escrow[g.row_base + row] : spot;
             out[static_cast<uint64_t>(row) * n] = static_cast<Store>(observed);
             row += 1;
         }
-        if (cash && !terminal) spot = apply_cash<double>(g.policy, t.cash[node], spot);
+        if (cash && !terminal) spot = apply_cash<double>(g.policy, sr->cash, spot);
     }
     if (!ok) atomicMin(fail + item.x, static_cast<unsigned long long>(p));
 }
@@ -1438,7 +1438,7 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
     if (s_first >= strike_end) return;
     const int ns = static_cast<int>(min(static_cast<uint32_t>(TS), strike_end - s_first));
     const uint32_t obs = g->obs, last = obs - 1, intervals = g->intervals, n = g->ne;
-    const uint32_t step_base = g->step_base, node_base = g->node_base, row_base = g->row_base;
+    const uint32_t step_base = g->step_base, row_base = g->row_base;
     const uint32_t origin_row = g->origin_row, count_base = g->count_base;
     const int policy = g->policy;
     const double sigma = g->sigma;
@@ -1523,9 +1523,9 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             s.top_up(ok);
             if (ok && !zignor_normal(s, sx, ssq, g_zig_f, FOHIP_ZIG_R, z, draws, se)) ok = false;
         }
-        const uint32_t si = step_base + iv;
+        const FOHIP_UNIFORM StepRec *sr = uniform(t.steps + step_base + iv);
         double growth;
-        const double next = gbm_advance(spot, *uniform(t.diffusion + si), *uniform(t.log_drift + si), z, growth, se);
+        const double next = gbm_advance(spot, sr->diffusion, sr->log_drift, z, growth, se);
         if (!isfinite(next)) ok = false;
         if (!ok) active = 0u;
         const double before = spot;
@@ -1534,27 +1534,26 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
             if (before == 0.0) {
                 tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
             } else {
-                const double dt = *uniform(t.dt + si);
+                const double dt = sr->dt;
                 tan[0] = tan[0] * growth;
-                tan[1] = fma(tan[1], growth, spot * fma(*uniform(t.sqrt_dt + si), z, -sigma * dt));
+                tan[1] = fma(tan[1], growth, spot * fma(sr->sqrt_dt, z, -sigma * dt));
                 if (GREEKS == 4) {
                     tan[2] = fma(tan[2], growth, spot * dt);
                     tan[3] = fma(tan[3], growth, -spot * dt);
                 }
             }
         }
-        const uint32_t node = node_base + iv + 1;
-        const uint32_t flags = *uniform(t.node_flags + node);
+        const uint32_t flags = sr->flags;
         const bool cash = flags & kNodeCash;
         const bool terminal = flags & kNodeTerminal;
-        const double amount = cash ? *uniform(t.cash + node) : 0.0;
+        const double amount = sr->cash;   // 0 where the node pays none
         if (cash && terminal) {
             if (GREEKS && policy == 0 && !(spot > amount)) tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
             spot = apply_cash<double>(policy, amount, spot);
         }
         if ((flags & kNodeObserved) && row < last) {
-            const double disc = *uniform(t.disc0 + row_base + row);
-            const double t_row = GREEKS ? *uniform(t.row_time + row_base + row) : 0.0;
+            const double disc = sr->disc0;   // this row's (the record of an observed node holds it)
+            const double t_row = GREEKS ? sr->row_time : 0.0;
             // the exercise decision observes the stock: F + A(node) for escrowed groups
             const double S = ESCROW ? spot + *uniform(t.row_escrow + row_base + row) : spot;
             // discounted observed tangents, shared by every strike that stops on this row: with

@@ -114,8 +114,23 @@ constexpr uint32_t kNodeObserved = 1u;
 constexpr uint32_t kNodeTerminal = 2u;
 constexpr uint32_t kNodeCash = 4u;
 
+// One interval of a group as the generation kernels read it, in one 64-byte scalar load: the
+// transition coefficients of interval i (node i -> i + 1), node i + 1's flags and cash amount and,
+// when node i + 1 is an observation row, that row's discount factor and time. The same values as
+// the separate tables below; the kernels that step paths read this form so that their loop holds
+// one table pointer instead of eight.
+struct alignas(64) StepRec {
+    double log_drift, diffusion, dt, sqrt_dt;
+    double cash;
+    double disc0, row_time;
+    uint32_t flags;
+    uint32_t pad;
+};
+static_assert(sizeof(StepRec) == 64, "one scalar load per interval");
+
 struct DevTables {
     const DevGroup *groups;
+    const StepRec *steps;         // [interval]
     const double *log_drift;      // [interval]
     const double *diffusion;      // [interval]
     const float *log_drift32;

@@ -77,7 +77,7 @@ def test_squeeze_decisions_agree_with_the_exact_wedge_test():
         t = (x - a).astype(np.float32) * inv_w
         q = np.float32(c) * (t * (t - np.float32(1.0))) + t   # an FMA in the kernel; the
         # separate rounding here is within the slack
-        uf = (ubits >> np.uint64(40)).astype(np.uint32).astype(np.float32) * np.float32(2.0 ** -24)
+        uf = (ubits >> np.uint64(41)).astype(np.uint32).astype(np.float32) * np.float32(2.0 ** -23)
         d = uf - q
         ordinate = fs[i + 1] + (fs[i] - fs[i + 1]) * u
         exact = ordinate < np.exp(-0.5 * x * x)
