@@ -18,6 +18,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fohip.h"
@@ -317,10 +318,13 @@ namespace {
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
+    // Grows the buffer. A batch queued behind another may still be reading the old buffer, so the
+    // device drains before it is released (growth happens on the first requests of a shape only).
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap) return hipSuccess;
         if (ptr) {
-            hipError_t e = hipFree(ptr);
+            hipError_t e = hipDeviceSynchronize();
+            if (e == hipSuccess) e = hipFree(ptr);
             if (e != hipSuccess) return e;
             ptr = nullptr;
             cap = 0;
@@ -895,7 +899,8 @@ namespace {
 // One memory-bounded batch of consecutive groups on its way through the device: the host tables
 // built for it, the slot buffers it claimed and the host copies its results come back into.
 struct BatchRun {
-    int slot = 0;
+    int slot = 0;   // device buffers
+    int io = 0;     // events and page-locked staging
     std::vector<GroupPlan> plans;
     uint32_t row0 = 0, nrows = 0, ngroups = 0;
     std::vector<double> bstrikes, binv;
@@ -1055,6 +1060,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         auto B = std::make_unique<BatchRun>();
         BatchRun &R = *B;
         R.slot = overlapped ? static_cast<int>(b & 1) : 0;
+        R.io = fixed ? 0 : static_cast<int>(b & 1);
         const auto [gb, ge] = batches[b];
         R.plans.assign(all.begin() + gb, all.begin() + ge);
         // strike rows of this batch are contiguous: rebase to the batch
@@ -1153,6 +1159,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
     // ---- slot buffers, upload and simulation (generation stream); backward induction (fit stream)
     auto issue_front = [&](BatchRun &R) -> int {
         BatchSlot &S = ex->slot[R.slot];
+        BatchSlot &E = ex->slot[R.io];
         // every buffer of the batch is claimed (and a fixed workspace found large enough) before
         // any work is queued
         ws.begin_batch();
@@ -1184,7 +1191,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         // staging area too, after the tables
         size_t staged = R.tables_bytes;
         for (const auto &d : R.downloads()) staged += (d.bytes + 255) & ~size_t(255);
-        HIP_TRY(ws.host_staging(S.staging, staged, &R.staging));
+        HIP_TRY(ws.host_staging(E.staging, staged, &R.staging));
         for (int i = 0; i < BatchRun::kTableParts; ++i)
             if (R.part_bytes[i]) std::memcpy(R.staging + R.part_off[i], R.part_host[i], R.part_bytes[i]);
         char *tb = static_cast<char *>(R.tables_d);
@@ -1234,8 +1241,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         PROF_LAUNCH(gen_st, "k_simulate",
                     launch_simulate(gen_st, f32, t, R.it, static_cast<uint32_t>(R.items_t.size()), R.paths_d, 0,
                                     static_cast<unsigned long long *>(R.fail_t_d)));
-        HIP_TRY(hipEventRecord(S.simulated, gen_st));
-        HIP_TRY(hipStreamWaitEvent(fit_st, S.simulated, 0));
+        HIP_TRY(hipEventRecord(E.simulated, gen_st));
+        HIP_TRY(hipStreamWaitEvent(fit_st, E.simulated, 0));
         // ---- backward induction
         uint8_t *stops = use_stops ? static_cast<uint8_t *>(R.stops_d) : nullptr;
         unsigned long long *ties_t = static_cast<unsigned long long *>(R.ties_d);
@@ -1276,18 +1283,18 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
                                      R.max_strikes, R.paths_d, R.values_d, stops, models, part_t, errors, ties_t));
         PROF_LAUNCH(fit_st, "k_fit_reduce_origin",
                     launch_fit_reduce_origin(fit_st, t, fp, R.rows, R.nrows, part_t, models, errors));
-        HIP_TRY(hipEventRecord(S.fitted, fit_st));
+        HIP_TRY(hipEventRecord(E.fitted, fit_st));
         return FOHIP_OK;
     };
 
     // ---- evaluation and downloads (generation stream, after the batch's backward induction)
     auto issue_back = [&](BatchRun &R) -> int {
-        BatchSlot &S = ex->slot[R.slot];
+        BatchSlot &E = ex->slot[R.io];
         const DevTables &t = R.t;
         FitParams fp = spec.fp;
         fp.tie_window = ex->near_tie_window;
         Model *models = static_cast<Model *>(R.models_d);
-        HIP_TRY(hipStreamWaitEvent(gen_st, S.fitted, 0));
+        HIP_TRY(hipStreamWaitEvent(gen_st, E.fitted, 0));
         const double *znorm = nullptr;
         if (R.stored_normals) {
             PROF_LAUNCH(gen_st, "k_eval_normals",
@@ -1316,14 +1323,13 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             if (d.bytes) HIP_TRY(hipMemcpyAsync(R.staging + off, *d.dev, d.bytes, hipMemcpyDeviceToHost, gen_st));
             off += (d.bytes + 255) & ~size_t(255);
         }
-        HIP_TRY(hipEventRecord(S.done, gen_st));
+        HIP_TRY(hipEventRecord(E.done, gen_st));
         return FOHIP_OK;
     };
 
     // ---- results of a finished batch: failures in the reference's processing order, row outputs
     auto finish = [&](BatchRun &R) -> int {
-        BatchSlot &S = ex->slot[R.slot];
-        HIP_TRY(hipEventSynchronize(S.done));
+        HIP_TRY(hipEventSynchronize(ex->slot[R.io].done));
         HIP_TRY(hipGetLastError());
         {
             size_t off = R.tables_bytes;
@@ -1477,12 +1483,23 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
 
     int rc = FOHIP_OK;
     if (!overlapped) {
-        // the host builds batch b + 1's tables while the device runs batch b
+        // The host builds batch b + 1 and queues it behind batch b before it waits for b's
+        // results, so the device goes from one batch to the next without waiting for the host.
+        // Consecutive batches use the same device buffers, ordered by the generation stream (b + 1's
+        // upload and simulation follow b's evaluation, which follows b's backward induction), and
+        // alternate between the two slots' events and page-locked staging. The fixed region of the
+        // allocation-free entry point has one staging area: there b + 1 is queued after b is done.
+        const bool pipelined = !fixed;
         std::unique_ptr<BatchRun> cur = prepare(0);
+        if ((rc = issue_front(*cur)) || (rc = issue_back(*cur))) return rc;
         for (size_t b = 0; b < batches.size(); ++b) {
-            if ((rc = issue_front(*cur)) || (rc = issue_back(*cur))) return rc;
             std::unique_ptr<BatchRun> next = b + 1 < batches.size() ? prepare(b + 1) : nullptr;
+            int rc_next = FOHIP_OK;
+            if (next && pipelined && !(rc_next = issue_front(*next))) rc_next = issue_back(*next);
+            // a failure to queue b + 1 is reported after batch b's own results, which precede it
             if ((rc = finish(*cur))) return rc;
+            if (rc_next) return rc_next;
+            if (next && !pipelined && ((rc = issue_front(*next)) || (rc = issue_back(*next)))) return rc;
             cur = std::move(next);
         }
     } else {
@@ -1960,7 +1977,10 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
     if (rq->mesh_kind < 0 || rq->mesh_kind > 2) return fail(err, FOHIP_E_EXERCISE_MESH, 0, "unknown exercise mesh");
     strikes.assign(rq->strikes, rq->strikes + rq->rows);
     plans.resize(groups);
-    for (size_t g = 0; g < groups; ++g) {
+    // One group's grid, curves and schedule (batch/pricing.rs:840-965). Groups are independent, so
+    // large requests build them on several host threads; the error reported is the lowest failing
+    // group's, rebuilt serially so that its message is the one a serial pass would give.
+    auto build_one = [&](size_t g, fohip_error *err) -> int {
         GroupPlan &p = plans[g];
         const double T = rq->maturities[g], rate = rq->rates[g];
         std::vector<double> cash_t, cash_a;
@@ -2127,7 +2147,37 @@ static int build_grouped_plans(const fohip_grouped_request *rq, std::vector<Grou
         p.eval_seed = rq->evaluation_seed;
         p.train_domain = rq->training_domain_start + static_cast<uint32_t>(g);
         p.eval_domain = rq->evaluation_domain_start + static_cast<uint32_t>(g);
+        return FOHIP_OK;
+    };
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t nthreads = groups >= 1024 ? std::min<size_t>({hw, 16, groups / 256}) : 1;
+    if (nthreads <= 1) {
+        for (size_t g = 0; g < groups; ++g) {
+            const int rc = build_one(g, err);
+            if (rc) return rc;
+        }
+        return FOHIP_OK;
     }
+    std::vector<size_t> first_failure(nthreads, groups);
+    {
+        std::vector<std::thread> pool;
+        pool.reserve(nthreads);
+        for (size_t k = 0; k < nthreads; ++k) {
+            pool.emplace_back([&, k]() {
+                fohip_error local;
+                std::memset(&local, 0, sizeof local);
+                const size_t g0 = groups * k / nthreads, g1 = groups * (k + 1) / nthreads;
+                for (size_t g = g0; g < g1; ++g)
+                    if (build_one(g, &local)) {
+                        first_failure[k] = g;
+                        return;
+                    }
+            });
+        }
+        for (std::thread &t : pool) t.join();
+    }
+    for (size_t k = 0; k < nthreads; ++k)
+        if (first_failure[k] < groups) return build_one(first_failure[k], err);
     return FOHIP_OK;
 }
 
