@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     double spot = g.start;
     uint32_t row = 0;
     if (g.origin_row) {
-        out[0] = static_cast<Store>(g.S0);
+        __builtin_nontemporal_store(static_cast<Store>(g.S0), out);
         row = 1;
     }
     bool ok = true;
@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
         if (flags & kNodeObserved) {
             // escrowed groups observe F + A(node) (path.rs:844-880); A is 0 at expiry
             const double observed = g.escrowed ? spot + t.row_escrow[g.row_base + row] : spot;
-            out[static_cast<uint64_t>(row) * n] = static_cast<Store>(observed);
+            // a streaming store: the matrix is read back only by the backward pass, rows in reverse
+            __builtin_nontemporal_store(static_cast<Store>(observed), out + static_cast<uint64_t>(row) * n);
             row += 1;
         }
         if (cash && !terminal) spot = apply_cash<double>(g.policy, sr->cash, spot);
