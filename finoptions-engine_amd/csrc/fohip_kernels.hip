@@ -793,7 +793,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_bulk_step(DevTables t, FitPar
     for (int k = 0; k < kBulkPathsPerThread; ++k) {
         const uint32_t p = base + k * kBlock;
         s0v[k] = p < n ? S_r[p] : Store(0);
-        s1v[k] = p < n ? S_next[p] : Store(0);
+        // row r + 1 is read here for the last time (it was row r of the previous step): a streaming
+        // load leaves the caches to row r and the cashflows, which the next step reads again
+        s1v[k] = p < n ? __builtin_nontemporal_load(S_next + p) : Store(0);
     }
     const uint32_t row_base = g->row_base;
     const double d_next = *uniform(t.disc0 + row_base + r + 1);
@@ -981,7 +983,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_fit_stop_step(DevTables t, FitPar
     for (int k = 0; k < kBulkPathsPerThread; ++k) {
         const uint32_t p = base + k * kBlock;
         s0v[k] = p < n ? S_r[p] : Store(0);
-        s1v[k] = p < n ? S_next[p] : Store(0);
+        // row r + 1 is read here for the last time (it was row r of the previous step): a streaming
+        // load leaves the caches to row r and the cashflows, which the next step reads again
+        s1v[k] = p < n ? __builtin_nontemporal_load(S_next + p) : Store(0);
     }
     const double d_next = *uniform(t.disc0 + row_base + r + 1);
     const double inv_d = 1.0 / *uniform(t.disc0 + row_base + r);
