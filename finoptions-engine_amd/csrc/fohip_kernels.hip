@@ -127,22 +127,32 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
     __shared__ uint64_t ring[kRingSlots * kBlock];
     load_zig_exp(sx, ssq, se);
     const uint2 item = items[blockIdx.x];
-    const DevGroup &g = t.groups[item.x];
-    const uint32_t n = evaluation ? g.ne : g.nt;
+    // the group's fields as wave-uniform values read once: through a plain reference the compiler
+    // reloads them after every store of the path loop
+    const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + item.x);
+    const uint32_t n = evaluation ? g->ne : g->nt;
     const uint32_t p = item.y * kChunk + blockIdx.y * kBlock + threadIdx.x;
     if (p >= n) return;
     RingStream s;
-    s.init(evaluation ? g.eval_seed : g.train_seed, evaluation ? g.eval_domain : g.train_domain, p, ring,
+    s.init(evaluation ? g->eval_seed : g->train_seed, evaluation ? g->eval_domain : g->train_domain, p, ring,
            threadIdx.x, kBlock);
-    Store *out = paths_out + g.path_base + p;
-    double spot = g.start;
+    const uint32_t intervals = g->intervals, row_base = g->row_base;
+    const int policy = g->policy;
+    const bool escrowed = g->escrowed != 0;
+    const FOHIP_UNIFORM StepRec *sr = uniform(t.steps + g->step_base);
+    Store *out = paths_out + g->path_base + p;
+    double spot = g->start;
     uint32_t row = 0;
-    if (g.origin_row) {
-        __builtin_nontemporal_store(static_cast<Store>(g.S0), out);
+    if (g->origin_row) {
+        __builtin_nontemporal_store(static_cast<Store>(g->S0), out);
         row = 1;
     }
     bool ok = true;
-    for (uint32_t i = 0; i < g.intervals; ++i) {
+    for (uint32_t i = 0; i < intervals; ++i, ++sr) {
+        // the interval's record, asked for before the normal is drawn so that the scalar loads
+        // complete while it is
+        const double diffusion = sr->diffusion, log_drift = sr->log_drift, amount = sr->cash;
+        const uint32_t flags = sr->flags;
         double z;
         uint32_t draws = 0;
         s.top_up(true);
@@ -151,25 +161,23 @@ __global__ __launch_bounds__(kBlock) void k_simulate(DevTables t, const uint2 *_
             break;
         }
         double growth;
-        const FOHIP_UNIFORM StepRec *sr = uniform(t.steps + g.step_base + i);
-        const double next = gbm_advance(spot, sr->diffusion, sr->log_drift, z, growth, se);
+        const double next = gbm_advance(spot, diffusion, log_drift, z, growth, se);
         if (!isfinite(next)) {
             ok = false;
             break;
         }
         spot = next;
-        const uint32_t flags = sr->flags;
         const bool cash = flags & kNodeCash;
         const bool terminal = flags & kNodeTerminal;
-        if (cash && terminal) spot = apply_cash<double>(g.policy, sr->cash, spot);
+        if (cash && terminal) spot = apply_cash<double>(policy, amount, spot);
         if (flags & kNodeObserved) {
             // escrowed groups observe F + A(node) (path.rs:844-880); A is 0 at expiry
-            const double observed = g.escrowed ? spot + t.row_escrow[g.row_base + row] : spot;
+            const double observed = escrowed ? spot + t.row_escrow[row_base + row] : spot;
             // a streaming store: the matrix is read back only by the backward pass, rows in reverse
             __builtin_nontemporal_store(static_cast<Store>(observed), out + static_cast<uint64_t>(row) * n);
             row += 1;
         }
-        if (cash && !terminal) spot = apply_cash<double>(g.policy, sr->cash, spot);
+        if (cash && !terminal) spot = apply_cash<double>(policy, amount, spot);
     }
     if (!ok) atomicMin(fail + item.x, static_cast<unsigned long long>(p));
 }
