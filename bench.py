@@ -40,9 +40,9 @@ C5_GROUPS, C5_STRIKES, C5_PATHS = 4096, 256, 32_768
 BUMP = 1e-4
 # HBM bytes per fit-step launch of the headline pricing, measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a copy-kernel calibration (2.000 x FETCH_SIZE, 1.000 x WRITE_SIZE
-# KiB) and averaged over the pricing's 3011 launches: 897.9 MB read + 36.6 MB written.
+# KiB) and averaged over the pricing's 3011 launches: 902.3 MB read + 36.6 MB written.
 MEASURED_TRAFFIC: dict[str, dict] = {
-    "k_fit_bulk_step": {"bytes_per_launch": 934_500_000,
+    "k_fit_bulk_step": {"bytes_per_launch": 938_900_000,
                         "source": "profiles/c4_ragged_batch_hbm_traffic.md"},
 }
 
