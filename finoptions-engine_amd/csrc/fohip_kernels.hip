@@ -1544,7 +1544,10 @@ __global__ __launch_bounds__(kBlock, (eval_waves<FAST, TS, GREEKS, STORED>())) v
         const double before = spot;
         spot = ok ? next : spot;
         if (GREEKS) {
-            if (before == 0.0) {
+            // an absorbed spot model path (S = 0) is never in the money again (K > 0), so its
+            // tangents are never read and need no clearing; an escrowed path at F = 0 still
+            // observes the prepaid cash and must carry zero tangents
+            if (ESCROW && before == 0.0) {
                 tan[0] = tan[1] = tan[2] = tan[3] = 0.0;
             } else {
                 const double dt = sr->dt;
