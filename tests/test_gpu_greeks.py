@@ -51,6 +51,70 @@ def test_greeks_do_not_change_prices(hip_f64):
     np.testing.assert_array_equal(plain.early_exercise_paths, with_greeks.early_exercise_paths)
 
 
+def _cash(cols: dict) -> dict:
+    """The same state with two liquidator cash dividends in place of the continuous yield."""
+    cols = dict(cols)
+    cols.pop("yields")
+    cols["cash_offsets"] = np.array([0, 2], dtype=np.int64)
+    cols["cash_times"] = np.array([0.3, 0.7])
+    cols["cash_amounts"] = np.array([2.0, 3.0])
+    return cols
+
+
+@pytest.mark.parametrize("executor", ["hip_f64", "hip_fused"])
+def test_delta_and_vega_alone_equal_those_of_all_four_greeks(executor, request):
+    """A request for delta and vega alone carries two tangents per path instead of four; the two
+    Greeks must be bitwise those of a request for all four: on a single-strike group (the
+    two-tangent form) with cash dividends, on one with a continuous yield, on a multi-strike
+    state, and through the single-call entry point."""
+    import finoptions_hip as fh
+    from conftest import c1_inputs
+
+    ex = request.getfixturevalue(executor)
+    both = (fh.Greek.DELTA, fh.Greek.VEGA)
+    all4 = tuple(getattr(fh.Greek, g) for g in ALL_GREEKS)
+    for cols in (_cash(c2_columns(strikes=1)), c2_columns(strikes=1), c2_columns(strikes=6)):
+        greeks = all4 if "yields" in cols else (*both, fh.Greek.RHO)
+        two = _price(fh, ex, cols, 16_384, both, intervals=12)
+        four = _price(fh, ex, cols, 16_384, greeks, intervals=12)
+        for g in both:
+            np.testing.assert_array_equal(two.greeks[g], four.greeks[g])
+            np.testing.assert_array_equal(two.greek_standard_errors[g],
+                                          four.greek_standard_errors[g])
+        np.testing.assert_array_equal(two.prices, four.prices)
+    inp = c1_inputs()
+    method = hip_method(8192, 8192, 7, support=fh.InsufficientExerciseSupportPolicy.CONTINUE)
+    two = fh.price_lsm_call(**inp, executor=ex, method=method, greeks=both)
+    four = fh.price_lsm_call(**inp, executor=ex, method=method, greeks=(*both, fh.Greek.RHO))
+    assert [g for g, _, _ in two.greeks] == list(both)
+    for g in both:
+        assert two.greek(g) == four.greek(g)
+        assert two.greek_standard_error(g) == four.greek_standard_error(g)
+
+
+def test_unrequested_greeks_are_nan_in_the_c_result(hip_fused, monkeypatch):
+    """fohip_price_lsm_call fills only the requested Greeks and writes NaN into the others."""
+    import finoptions_hip as fh
+    from conftest import c1_inputs
+    from finoptions_hip import lsm as lsm_module
+
+    seen = []
+
+    class Recording(lsm_module.FohipCallResult):
+        def __init__(self):
+            super().__init__()
+            seen.append(self)
+
+    monkeypatch.setattr(lsm_module, "FohipCallResult", Recording)
+    res = fh.price_lsm_call(**c1_inputs(), executor=hip_fused,
+                            method=hip_method(4096, 4096, 7), greeks=(fh.Greek.VEGA,))
+    assert [g for g, _, _ in res.greeks] == [fh.Greek.VEGA]
+    (raw,) = seen
+    assert np.isfinite(raw.greeks[1]) and raw.greeks[1] == res.greek(fh.Greek.VEGA)
+    for i in (0, 2, 3):
+        assert np.isnan(raw.greeks[i]) and np.isnan(raw.greek_standard_errors[i])
+
+
 def _oracle_vjp(oracle, req, rows_per_group, weights, names):
     """Per-group sums of weight x the oracle's pathwise sensitivity, group by group."""
     out = {n: [] for n in names}
