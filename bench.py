@@ -290,6 +290,9 @@ def main() -> int:
                     default="auto", help="cashflow encoding of the fused fit step")
     ap.add_argument("--batch-schedule", choices=("serial", "overlapped"), default="serial",
                     help="how the memory-bounded batches of a request share the device")
+    ap.add_argument("--fit-schedule", choices=("auto", "per_node", "sweep"), default="auto",
+                    help="launch form of the fused backward induction: one fit pass per exercise "
+                         "row, or one persistent sweep per batch (single-strike groups)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the reference copy for the rocprofv3 FETCH_SIZE / WRITE_SIZE scale "
                          "factors: three elementwise copies of 256 MiB (64 Mi float32)")
@@ -305,7 +308,8 @@ def main() -> int:
     import finoptions_hip as fh
 
     fast = fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
-    fast.tune(fit_cashflows=args.fit_cashflows, batch_schedule=args.batch_schedule)
+    fast.tune(fit_cashflows=args.fit_cashflows, batch_schedule=args.batch_schedule,
+              fit_schedule=args.fit_schedule)
     greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
     if args.only in ("c2", "c5"):
         if args.only == "c2":

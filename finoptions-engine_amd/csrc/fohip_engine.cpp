@@ -381,12 +381,12 @@ struct PinnedBuf {
 // (fit stream), results downloaded (generation stream).
 struct BatchSlot {
     DevBuf paths, values, stops, models, part_t, part_e, counts, nosup, errors, fail_t, fail_e, tables, eval_out,
-        summary, replay, znorm, ties;
+        summary, replay, znorm, ties, sweep_sync, sweep_part;
     PinnedBuf staging;   // the batch's tables and items on the way in, results on the way out
     hipEvent_t simulated = nullptr, fitted = nullptr, done = nullptr;
     void release() {
         for (DevBuf *b : {&paths, &values, &stops, &models, &part_t, &part_e, &counts, &nosup, &errors, &fail_t,
-                          &fail_e, &tables, &eval_out, &summary, &replay, &znorm, &ties})
+                          &fail_e, &tables, &eval_out, &summary, &replay, &znorm, &ties, &sweep_sync, &sweep_part})
             b->release();
         staging.release();
         for (hipEvent_t e : {simulated, fitted, done})
@@ -403,12 +403,15 @@ struct fohip_executor {
     hipStream_t stream = nullptr;   // backward induction; every single-stream entry point
     hipStream_t gen = nullptr;      // uploads, simulation, evaluation and downloads of batches
     char name[256] = {0};
-    fohip_tuning tuning = {0, 0, 0, 0};
+    fohip_tuning tuning = {0, 0, 0, 0, 0};
     double near_tie_window = 0.0;   // 0: exact ties; w > 0: |intrinsic - continuation| <= w |continuation|
     size_t batch_budget = 0;   // default bytes for paths + values of a batch (serial schedule)
     std::mutex mu;
     BatchSlot slot[2];
     DevBuf items_t, probe_a, probe_b, probe_c;
+    // blocks of the fit sweep the device holds at once, per (binary32 storage, windowed ties);
+    // -1 until first asked
+    int64_t sweep_slots[2][2] = {{-1, -1}, {-1, -1}};
     // host copies from the last grouped call, kept for fohip_grouped_diagnostics
     std::vector<Model> last_models;
     std::vector<uint32_t> last_model_base, last_obs, last_strike_begin, last_strike_end;
@@ -596,7 +599,7 @@ extern "C" int fohip_reset_profile(fohip_executor *ex) {
 extern "C" int fohip_set_tuning(fohip_executor *ex, const fohip_tuning *t) {
     if (!ex || !t) return FOHIP_E_INVALID_ARGUMENT;
     if (t->eval_strike_tile > 16 || t->fit_cashflows > FOHIP_CASHFLOWS_STOPPING_ROWS ||
-        t->batch_schedule > FOHIP_BATCHES_OVERLAPPED)
+        t->batch_schedule > FOHIP_BATCHES_OVERLAPPED || t->fit_schedule > FOHIP_FIT_SWEEP)
         return FOHIP_E_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(ex->mu);
     ex->tuning = *t;
@@ -896,6 +899,87 @@ struct Workspace {
 
 namespace {
 
+// ---- fit sweep schedule
+//
+// A group of c fused-fit chunks is swept by a team of m = ceil(c / chunks_per_block) blocks, each
+// holding at most chunks_per_block consecutive chunks (the counts differ by at most one). The
+// blocks of the launch take the entries by ticket: teams in order of decreasing cost
+// (fit_rows x (chunks per member + the step's gathering latency), so the longest sweeps start
+// first and the short ones fill in at the end), each team's members on consecutive tickets.
+constexpr double kSweepStepLatency = 4.0;   // a step's gathering and solve, in chunk passes
+constexpr uint32_t kSweepSpinLimit = 1u << 22;   // polls of one wait (each a device-memory round trip): seconds
+
+struct SweepSchedule {
+    std::vector<SweepEntry> entries;   // ticket order
+    uint32_t teams = 0;
+    uint32_t most_members = 0;
+    uint64_t records = 0;              // partial records: 2 per chunk of every team
+};
+
+void plan_sweep(const uint32_t *chunks, const uint32_t *fit_rows, uint32_t groups, uint32_t chunks_per_block,
+                SweepSchedule &out) {
+    out = SweepSchedule();
+    struct Team {
+        uint32_t group, members;
+        double cost;
+    };
+    std::vector<Team> teams;
+    for (uint32_t g = 0; g < groups; ++g) {
+        if (fit_rows[g] == 0 || chunks[g] == 0) continue;
+        const uint32_t m = (chunks[g] + chunks_per_block - 1) / chunks_per_block;
+        const uint32_t per = (chunks[g] + m - 1) / m;
+        teams.push_back({g, m, static_cast<double>(fit_rows[g]) * (per + kSweepStepLatency)});
+    }
+    std::stable_sort(teams.begin(), teams.end(), [](const Team &a, const Team &b) { return a.cost > b.cost; });
+    for (uint32_t id = 0; id < teams.size(); ++id) {
+        const Team &T = teams[id];
+        const uint32_t c = chunks[T.group];
+        for (uint32_t i = 0; i < T.members; ++i) {
+            SweepEntry e{};
+            e.group = T.group;
+            e.chunk_begin = static_cast<uint32_t>(static_cast<uint64_t>(c) * i / T.members);
+            e.chunk_end = static_cast<uint32_t>(static_cast<uint64_t>(c) * (i + 1) / T.members);
+            e.members = T.members;
+            e.team = id;
+            e.leader = i == 0 ? 1u : 0u;
+            e.part_base = out.records;
+            out.entries.push_back(e);
+        }
+        out.records += 2ull * c;
+        out.most_members = std::max(out.most_members, T.members);
+    }
+    out.teams = static_cast<uint32_t>(teams.size());
+}
+
+}  // namespace
+
+extern "C" int fohip_sweep_schedule(const uint32_t *chunks, const uint32_t *fit_rows, uint32_t groups,
+                                    uint32_t chunks_per_block, uint32_t *entries, uint64_t entry_capacity,
+                                    uint64_t *entry_count, uint32_t *teams) {
+    if ((groups && (!chunks || !fit_rows)) || !entry_count || !teams || chunks_per_block == 0)
+        return FOHIP_E_INVALID_ARGUMENT;
+    SweepSchedule plan;
+    plan_sweep(chunks, fit_rows, groups, chunks_per_block, plan);
+    *entry_count = plan.entries.size();
+    *teams = plan.teams;
+    if (plan.entries.size() > entry_capacity || (!entries && !plan.entries.empty())) return FOHIP_E_INVALID_ARGUMENT;
+    for (size_t e = 0; e < plan.entries.size(); ++e) {
+        const SweepEntry &x = plan.entries[e];
+        uint32_t *w = entries + 8 * e;
+        w[0] = x.group;
+        w[1] = x.chunk_begin;
+        w[2] = x.chunk_end;
+        w[3] = x.members;
+        w[4] = x.team;
+        w[5] = x.leader;
+        w[6] = static_cast<uint32_t>(x.part_base);
+        w[7] = static_cast<uint32_t>(x.part_base >> 32);
+    }
+    return FOHIP_OK;
+}
+
+namespace {
+
 // One memory-bounded batch of consecutive groups on its way through the device: the host tables
 // built for it, the slot buffers it claimed and the host copies its results come back into.
 struct BatchRun {
@@ -912,7 +996,7 @@ struct BatchRun {
     uint32_t max_strikes = 0, max_obs = 0, max_fit = 0;
     std::vector<uint2> items_t, items_e, items_b;
     std::vector<uint32_t> rows_sorted, step_items, step_rows, step_items_b;
-    static constexpr int kTableParts = 23;
+    static constexpr int kTableParts = 24;
     size_t part_bytes[kTableParts] = {};
     const void *part_host[kTableParts] = {};
     size_t part_off[kTableParts] = {};
@@ -926,6 +1010,13 @@ struct BatchRun {
     DevTables t = {};
     const uint2 *it = nullptr, *ite = nullptr, *itb = nullptr;
     const uint32_t *rows = nullptr;
+    // fit sweep (single-strike groups): schedule, device state and the abort word read back
+    bool sweep = false;
+    uint32_t sweep_slots = 0;
+    SweepSchedule sw;
+    const SweepEntry *sweep_entries_d = nullptr;
+    void *sweep_sync_d = nullptr, *sweep_part_d = nullptr, *sweep_abort_d = nullptr;
+    uint32_t sweep_abort = 0;
     char *staging = nullptr;
     // host copies of the results
     std::vector<unsigned long long> fail_t, fail_e, ties_h;
@@ -947,7 +1038,8 @@ struct BatchRun {
                 {summary.data(), (&summary_d), nrows * sizeof(RowSummary)},
                 {models_h.data(), (&models_d), models_h.size() * sizeof(Model)},
                 {counts_h.data(), (&counts_d), counts_h.size() * sizeof(uint32_t)},
-                {ties_h.data(), (&ties_d), ties_h.size() * sizeof(unsigned long long)}};
+                {ties_h.data(), (&ties_d), ties_h.size() * sizeof(unsigned long long)},
+                {&sweep_abort, (&sweep_abort_d), sweep ? sizeof(uint32_t) : 0}};
     }
 };
 
@@ -1000,6 +1092,34 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
     if (spec.fused) {
         const uint32_t mode = ex->tuning.fit_cashflows;
         use_stops = most_obs <= 255 && mode == FOHIP_CASHFLOWS_STOPPING_ROWS;
+    }
+    // The fit sweep, one choice per request and only when the tuning asks for it: fused method,
+    // value-column cashflows, one strike per group, at most 2^28 training paths per group (one
+    // buffer resource's byte range), and no team larger than the device's CU count (every team's
+    // blocks can run at once: one per CU). Per-node launches stay the default: on the ragged batch
+    // the sweep's fit takes 1.6 s against 0.54 s, because each backward step of a team waits about
+    // 16 us for its members' records and its solve with one block per CU and nothing to overlap.
+    uint32_t sweep_slots = 0;
+    const uint32_t sweep_chunks = f32 ? kSweepChunks32 : kSweepChunks64;
+    if (spec.fused && !use_stops && most_strikes == 1 && ex->tuning.fit_schedule == FOHIP_FIT_SWEEP) {
+        const int win = ex->near_tie_window != 0.0 ? 1 : 0;
+        int64_t &slots = ex->sweep_slots[f32 ? 1 : 0][win];
+        if (slots < 0) {
+            int cus = 0;
+            const int per_cu = sweep_resident_blocks(f32, win != 0);
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess) cus = 0;
+            (void)hipGetLastError();
+            slots = static_cast<int64_t>(per_cu) * cus;
+        }
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ex->device) != hipSuccess) cus = 0;
+        bool fits = slots > 0 && cus > 0;
+        for (const GroupPlan &p : all) {
+            const uint64_t cb = (p.nt + kBulkChunk - 1) / kBulkChunk;
+            if (p.nt >= (1ull << 28) || (cb + sweep_chunks - 1) / sweep_chunks > static_cast<uint64_t>(cus))
+                fits = false;
+        }
+        if (fits) sweep_slots = static_cast<uint32_t>(slots);
     }
     // The evaluation's strike tile, one per request (results do not depend on it); windowed
     // near-tie counts run the evaluation's diagnostic form over single-strike tiles.
@@ -1116,6 +1236,16 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             R.step_items_b[j] = nb;
             R.step_rows[j] = nr;
         }
+        if (sweep_slots) {
+            std::vector<uint32_t> ch(R.ngroups), fr(R.ngroups);
+            for (uint32_t g = 0; g < R.ngroups; ++g) {
+                ch[g] = R.gt.groups[g].chunks_b;
+                fr[g] = R.gt.groups[g].fit_rows;
+            }
+            plan_sweep(ch.data(), fr.data(), R.ngroups, sweep_chunks, R.sw);
+            R.sweep = !R.sw.entries.empty();
+            R.sweep_slots = sweep_slots;
+        }
         const GroupTables &gt = R.gt;
         const size_t bytes[BatchRun::kTableParts] = {
             gt.groups.size() * sizeof(DevGroup), gt.log_drift.size() * 8, gt.diffusion.size() * 8,
@@ -1124,14 +1254,16 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             gt.row_time.size() * 8, gt.row_node.size() * 4, R.bstrikes.size() * 8, R.strike_group.size() * 4,
             gt.row_escrow.size() * 8, gt.row_escrow_dr.size() * 8, R.binv.size() * 8,
             R.items_t.size() * sizeof(uint2), R.items_e.size() * sizeof(uint2), R.items_b.size() * sizeof(uint2),
-            R.rows_sorted.size() * sizeof(uint32_t), gt.steps.size() * sizeof(StepRec)};
+            R.rows_sorted.size() * sizeof(uint32_t), gt.steps.size() * sizeof(StepRec),
+            R.sweep ? R.sw.entries.size() * sizeof(SweepEntry) : 0};
         const void *host[BatchRun::kTableParts] = {
             gt.groups.data(), gt.log_drift.data(), gt.diffusion.data(), gt.log_drift32.data(),
             gt.diffusion32.data(), gt.dt.data(), gt.sqrt_dt.data(), gt.cash.data(),
             gt.node_flags.data(), gt.disc0.data(), gt.disc_step.data(), gt.row_time.data(),
             gt.row_node.data(), R.bstrikes.data(), R.strike_group.data(),
             gt.row_escrow.data(), gt.row_escrow_dr.data(), R.binv.data(),
-            R.items_t.data(), R.items_e.data(), R.items_b.data(), R.rows_sorted.data(), gt.steps.data()};
+            R.items_t.data(), R.items_e.data(), R.items_b.data(), R.rows_sorted.data(), gt.steps.data(),
+            R.sw.entries.data()};
         for (int i = 0; i < BatchRun::kTableParts; ++i) {
             R.part_bytes[i] = bytes[i];
             R.part_host[i] = host[i];
@@ -1182,6 +1314,12 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         HIP_TRY(ws.claim(S.ties, std::max<uint32_t>(R.nrows, 1) * sizeof(unsigned long long), &R.ties_d));
         HIP_TRY(ws.claim(S.replay, 4 * sizeof(double), &R.replay_d));
         if (R.stored_normals) HIP_TRY(ws.claim(S.znorm, R.znorm_elems * sizeof(double), &R.znorm_d));
+        if (R.sweep) {
+            HIP_TRY(ws.claim(S.sweep_sync, (static_cast<size_t>(R.sw.teams) + 2) * sizeof(uint32_t), &R.sweep_sync_d));
+            HIP_TRY(ws.claim(S.sweep_part, std::max<uint64_t>(R.sw.records, 1) * kSweepStride * sizeof(double),
+                             &R.sweep_part_d));
+            if (R.sweep_sync_d) R.sweep_abort_d = static_cast<uint32_t *>(R.sweep_sync_d) + R.sw.teams;
+        }
         if (ws.overflow) {
             return fail(err, FOHIP_E_UNSUPPORTED, 0,
                         "request needs %llu bytes of device scratch; the allocation-free entry point holds %llu",
@@ -1221,6 +1359,7 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         R.itb = reinterpret_cast<const uint2 *>(tb + o[20]);
         R.rows = reinterpret_cast<const uint32_t *>(tb + o[21]);
         t.steps = reinterpret_cast<const StepRec *>(tb + o[22]);
+        R.sweep_entries_d = reinterpret_cast<const SweepEntry *>(tb + o[23]);
         // rules zeroed, exercise and no-support counters zeroed, failure keys set to "none"
         static_assert(sizeof(Model) % 4 == 0 && sizeof(RowError) % 4 == 0, "fills write 32-bit words");
         FillList fills = {};
@@ -1236,6 +1375,8 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         add_fill(R.ties_d, R.ties_h.size() * sizeof(unsigned long long), 0u);
         // every path starts stopped at expiry: stopping byte 0
         if (use_stops) add_fill(R.stops_d, (R.value_elems + 3) & ~uint64_t(3), 0u);
+        // team arrival counters, the abort word and the ticket counter of the fit sweep
+        if (R.sweep) add_fill(R.sweep_sync_d, (static_cast<size_t>(R.sw.teams) + 2) * sizeof(uint32_t), 0u);
         HIP_TRY(launch_fill(gen_st, fills));
         // ---- training simulation
         PROF_LAUNCH(gen_st, "k_simulate",
@@ -1251,7 +1392,15 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
         double *part_t = static_cast<double *>(R.part_t_d);
         FitParams fp = spec.fp;
         fp.tie_window = ex->near_tie_window;
-        for (uint32_t j = 0; j < R.max_fit; ++j) {
+        if (R.sweep) {   // every backward step of the batch in one persistent launch
+            SweepPlan plan{R.sweep_entries_d, static_cast<uint32_t *>(R.sweep_sync_d),
+                           static_cast<double *>(R.sweep_part_d), static_cast<uint32_t>(R.sw.entries.size()),
+                           R.sw.teams, kSweepSpinLimit, 0u};
+            PROF_LAUNCH(fit_st, "k_fit_sweep",
+                        launch_fit_sweep(fit_st, f32, t, fp, plan, R.sweep_slots, R.paths_d, R.values_d, models,
+                                         errors));
+        }
+        for (uint32_t j = 0; j < (R.sweep ? 0u : R.max_fit); ++j) {
             const uint32_t ni = R.step_items[j], nr = R.step_rows[j];
             if (spec.fused) {   // one fused HBM pass per backward step
                 PROF_LAUNCH(fit_st, "k_fit_bulk_step",
@@ -1339,6 +1488,11 @@ static int run_groups(fohip_executor *ex, const std::vector<GroupPlan> &all, con
             }
         }
         if (ex->profiling) ex->collect();
+        if (R.sweep && R.sweep_abort != 0u)
+            return fail(err, FOHIP_E_HIP, FOHIP_PHASE_FIT,
+                        "fit sweep: a team of blocks did not gather within the wait limit (%u blocks launched, "
+                        "largest team %u); the results of this request are void",
+                        R.sweep_slots, R.sw.most_members);
         const DevTables &t = R.t;
         FitParams fp = spec.fp;
         // Failures, in the reference's processing order: groups in order; within a group the

@@ -1290,6 +1290,361 @@ __device__ __forceinline__ void solve_node_from_sums(Model &m, const FitParams &
     stieltjes_from_moments(m, fp, err, step, mu, nu);
 }
 
+// ------------------------------------------------------------------------------- fit sweep
+
+// Buffer resources: the fourth descriptor word of a raw (stride 0) 32-bit-format buffer, and the
+// cache-policy bit of a streaming load.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kBufferNonTemporal = 2;
+
+// A double that every lane of the wave holds (read from LDS), as a wave-uniform scalar.
+__device__ __forceinline__ double wave_uniform(double x) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Element `lane` of a segment of `bytes` bytes, or zero past its end, read with a streaming
+// (non-temporal) buffer load: the resource's range check does the masking.
+__device__ __forceinline__ double load_streaming(const double *seg, uint32_t bytes, uint32_t lane) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(seg), 0, static_cast<int>(bytes), kBufferWord3);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, lane * 8u, 0, kBufferNonTemporal));
+}
+__device__ __forceinline__ float load_streaming(const float *seg, uint32_t bytes, uint32_t lane) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(seg), 0, static_cast<int>(bytes), kBufferWord3);
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4u, 0, kBufferNonTemporal));
+}
+
+// Persistent backward induction of single-strike groups (fused methods, value-column cashflows).
+// A group's fused-fit chunks are split over a team of blocks. A block keeps its chunks' spot values
+// of rows r + 1 and r in registers and their cashflows in LDS from the first backward step to the
+// last, so a step reads row r from HBM and nothing else (the per-node launches read rows r and
+// r + 1 and the cashflow column and write exercised cashflows back at every step); row r - 1 is
+// loaded while the team gathers. A block is four worker waves, whose threads hold paths exactly as
+// the per-node kernel's threads do, and a control wave: each chunk's partial sums are the per-node
+// kernel's, operation for operation (paths per thread, accumulation order, wave transpose, combine
+// of the four waves). The control wave writes the block's partial records with device-scope
+// stores (through to the coherence point, so no cache writeback is needed), waits for them to
+// complete and arrives at the team's counter; once every member has arrived it reads the team's
+// records with device-scope loads, merges them in k_fit_bulk_reduce's order and solves the rule
+// itself, so every member holds the same rule and takes the same branches, and only the leader
+// stores rules and failures. The control wave holds no path data, so the solve's registers and the rows' never
+// add up. A failure in any chunk (recorded with the per-node keys) or in the solve ends the
+// group's sweep at that step for every member, as the per-node reduce skips a dead row. The final
+// cashflows go to the value column for k_fit_final.
+// Termination: a block takes entries by ticket from a device-memory counter only while it runs,
+// and a team's entries hold consecutive tickets, so at any time every team but the one whose
+// tickets are being handed out has all its members running and finishes; blocks it frees take
+// the remaining tickets of that team. Every team completes once the device can hold the largest
+// team's blocks at once (the host checks this against the CU count), whatever else runs beside
+// the sweep and however many of the grid's blocks are resident. A wait longer than
+// plan.spin_limit polls (seconds) raises the abort word, which every waiting member sees, and
+// all blocks return.
+constexpr int kSweepWorkers = kWaves;               // worker waves: one chunk's 256 threads
+constexpr int kSweepBlock = (kSweepWorkers + 1) * 64;
+
+template <typename Store, bool WIN, int Q>
+__global__ __launch_bounds__(kSweepBlock, 3) void k_fit_sweep(DevTables t, FitParams fp, SweepPlan plan,
+                                                              const Store *__restrict__ paths,
+                                                              Store *__restrict__ values,
+                                                              Model *__restrict__ models,
+                                                              RowError *__restrict__ errors) {
+    constexpr int kFields = 17;   // record fields written: 14 sums, pivot, ties, failure
+    __shared__ double red[kSweepWorkers][Q][kSweepStride];
+    __shared__ double transpose[kSweepWorkers][14 * 66];   // workers' transposes; the merge's buffer
+    __shared__ Store cash[Q][kBulkPathsPerThread][kBlock];   // the chunks' cashflows, lane-contiguous
+    __shared__ Model rule;          // the rule the team fitted last: row r + 1 of the next step
+    __shared__ RowError sink;       // failures of this member's own solve
+    __shared__ uint32_t gathered;   // 0 the team has arrived, 2 abort
+    __shared__ uint32_t verdict;    // 0 next step, 1 the group's sweep ends here
+    __shared__ uint32_t ticket;     // the entry the block runs next (>= entry_count: none left)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t *abort_word = plan.sync + plan.teams;
+    if (wave == 0) {
+        // ---------------------------------------------------------------- control wave
+        uint32_t *tickets = abort_word + 1;
+        for (;;) {
+            if (lane == 0)
+                ticket = __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u
+                             ? plan.entry_count
+                             : atomicAdd(tickets, 1u);
+            __syncthreads();   // every wave knows the entry
+            const uint32_t e = ticket;
+            if (e >= plan.entry_count) return;
+            const FOHIP_UNIFORM SweepEntry *job = uniform(plan.entries + e);
+            const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + job->group);
+            const uint32_t c0 = job->chunk_begin, nq = job->chunk_end - job->chunk_begin;
+            const uint32_t obs = g->obs, fit_rows = g->fit_rows, cb = g->chunks_b;
+            const uint32_t srow = g->strike_begin;
+            const double K = *uniform(t.strikes + srow);
+            double *team = plan.partials + job->part_base * kSweepStride;
+            uint32_t *arrivals = plan.sync + job->team;
+            for (uint32_t j = 0; j < fit_rows; ++j) {
+                const uint32_t r = obs - 2 - j;
+                __syncthreads();   // the workers' chunk sums are in `red`
+                double *recs = team + static_cast<uint64_t>(j & 1u) * cb * kSweepStride;
+                for (uint32_t idx = lane; idx < nq * kFields; idx += 64) {
+                    const uint32_t q = idx / kFields, f = idx % kFields;
+                    const double x = f == 14 ? red[0][q][14]
+                                             : (red[0][q][f] + red[1][q][f]) + (red[2][q][f] + red[3][q][f]);
+                    __hip_atomic_store(recs + static_cast<uint64_t>(c0 + q) * kSweepStride + f, x,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                // the records are device-scope stores (written through to the coherence point, no
+                // cache writeback needed): the arrival waits only for them to complete
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                uint32_t status = 0;
+                if (lane == 0) {
+                    atomicAdd(arrivals, 1u);
+                    const uint32_t target = job->members * (j + 1);
+                    uint32_t polls = 0;
+                    while (__hip_atomic_load(arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                        if ((++polls & 63u) == 0u &&
+                            (polls > plan.spin_limit ||
+                             __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
+                            atomicOr(abort_word, 1u);
+                            status = 2;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                asm volatile("" ::: "memory");   // the team's records are read after every arrival
+                status = __builtin_amdgcn_readfirstlane(status);
+                if (lane == 0) gathered = status;
+                if (status == 0u) {
+                    // k_fit_bulk_reduce's merge: lane-strided chunk sums in chunk order (two chunks
+                    // in flight), then the LDS transpose; ties are integers and failures flags,
+                    // exact in any order
+                    double acc[14];
+#pragma unroll
+                    for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+                    double ties = 0.0, fails = 0.0;
+                    uint32_t c = lane;
+                    for (; c + 64 < cb; c += 2 * 64) {
+                        double in[2][kFields];
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+                            const double *slot = recs + static_cast<uint64_t>(c + u * 64) * kSweepStride;
+#pragma unroll
+                            for (int i = 0; i < kFields; ++i)
+                                in[u][i] = __hip_atomic_load(slot + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+                            for (int i = 0; i < 14; ++i) acc[i] += in[u][i];
+                            ties += in[u][15];
+                            fails += in[u][16];
+                        }
+                    }
+                    for (; c < cb; c += 64) {
+                        const double *slot = recs + static_cast<uint64_t>(c) * kSweepStride;
+                        double in[kFields];
+#pragma unroll
+                        for (int i = 0; i < kFields; ++i)
+                            in[i] = __hip_atomic_load(slot + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                        for (int i = 0; i < 14; ++i) acc[i] += in[i];
+                        ties += in[15];
+                        fails += in[16];
+                    }
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) ties += __shfl_xor(ties, off, 64);
+                    const bool dead = __ballot(fails != 0.0) != 0ull;
+                    double *buf = transpose[0];   // the workers are between their transposes
+                    const double total = wave_sums_lds<14>(acc, buf);
+                    if ((lane & 3) == 0 && lane < 56) buf[lane >> 2] = total;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane == 0) {
+                        if (dead) {
+                            verdict = 1u;   // a chunk failed: the row is dead, as the per-node reduce finds it
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 14; ++i) acc[i] = buf[i];
+                            // chunk 0's pivot (every chunk holds it)
+                            const double pivot = __hip_atomic_load(recs + 14, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            sink.key = kNoError;
+                            Model m = {};
+                            solve_node_from_sums(m, fp, &sink, j, K, acc, pivot);
+                            m.ties = ties;
+                            if (job->leader != 0u) {
+                                models[g->model_base + r] = m;
+                                if (sink.key != kNoError) {
+                                    RowError &out = errors[srow];
+                                    out.key = sink.key;
+                                    out.code = sink.code;
+                                    out.index = sink.index;
+                                    out.observed = sink.observed;
+                                    out.limit = sink.limit;
+                                }
+                            }
+                            rule = m;
+                            verdict = sink.key != kNoError ? 1u : 0u;
+                        }
+                    }
+                }
+                __syncthreads();   // `gathered`, `verdict` and `rule` are set
+                if (status != 0u) return;
+                if (verdict != 0u) break;
+            }
+        }
+        return;
+    }
+    // -------------------------------------------------------------------- worker waves
+    const int worker = wave - 1;
+    const uint32_t tw = threadIdx.x - 64;   // the thread's place in a chunk, as in the per-node kernel
+    for (;;) {
+        __syncthreads();   // the control wave has taken the entry
+        const uint32_t e = ticket;
+        if (e >= plan.entry_count) return;
+        const FOHIP_UNIFORM SweepEntry *job = uniform(plan.entries + e);
+        const FOHIP_UNIFORM DevGroup *g = uniform(t.groups + job->group);
+        const uint32_t c0 = job->chunk_begin;
+        const uint32_t nq = job->chunk_end - c0;
+        const uint32_t n = g->nt, obs = g->obs, fit_rows = g->fit_rows;
+        const uint32_t srow = g->strike_begin, row_base = g->row_base;
+        const double K = *uniform(t.strikes + srow);
+        const double invK = *uniform(t.inv_strikes + srow);
+        const Store *S = paths + g->path_base;
+        Store s1[Q][kBulkPathsPerThread], s0[Q][kBulkPathsPerThread];
+        // A chunk slot's row segment is read through a buffer resource that starts at the slot's
+        // first path and ends at path n: one offset register (the thread's) serves every load, and
+        // lanes past n read zero from the hardware's range check, the per-node kernel's zero spots.
+        auto load_row = [&](uint32_t row, Store (&dst)[Q][kBulkPathsPerThread]) {
+            const Store *R = S + static_cast<uint64_t>(row) * n;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (static_cast<uint32_t>(q) >= nq) break;
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const uint32_t first = (c0 + q) * kBulkChunk + k * kBlock;
+                    const uint32_t bytes = first < n ? (n - first) * static_cast<uint32_t>(sizeof(Store)) : 0u;
+                    dst[q][k] = load_streaming(R + first, bytes, tw);
+                }
+            }
+        };
+        load_row(obs - 1, s1);
+        load_row(obs - 2, s0);
+        bool alive = true;
+        for (uint32_t j = 0; j < fit_rows; ++j) {
+            const uint32_t r = obs - 2 - j;
+            const double d_next = *uniform(t.disc0 + row_base + r + 1);
+            const double inv_d = 1.0 / *uniform(t.disc0 + row_base + r);
+            const bool apply = j > 0 && __builtin_amdgcn_readfirstlane(rule.kind) == kRuleRegression;
+            const double pivot = apply ? wave_uniform(rule.mean) : 1.0;
+            double ra = 0.0, rb = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0, h4 = 0.0;
+            if (apply) {
+                ra = wave_uniform(rule.a);
+                rb = wave_uniform(rule.b);
+                h0 = wave_uniform(rule.hz[0]);
+                h1 = wave_uniform(rule.hz[1]);
+                h2 = wave_uniform(rule.hz[2]);
+                h3 = wave_uniform(rule.hz[3]);
+                h4 = wave_uniform(rule.hz[4]);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (static_cast<uint32_t>(q) >= nq) break;
+                const uint32_t base = (c0 + q) * kBulkChunk + tw;
+                double acc[14];
+#pragma unroll
+                for (int i = 0; i < 14; ++i) acc[i] = 0.0;
+                uint32_t tie_wave = 0, bad_bits = 0, nonfinite_bits = 0;
+                // path by path: the cashflow after row r + 1's decision, its failure bits, then its
+                // power-sum terms (the per-node kernel's operations, in its order for every sum)
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const double x1 = s1[q][k];
+                    double v;
+                    bool bad = false;
+                    if (j == 0) {
+                        const Store vf = static_cast<Store>((x1 > K ? x1 - K : 0.0) * d_next);
+                        cash[q][k][tw] = vf;
+                        v = vf;
+                    } else if (apply) {
+                        // the rule's two leading addends in vector registers, as in the per-node kernel
+                        double rule_b = rb, rule_h3 = h3;
+                        asm volatile("" : "+v"(rule_b), "+v"(rule_h3));
+                        const double z = fma(x1, ra, rule_b);
+                        double c = fma(h4, z, rule_h3);
+                        c = fma_uniform_addend(c, z, h2);
+                        c = fma_uniform_addend(c, z, h1);
+                        const double cont = fma_uniform_addend(c, z, h0);
+                        const bool cand = x1 > K;
+                        const bool finite = isfinite(cont);
+                        const bool ex = cand && finite && should_exercise(fp.tie, x1 - K, cont);
+                        tie_wave += static_cast<uint32_t>(
+                            WIN ? __popcll(__ballot(cand && finite && near_tie(x1 - K, cont, fp.tie_window)))
+                                : __popcll(__ballot((x1 - K) == cont) & __ballot(cand)));
+                        const Store vf = static_cast<Store>((x1 - K) * d_next);
+                        const Store held = cash[q][k][tw];
+                        v = ex ? static_cast<double>(vf) : static_cast<double>(held);
+                        if (ex) cash[q][k][tw] = vf;
+                        bad = cand && !finite;
+                    } else {
+                        v = cash[q][k][tw];
+                    }
+                    bad_bits |= (bad ? 1u : 0u) << k;
+                    nonfinite_bits |= (!isfinite(v) ? 1u : 0u) << k;
+                    const double x0 = s0[q][k];
+                    const bool itm = x0 > K;
+                    const double u = fma(x0, invK, -pivot);
+                    double pw = itm ? 1.0 : 0.0;
+                    const double y = (v * inv_d) * pw;
+#pragma unroll
+                    for (int jj = 0; jj <= 8; ++jj) {
+                        acc[jj] += pw;
+                        if (jj <= 4) acc[9 + jj] = fma(y, pw, acc[9 + jj]);
+                        pw *= u;
+                    }
+                }
+                // failures are rare: one wave-uniform test for all slots
+                const bool failed = __ballot((bad_bits | nonfinite_bits) != 0u) != 0ull;
+                if (failed) record_fit_errors(&errors[srow], j, base, n, bad_bits, nonfinite_bits);
+                const double total = wave_sums_lds<14>(acc, transpose[worker]);
+                if ((lane & 3) == 0 && lane < 56) red[worker][q][lane >> 2] = total;
+                if (lane == 0) {
+                    red[worker][q][14] = pivot;
+                    red[worker][q][15] = static_cast<double>(tie_wave);
+                    red[worker][q][16] = failed ? 1.0 : 0.0;
+                }
+            }
+            __syncthreads();   // chunk sums to the control wave
+            // row r is the next step's row r + 1; row r - 1 loads while the team gathers
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) s1[q][k] = s0[q][k];
+            if (j + 1 < fit_rows) load_row(r - 1, s0);
+            __syncthreads();   // the team's rule, or the end of the sweep
+            if (gathered != 0u) return;
+            if (verdict != 0u) {
+                alive = false;
+                break;
+            }
+        }
+        if (alive) {   // final cashflows for k_fit_final
+            Store *V = values + g->value_base;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (static_cast<uint32_t>(q) >= nq) break;
+#pragma unroll
+                for (int k = 0; k < kBulkPathsPerThread; ++k) {
+                    const uint32_t p = (c0 + q) * kBulkChunk + k * kBlock + tw;
+                    if (p < n) V[p] = cash[q][k][tw];
+                }
+            }
+        }
+    }
+}
+
 // After the last fitted row: apply it, discount to the origin (policy.rs:1065-1079) and sum the
 // cashflows for the valuation-origin decision (policy.rs:267-281). Stored values are either
 // discounted to the last fitted node (FP64 method, the reference's representation and arithmetic)
@@ -2099,6 +2454,37 @@ hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitP
     if (nrows == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fit_bulk_reduce, dim3(nrows), dim3(64), 0, st, t, fp, step, rows, partials, models, errors);
     return hipGetLastError();
+}
+
+hipError_t launch_fit_sweep(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                            const SweepPlan &plan, uint32_t slots, const void *paths, void *values,
+                            Model *models, RowError *errors) {
+    if (slots == 0) return hipSuccess;
+    const bool win = fp.tie_window != 0.0;
+#define FOHIP_SWEEP(REAL, WINV, QV)                                                                            \
+    hipLaunchKernelGGL((k_fit_sweep<REAL, WINV, QV>), dim3(slots), dim3(kSweepBlock), 0, st, t, fp, plan,          \
+                       static_cast<const REAL *>(paths), static_cast<REAL *>(values), models, errors)
+    if (f32) {
+        if (win) FOHIP_SWEEP(float, true, kSweepChunks32);
+        else FOHIP_SWEEP(float, false, kSweepChunks32);
+    } else {
+        if (win) FOHIP_SWEEP(double, true, kSweepChunks64);
+        else FOHIP_SWEEP(double, false, kSweepChunks64);
+    }
+#undef FOHIP_SWEEP
+    return hipGetLastError();
+}
+
+int sweep_resident_blocks(bool f32, bool window) {
+    int n = 0;
+    hipError_t e;
+    if (f32)
+        e = window ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fit_sweep<float, true, kSweepChunks32>, kSweepBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fit_sweep<float, false, kSweepChunks32>, kSweepBlock, 0);
+    else
+        e = window ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fit_sweep<double, true, kSweepChunks64>, kSweepBlock, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fit_sweep<double, false, kSweepChunks64>, kSweepBlock, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,

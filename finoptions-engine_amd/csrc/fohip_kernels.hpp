@@ -20,6 +20,18 @@ struct FitParams {
     double tie_window;
 };
 
+// Schedule and team state of one fit sweep (see k_fit_sweep).
+struct SweepPlan {
+    const SweepEntry *entries;    // in ticket order: team by team, each team's members in turn
+    uint32_t *sync;               // [teams] arrival counters, the abort word, the ticket counter (all
+                                  // zero before the launch)
+    double *partials;             // kSweepStride doubles per record
+    uint32_t entry_count;
+    uint32_t teams;
+    uint32_t spin_limit;          // polls of one wait before the sweep gives up and aborts
+    uint32_t pad;
+};
+
 // Word fills of up to kMaxFills device regions in one launch (the per-pricing initialisation of
 // rules, counters and failure keys); each region is `words` 32-bit words set to `value`.
 constexpr int kMaxFills = 8;
@@ -68,6 +80,15 @@ hipError_t launch_fit_bulk_step(hipStream_t st, bool f32, const DevTables &t, co
 hipError_t launch_fit_bulk_reduce(hipStream_t st, const DevTables &t, const FitParams &fp, uint32_t step,
                                   const uint32_t *rows, uint32_t nrows, double *partials, Model *models,
                                   RowError *errors);
+// Fused method, single-strike groups: the whole backward induction of every group of the batch in
+// one persistent launch of `slots` blocks that take the plan's entries by ticket (bitwise the
+// per-node launches' rules, failures and final cashflows). Any grid is safe once the device can
+// hold the largest team's blocks at once.
+hipError_t launch_fit_sweep(hipStream_t st, bool f32, const DevTables &t, const FitParams &fp,
+                            const SweepPlan &plan, uint32_t slots, const void *paths, void *values,
+                            Model *models, RowError *errors);
+// Blocks of the sweep kernel one CU holds at once (0 on failure).
+int sweep_resident_blocks(bool f32, bool window);
 hipError_t launch_fit_final(hipStream_t st, bool f32, bool fused, const DevTables &t, const FitParams &fp,
                             const uint2 *items, uint32_t nitems, uint32_t max_strikes,
                             const void *paths, const void *values, const uint8_t *stops, const Model *models,

@@ -150,6 +150,22 @@ struct DevTables {
     const double *inv_strikes;    // [strike row] 1 / K
 };
 
+// One entry of the fit sweep's schedule (single-strike groups): the block that takes it joins the
+// team sweeping `group` and keeps the group's fused-fit chunks [chunk_begin, chunk_end) for the
+// whole backward induction. Blocks take entries by ticket, one team's entries after another.
+struct SweepEntry {
+    uint32_t group;
+    uint32_t chunk_begin, chunk_end;
+    uint32_t members;      // blocks in the team
+    uint32_t team;         // the team's arrival counter
+    uint32_t leader;       // 1: this member stores the team's rules and failures
+    uint64_t part_base;    // first partial record of the team: 2 step parities x chunks_b records
+};
+static_assert(sizeof(SweepEntry) == 32, "two entries per 64-byte line");
+constexpr int kSweepStride = 18;   // doubles per sweep partial record: 14 sums, pivot, ties, failure, pad
+constexpr int kSweepChunks64 = 3;  // chunks a sweep block keeps in registers (binary64 storage)
+constexpr int kSweepChunks32 = 5;  // the same with binary32 storage
+
 // Per-strike-row failure record: the smallest key wins (earliest failure in processing order).
 struct RowError {
     unsigned long long key;       // (step << 44) | (stage << 36) | path  (path < 2^36)

@@ -52,7 +52,8 @@ class FohipKernelStat(C.Structure):
 
 class FohipTuning(C.Structure):
     _fields_ = [("eval_strike_tile", C.c_uint32), ("fit_cashflows", C.c_uint32),
-                ("batch_schedule", C.c_uint32), ("batch_mib", C.c_uint32)]
+                ("batch_schedule", C.c_uint32), ("batch_mib", C.c_uint32),
+                ("fit_schedule", C.c_uint32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -186,7 +187,7 @@ EXPORTED = (
     "fohip_validate_constant_volatility", "fohip_validate_cash_dividend_amount",
     "fohip_validate_ragged_offsets", "fohip_validate_ragged_exercise_times",
     "fohip_validate_ragged_cash_dividends", "fohip_set_profiling", "fohip_get_profile",
-    "fohip_reset_profile", "fohip_price_lsm_call_noalloc",
+    "fohip_reset_profile", "fohip_price_lsm_call_noalloc", "fohip_sweep_schedule",
 )
 
 _lib: C.CDLL | None = None
@@ -233,6 +234,8 @@ def lib() -> C.CDLL:
                                            C.c_char_p, C.c_size_t]
     handle.fohip_set_tuning.argtypes = [C.c_void_p, C.POINTER(FohipTuning)]
     handle.fohip_set_near_tie_window.argtypes = [C.c_void_p, C.c_double]
+    handle.fohip_sweep_schedule.argtypes = [_u32p, _u32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint64,
+                                            C.POINTER(C.c_uint64), _u32p]
     handle.fohip_price_lsm_call.argtypes = [C.c_void_p, C.POINTER(FohipCallRequest),
                                             C.POINTER(FohipCallResult), E]
     handle.fohip_price_lsm_call_noalloc.argtypes = [C.c_void_p, C.POINTER(FohipCallRequest),

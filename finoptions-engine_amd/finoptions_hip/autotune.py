@@ -156,10 +156,10 @@ def tune_request(executor, price: Callable, *, group_offsets, initial_spots, str
     ``price(**sample)`` prices a request given as group columns (the batch module's untuned
     pricing function with the method and Greeks bound)."""
     key = shape_key(executor, group_offsets, mesh, dividends, method, greeks, path_counts)
-    schedule, mib = executor.tuning[2], executor.tuning[3]
+    schedule, mib, fit = executor.tuning[2], executor.tuning[3], executor.tuning[4]
     rec = _CACHE.records.get(key)
     if rec is not None:
-        executor.tune(**rec.choice, batch_schedule=schedule, batch_mib=mib)
+        executor.tune(**rec.choice, batch_schedule=schedule, batch_mib=mib, fit_schedule=fit)
         return rec
     n = _sample_groups(group_offsets, mesh, dividends, method, path_counts)
     sample = _slice(n, group_offsets, initial_spots, strikes, maturities, rates, volatilities,
@@ -176,7 +176,8 @@ def tune_request(executor, price: Callable, *, group_offsets, initial_spots, str
 
     def run(tile, cash):
         nonlocal reference
-        executor.tune(eval_strike_tile=tile, fit_cashflows=cash, batch_schedule=schedule, batch_mib=mib)
+        executor.tune(eval_strike_tile=tile, fit_cashflows=cash, batch_schedule=schedule, batch_mib=mib,
+                      fit_schedule=fit)
         out = price(**sample)   # warm-up: code objects, buffers
         bits = _bits(out)
         if reference is None:
@@ -201,7 +202,7 @@ def tune_request(executor, price: Callable, *, group_offsets, initial_spots, str
     rec.choice = {"eval_strike_tile": tile, "fit_cashflows": cash}
     _CACHE.records[key] = rec
     _CACHE.tuned += 1
-    executor.tune(**rec.choice, batch_schedule=schedule, batch_mib=mib)
+    executor.tune(**rec.choice, batch_schedule=schedule, batch_mib=mib, fit_schedule=fit)
     return rec
 
 

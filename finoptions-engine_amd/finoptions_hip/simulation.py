@@ -91,7 +91,7 @@ class HipExecutor:
     precision: HipPrecision = HipPrecision.F64
     autotune: bool = False
     _handle: _Handle = field(init=False, repr=False, compare=False)
-    _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto", "serial", 0))
+    _tuning: tuple = field(init=False, repr=False, compare=False, default=(0, "auto", "serial", 0, "auto"))
 
     def __post_init__(self) -> None:
         if type(self.device) is not int or self.device < 0:
@@ -125,7 +125,7 @@ class HipExecutor:
                 for i in range(n.value)}
 
     def tune(self, *, eval_strike_tile: int = 0, fit_cashflows: str = "auto",
-             batch_schedule: str = "serial", batch_mib: int = 0) -> None:
+             batch_schedule: str = "serial", batch_mib: int = 0, fit_schedule: str = "auto") -> None:
         """Set launch parameters; results are bitwise invariant to them.
 
         ``eval_strike_tile``: strikes per evaluation launch column (0: by shape). ``fit_cashflows``
@@ -136,27 +136,36 @@ class HipExecutor:
         after another; "overlapped" runs one batch's backward induction beside the next batch's
         simulation and the previous batch's evaluation on a second stream, in two slots of half
         the serial batch memory. ``batch_mib``: paths and cashflows per batch slot in MiB (0: a
-        quarter of device memory, an eighth per slot when overlapped).
+        quarter of device memory, an eighth per slot when overlapped). ``fit_schedule`` (fused
+        methods): "per_node" launches one fit pass and one reduce per exercise row; "sweep" runs a
+        batch's whole backward induction in one persistent launch where every group holds one
+        strike (teams of blocks keep their paths on chip, so each spot row is read once, but meet
+        in device memory at every row); "auto" is the per-node launches, faster on the shapes
+        measured.
         """
         cash_codes = {"auto": 0, "value_columns": 1, "stopping_rows": 2}
         schedule_codes = {"serial": 0, "overlapped": 1}
+        fit_codes = {"auto": 0, "per_node": 1, "sweep": 2}
         if fit_cashflows not in cash_codes:
             raise InputError(f"fit_cashflows must be one of {sorted(cash_codes)}; observed {fit_cashflows!r}")
         if batch_schedule not in schedule_codes:
             raise InputError(
                 f"batch_schedule must be one of {sorted(schedule_codes)}; observed {batch_schedule!r}")
+        if fit_schedule not in fit_codes:
+            raise InputError(f"fit_schedule must be one of {sorted(fit_codes)}; observed {fit_schedule!r}")
         if type(batch_mib) is not int or not 0 <= batch_mib < 2**32:
             raise InputError(f"batch_mib must be a plain int in [0, 2^32); observed {batch_mib!r}")
         t = FohipTuning(int(eval_strike_tile), cash_codes[fit_cashflows], schedule_codes[batch_schedule],
-                        batch_mib)
+                        batch_mib, fit_codes[fit_schedule])
         if lib().fohip_set_tuning(self._handle.ptr, C.byref(t)) != 0:
             raise InputError(f"unsupported tuning eval_strike_tile={eval_strike_tile}")
-        object.__setattr__(self, "_tuning", (int(eval_strike_tile), fit_cashflows, batch_schedule, batch_mib))
+        object.__setattr__(self, "_tuning",
+                           (int(eval_strike_tile), fit_cashflows, batch_schedule, batch_mib, fit_schedule))
 
     @property
-    def tuning(self) -> tuple[int, str, str, int]:
+    def tuning(self) -> tuple[int, str, str, int, str]:
         """The launch parameters last set with :meth:`tune` (tile, cashflow encoding, batch
-        schedule, slot MiB)."""
+        schedule, slot MiB, fit schedule)."""
         return self._tuning
 
     def set_near_tie_window(self, window: float = 0.0) -> None:

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define FOHIP_ABI_VERSION 4
+#define FOHIP_ABI_VERSION 5
 
 /* ---- error codes (the numbering is shared with the CPU oracle) ---- */
 enum {
@@ -147,6 +147,7 @@ int fohip_abi_version(void);
  * trees are fixed by the path chunks, not by these values. Zero fields keep defaults. */
 enum { FOHIP_CASHFLOWS_AUTO = 0, FOHIP_CASHFLOWS_VALUE_COLUMNS = 1, FOHIP_CASHFLOWS_STOPPING_ROWS = 2 };
 enum { FOHIP_BATCHES_SERIAL = 0, FOHIP_BATCHES_OVERLAPPED = 1 };
+enum { FOHIP_FIT_AUTO = 0, FOHIP_FIT_PER_NODE = 1, FOHIP_FIT_SWEEP = 2 };
 typedef struct {
     uint32_t eval_strike_tile;      /* strikes per evaluation launch column (1..16; Greeks 1, 4, 6, 8) */
     /* Fused methods: how the backward induction keeps each (strike, path) cashflow. Value
@@ -167,6 +168,15 @@ typedef struct {
      * the serial schedule, an eighth per slot when overlapped). Smaller slots mean more, shorter
      * batches; results do not depend on the batch split. */
     uint32_t batch_mib;
+    /* Fused methods: how the backward induction is launched. Per-node launches run one fit pass
+     * and one reduce per exercise row over every group of a batch. The sweep, for requests whose
+     * groups all hold one strike, runs the whole backward induction of a batch in one persistent
+     * launch: each group's paths are split over a team of blocks that keep their spot rows and
+     * cashflows on chip from the last row to the first and meet at a device-memory counter per
+     * row, so each row is read from HBM once. Both give bitwise the same results. Auto is the
+     * per-node launches: on the shapes measured the sweep's per-row gathering of a team costs
+     * more than the HBM traffic it saves. */
+    uint32_t fit_schedule;
 } fohip_tuning;
 int fohip_set_tuning(fohip_executor *executor, const fohip_tuning *tuning);
 
@@ -403,6 +413,18 @@ int fohip_gbm_exp_probe(fohip_executor *executor, const double *x, size_t n, dou
 int fohip_stream_normals(fohip_executor *executor, uint64_t seed, uint32_t domain, uint32_t path0,
                          uint32_t paths, uint32_t count, double *out, uint32_t *draws,
                          fohip_error *err);
+
+/* The fit sweep's schedule for groups with the given fused-fit chunk counts (2048 training paths
+ * each) and backward steps, teams of blocks holding at most `chunks_per_block` chunks each:
+ * entries[8 e .. 8 e + 7] = (group, chunk_begin, chunk_end, members, team, leader, first partial
+ * record low, high word), in the order the launch's blocks take them by ticket (teams by
+ * decreasing cost, each team's members consecutively). Groups with no backward step get no
+ * entry. Host logic only (no device needed), for tests and diagnostics; no counterpart in the
+ * reference. FOHIP_E_INVALID_ARGUMENT when `entry_capacity` is too small (*entry_count then
+ * holds the size needed). */
+int fohip_sweep_schedule(const uint32_t *chunks, const uint32_t *fit_rows, uint32_t groups,
+                         uint32_t chunks_per_block, uint32_t *entries, uint64_t entry_capacity,
+                         uint64_t *entry_count, uint32_t *teams);
 
 /* ---- host-side validation of the financial inputs (no device needed) ---- */
 int fohip_validate_time_grid(const double *times, size_t nodes, fohip_error *err);

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(L.EXPORTED)
     for name in sorted(declared):
         assert hasattr(handle, name), name
-    assert handle.fohip_abi_version() == 4
+    assert handle.fohip_abi_version() == 5
 
 
 def test_library_has_gfx950_code_object():
