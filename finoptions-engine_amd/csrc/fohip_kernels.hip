@@ -1756,11 +1756,12 @@ __device__ __noinline__ void record_eval_errors(RowError *e, uint32_t row, uint3
 }
 
 // Waves per SIMD asked of an instantiation. The fused Greek evaluation over 6- and 8-strike tiles
-// of stored normals runs at 5 (96 VGPRs) rather than 4: the extra wave hides more of the decision
-// chain's latency. The 6-strike form, the default, fits without spills; the 8-strike form spills
-// its payoff tangents. Single-strike tiles that regenerate their paths run at 6 (80 VGPRs, which
-// their LDS allows) when they carry at most the two tangents of delta and vega; with all four
-// they would spill and keep the compiler's own choice, as every other form does.
+// of stored normals runs at 5 or more (at most 96 VGPRs) rather than 4: the extra wave hides more
+// of the decision chain's latency. With all four tangents the 6-strike form, the default, fits
+// without spills and the 8-strike form spills its payoff tangents; with the two of delta and vega
+// both fit in 80 VGPRs (6 to 7 waves). Single-strike tiles that regenerate their paths run at 6
+// (80 VGPRs, which their LDS allows) when they carry at most two tangents; with all four they
+// would spill and keep the compiler's own choice, as every other form does.
 template <bool FAST, int TS, int GREEKS, bool STORED>
 constexpr int eval_waves() {
     return (FAST && GREEKS && STORED && (TS == 6 || TS == 8)) ? 5 : ((TS == 1 && !STORED && GREEKS != 4) ? 6 : 1);
@@ -2599,12 +2600,17 @@ static hipError_t launch_eval_r(hipStream_t st, int gk, bool escrow, uint32_t ti
     if (tile == TSV && gk == GV && (znorm != nullptr) == ZV)                                                       \
         return escrow ? launch_eval_t<FAST, TSV, GV, true, ZV>(FOHIP_EVAL_ARGS)                                    \
                       : launch_eval_t<FAST, TSV, GV, false, ZV>(FOHIP_EVAL_ARGS);
-    // single-strike tiles (one group per contract) carry only the tangents the requested Greeks
-    // need; wider tiles carry all four
-    if (tile == 1 && gk == 2) {
-        FOHIP_EVAL_CASE(1, 2, false)
+    // every tile carries only the tangents the requested Greeks need: delta and vega take those of
+    // S0 and sigma, the others all four
+    FOHIP_EVAL_CASE(1, 2, false)
+    FOHIP_EVAL_CASE(4, 2, false)
+    FOHIP_EVAL_CASE(6, 2, false)
+    FOHIP_EVAL_CASE(8, 2, false)
+    if (FAST) {
+        FOHIP_EVAL_CASE(4, 2, true)
+        FOHIP_EVAL_CASE(6, 2, true)
+        FOHIP_EVAL_CASE(8, 2, true)
     }
-    if (gk == 2) gk = 4;
     FOHIP_EVAL_CASE(1, 0, false)
     FOHIP_EVAL_CASE(4, 0, false)
     FOHIP_EVAL_CASE(16, 0, false)
