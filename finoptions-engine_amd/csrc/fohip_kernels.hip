@@ -1678,12 +1678,21 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
     const double d_last = stored_at_origin ? t.disc0[g.row_base + last_row] : 1.0;
     double sum[1] = {0.0};
     uint32_t tie_count = 0;
+    // every slot's spot and cashflow are loaded before the first is used, so the loads are in
+    // flight together (the sums below still run over the slots in order)
+    Real s1v[kPathsPerThread], vv[kPathsPerThread];
+#pragma unroll
+    for (int k = 0; k < kPathsPerThread; ++k) {
+        const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
+        s1v[k] = p < n ? S_last[p] : Real(0);
+        vv[k] = (!STOPS && g.fit_rows != 0 && p < n) ? V[p] : Real(0);
+    }
 #pragma unroll
     for (int k = 0; k < kPathsPerThread; ++k) {
         const uint32_t p = item.y * kChunk + k * kBlock + threadIdx.x;
         if (p >= n) break;
         double v;
-        const double s1 = S_last[p];
+        const double s1 = s1v[k];
         if (g.fit_rows == 0) {
             v = s1 > K ? s1 - K : 0.0;
         } else {
@@ -1692,7 +1701,7 @@ __global__ __launch_bounds__(kBlock) void k_fit_final(DevTables t, FitParams fp,
                 const double held = paths[g.path_base + static_cast<uint64_t>(stop) * n + p];
                 v = (held > K ? held - K : 0.0) * t.disc0[g.row_base + stop];
             } else {
-                v = V[p];
+                v = vv[k];
             }
             if (apply && s1 > K) {
                 const double cont = kBulk ? continuation_fast(&m, s1) : predict_ref(&m, fp.degree, s1 / K);
