@@ -2165,60 +2165,86 @@ __global__ __launch_bounds__(kBlock) void k_eval_reduce(DevTables t, const uint3
 
 // Per strike row: the 12-column summary of try_from_parts (batch/pricing.rs:1130-1230) computed
 // from the fitted rules and stopping counts, plus the first evaluation row whose declared
-// missing-support rule met an active in-the-money path (policy.rs:684-703).
+// missing-support rule met an active in-the-money path (policy.rs:684-703). One wave per strike
+// row, its lanes striding over the rows: every column is a count, a sum of integers (the tie
+// counts), a maximum, a minimum or a first index, so the result does not depend on the order.
 __global__ __launch_bounds__(kBlock) void k_row_summary(DevTables t, const Model *__restrict__ models,
                                                         const uint32_t *__restrict__ counts,
                                                         const uint32_t *__restrict__ nosupport, uint32_t rows,
                                                         RowSummary *__restrict__ out) {
-    const uint32_t srow = blockIdx.x * blockDim.x + threadIdx.x;
-    if (srow >= rows) return;
+    const uint32_t srow = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (srow >= rows) return;   // wave-uniform
     const DevGroup &g = t.groups[t.strike_group[srow]];
     const uint32_t local = srow - g.strike_begin;
-    RowSummary rs;
-    rs.origin_fitted = 0;
-    rs.origin_exercised = 0;
-    rs.regression_nodes = 0;
-    rs.no_support_nodes = 0;
-    rs.insufficient_nodes = 0;
-    rs.min_itm = 0xFFFFFFFFu;
-    rs.max_condition = 0.0;
-    rs.early = 0;
-    rs.expiry = 0;
-    rs.nosupport_row = -1;
-    rs.nosupport_count = 0;
-    rs.origin_estimate = 0.0;
-    rs.ties_training = 0.0;
-    for (uint32_t r = 0; r + 1 < g.obs; ++r) {
-        const Model &m = models[g.model_base + local * g.obs + r];
-        rs.ties_training += m.ties;
+    const uint32_t model0 = g.model_base + local * g.obs, count0 = g.count_base + local * g.obs;
+    double ties = 0.0, max_condition = 0.0, estimate = 0.0;
+    uint32_t regression = 0, no_support = 0, insufficient = 0, min_itm = 0xFFFFFFFFu, early = 0;
+    uint32_t origin = 0, exercised = 0, first_nosupport = 0xFFFFFFFFu;
+    for (uint32_t r = lane; r + 1 < g.obs; r += 64) {
+        const Model &m = models[model0 + r];
+        ties += m.ties;
         switch (m.kind) {
             case kRuleOrigin:
-                rs.origin_fitted = 1;
-                rs.origin_exercised = m.exercise ? 1 : 0;
-                rs.origin_estimate = m.estimate;
+                origin = 1;
+                exercised = m.exercise ? 1 : 0;
+                estimate = m.estimate;
                 break;
             case kRuleNoSupport:
-                rs.no_support_nodes += 1;
+                no_support += 1;
                 break;
             case kRuleContinue:
-                rs.insufficient_nodes += 1;
+                insufficient += 1;
                 break;
             case kRuleRegression:
-                rs.regression_nodes += 1;
-                rs.min_itm = min(rs.min_itm, m.itm);
-                rs.max_condition = fmax(rs.max_condition, m.condition);
+                regression += 1;
+                min_itm = min(min_itm, m.itm);
+                max_condition = fmax(max_condition, m.condition);
                 break;
             default:
                 break;
         }
-        rs.early += counts[g.count_base + local * g.obs + r];
-        if (rs.nosupport_row < 0 && nosupport[g.count_base + local * g.obs + r] > 0) {
-            rs.nosupport_row = static_cast<int32_t>(r);
-            rs.nosupport_count = nosupport[g.count_base + local * g.obs + r];
-        }
+        early += counts[count0 + r];
+        if (first_nosupport == 0xFFFFFFFFu && nosupport[count0 + r] > 0) first_nosupport = r;
     }
-    if (rs.regression_nodes == 0) rs.min_itm = 0;
-    rs.expiry = counts[g.count_base + local * g.obs + g.obs - 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ties += __shfl_xor(ties, off, 64);
+        max_condition = fmax(max_condition, __shfl_xor(max_condition, off, 64));
+        regression += __shfl_xor(regression, off, 64);
+        no_support += __shfl_xor(no_support, off, 64);
+        insufficient += __shfl_xor(insufficient, off, 64);
+        min_itm = min(min_itm, static_cast<uint32_t>(__shfl_xor(min_itm, off, 64)));
+        early += __shfl_xor(early, off, 64);
+        first_nosupport = min(first_nosupport, static_cast<uint32_t>(__shfl_xor(first_nosupport, off, 64)));
+    }
+    // the origin rule (row 0) lies in one lane
+    const unsigned long long holder = __ballot(origin != 0u);
+    if (holder != 0ull) {
+        const int from = __ffsll(static_cast<long long>(holder)) - 1;
+        exercised = __shfl(exercised, from, 64);
+        estimate = __shfl(estimate, from, 64);
+    }
+    if (lane != 0) return;
+    RowSummary rs;
+    rs.origin_fitted = holder != 0ull ? 1 : 0;
+    rs.origin_exercised = holder != 0ull && exercised ? 1 : 0;
+    rs.origin_estimate = holder != 0ull ? estimate : 0.0;
+    rs.regression_nodes = regression;
+    rs.no_support_nodes = no_support;
+    rs.insufficient_nodes = insufficient;
+    rs.min_itm = regression == 0 ? 0u : min_itm;
+    rs.max_condition = max_condition;
+    rs.early = early;
+    rs.expiry = counts[count0 + g.obs - 1];
+    rs.ties_training = ties;
+    if (first_nosupport != 0xFFFFFFFFu) {
+        rs.nosupport_row = static_cast<int32_t>(first_nosupport);
+        rs.nosupport_count = nosupport[count0 + first_nosupport];
+    } else {
+        rs.nosupport_row = -1;
+        rs.nosupport_count = 0;
+    }
     out[srow] = rs;
 }
 
@@ -2675,7 +2701,7 @@ hipError_t launch_eval_normals(hipStream_t st, const DevTables &t, const uint2 *
 hipError_t launch_row_summary(hipStream_t st, const DevTables &t, const Model *models, const uint32_t *counts,
                               const uint32_t *nosupport, uint32_t rows, RowSummary *out) {
     if (rows == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_row_summary, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, models, counts,
+    hipLaunchKernelGGL(k_row_summary, dim3((rows + kWaves - 1) / kWaves), dim3(kBlock), 0, st, t, models, counts,
                        nosupport, rows, out);
     return hipGetLastError();
 }
