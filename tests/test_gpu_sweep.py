@@ -139,3 +139,20 @@ def test_sweep_failure_equals_per_node(small_batch):
     ok = _price(fh, ex, small_batch)
     ex.tune(fit_schedule="sweep")
     _assert_bitwise(ok, _price(fh, ex, small_batch), "after failures")
+
+
+def test_sweep_equals_per_node_at_ragged_batch_team_sizes():
+    """600 contracts drawn like the ragged batch's (16 384 to 262 144 paths: teams of 3 to 43
+    blocks, 4 to 252 backward steps), with delta and vega."""
+    import finoptions_hip as fh
+    from finoptions_hip.workloads import ragged_batch, ragged_batch_request
+
+    w = ragged_batch(600, seed=31)
+    ex = fh.HipExecutor(device=0, precision=fh.HipPrecision.FUSED_F64)
+    greeks = (fh.Greek.DELTA, fh.Greek.VEGA)
+    out = []
+    for schedule in ("per_node", "sweep"):
+        ex.tune(fit_schedule=schedule)
+        out.append(fh.price_lsm_call_batch(executor=ex, greeks=greeks, **ragged_batch_request(w, seed=31)))
+    print(f"{len(w['paths'])} contracts, up to {int(w['paths'].max())} paths", flush=True)
+    _assert_bitwise(out[0], out[1], "ragged team sizes")
