@@ -64,16 +64,20 @@ def _cash(cols: dict) -> dict:
 @pytest.mark.parametrize("executor", ["hip_f64", "hip_fused"])
 def test_delta_and_vega_alone_equal_those_of_all_four_greeks(executor, request):
     """A request for delta and vega alone carries two tangents per path instead of four; the two
-    Greeks must be bitwise those of a request for all four: on a single-strike group (the
-    two-tangent form) with cash dividends, on one with a continuous yield, on a multi-strike
-    state, and through the single-call entry point."""
+    Greeks must be bitwise those of a request for all four: on a single-strike group with cash
+    dividends, on one with a continuous yield, on one 6-strike tile, on three tiles over stored
+    normals (with a continuous yield and with escrowed cash dividends), and through the
+    single-call entry point."""
     import finoptions_hip as fh
     from conftest import c1_inputs
 
     ex = request.getfixturevalue(executor)
     both = (fh.Greek.DELTA, fh.Greek.VEGA)
     all4 = tuple(getattr(fh.Greek, g) for g in ALL_GREEKS)
-    for cols in (_cash(c2_columns(strikes=1)), c2_columns(strikes=1), c2_columns(strikes=6)):
+    escrowed = dict(_cash(c2_columns(strikes=16)), escrowed=1)
+    for cols in (_cash(c2_columns(strikes=1)), c2_columns(strikes=1), c2_columns(strikes=6),
+                 c2_columns(strikes=16), escrowed):
+        # 16 strikes: three 6-strike tiles over stored evaluation normals
         greeks = all4 if "yields" in cols else (*both, fh.Greek.RHO)
         two = _price(fh, ex, cols, 16_384, both, intervals=12)
         four = _price(fh, ex, cols, 16_384, greeks, intervals=12)
