@@ -257,6 +257,67 @@ def _c2_fixture() -> dict | None:
                 else (np.array(v) if isinstance(v, list) else v)) for k, v in raw.items()}
 
 
+def c1_leg(fh, fast, plain, threads: int) -> dict:
+    """The single scheduled call (S0 = K = 100, times 0 / 0.5 / 1, cash 2.0 at node 1, exercise
+    node 1, 65 536 + 65 536 paths): the fused call with all four Greeks, the plain FP64 call with
+    4 bump pricings of delta and vega, and the oracle's call on 1 and `threads` threads."""
+    schedule = fh.EventSchedule(times=(0.0, 0.5, 1.0), dividends=(fh.CashDividend(node=1, amount=2.0),),
+                                exercise_nodes=(1,), expiry_node=2)
+    curve = fh.LogDiscountCurve(times=schedule.times, log_discounts=(0.0, -0.01, -0.025))
+    method = fh.Lsm(training_paths=65_536, evaluation_paths=65_536,
+                    training_stream=fh.RandomStream(seed=7, domain=1),
+                    evaluation_stream=fh.RandomStream(seed=7, domain=2),
+                    insufficient_exercise_support=fh.InsufficientExerciseSupportPolicy.FAIL)
+
+    def call(executor, spot=100.0, vol=0.20, greeks=()):
+        return fh.price_lsm_call(initial_spot=spot, strike=100.0, schedule=schedule, curve=curve,
+                                 dividend_yield_curve=fh.CumulativeDividendYieldCurve.zero(times=schedule.times),
+                                 volatility=fh.ConstantVolatility(vol), dividend_policy=fh.DividendPolicy.LIQUIDATOR,
+                                 executor=executor, method=method, greeks=greeks)
+
+    all4 = (fh.Greek.DELTA, fh.Greek.VEGA, fh.Greek.RHO, fh.Greek.DIVIDEND_YIELD)
+    for _ in range(5):
+        call(fast, greeks=all4)
+    reps = 50
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = call(fast, greeks=all4)
+    ms_fast = 1e3 * (time.perf_counter() - t0) / reps
+    out = {"workload": "one call: S0 = K = 100, times (0, 0.5, 1), cash 2.0 at node 1 (liquidator), "
+                       "exercise node 1, logD (0, -0.01, -0.025), sigma 0.2, 65536 + 65536 paths, "
+                       "streams (7, 1) / (7, 2); 262144 path-steps",
+           "fast_ms_with_4_greeks": round(ms_fast, 3), "price": res.price}
+    if plain is not None:
+        for _ in range(3):
+            call(plain)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            call(plain)
+            for kw in ({"spot": 100 * (1 + BUMP)}, {"spot": 100 * (1 - BUMP)},
+                       {"vol": 0.20 * (1 + BUMP)}, {"vol": 0.20 * (1 - BUMP)}):
+                call(plain, **kw)
+        ms_plain = 1e3 * (time.perf_counter() - t0) / 10
+        out["plain_ms_price_bump_greeks"] = round(ms_plain, 3)
+        out["fast_speedup_vs_plain_price_bump_greeks"] = round(ms_plain / ms_fast, 2)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    plan = pyoracle.Plan(times=[0.0, 0.5, 1.0], log_discounts=[0.0, -0.01, -0.025], cum_yields=[0.0, 0.0, 0.0],
+                         volatility=0.20, dividend_policy=0, dividend_nodes=[1], dividend_amounts=[2.0],
+                         exercise_nodes=[1], expiry_node=2)
+    for th in sorted({1, threads}):
+        pyoracle.set_threads(th)
+        best = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ref = pyoracle.price_lsm_call(plan, 100.0, 100.0, 65_536, 65_536, 7, 1, 7, 2)
+            best = min(best, time.perf_counter() - t0)
+        out[f"cpu_oracle_ms_{th}_thread{'s' if th > 1 else ''}"] = round(1e3 * best, 2)
+    out["oracle_price"] = ref["price"]
+    out["fast_price_rel_err_vs_oracle"] = abs(res.price - ref["price"]) / ref["price"]
+    return out
+
+
 def copy_calibration() -> int:
     """Known-size traffic for the HBM counters: each launch reads and writes exactly 256 MiB."""
     import torch
@@ -442,6 +503,9 @@ def main() -> int:
                         "32768 paths per state",
             "ms_per_step": round(ms_c5, 1), "path_steps_per_s": steps5 / (ms_c5 / 1e3),
             "strike_rows_per_s": C5_GROUPS * C5_STRIKES / (ms_c5 / 1e3)}
+        # ---- the single scheduled call
+        c1_threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        extras["c1_single_call"] = c1_leg(fh, fast, None if args.skip_plain else plain, c1_threads)
 
     cpu = None
     if not args.skip_cpu and plain_res is not None:
